@@ -1,0 +1,139 @@
+/*
+ * tkmi.h — C ABI of the MI355X (gfx950) backend for TensorKit's symmetric-tensor
+ * hot path: grouped permute/braid of block-sparse TensorMaps and grouped
+ * per-coupled-sector GEMM.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every function returns
+ * TKMI_OK (0) on success; on failure the message is available from
+ * tkmi_last_error() (thread-local) and nothing has been launched.
+ *
+ * Conventions
+ *  - All indices / offsets are 0-based element offsets into flat data vectors.
+ *  - A "strided structure" is the reference's StridedStructure
+ *    (sizes[N], strides[N], offset) (src/spaces/structure.jl:2), stored as
+ *    2N+1 consecutive int64 values.
+ *  - `perm` is the linearised index permutation p = (p1..., p2...) of
+ *    TO.tensoradd!: dst index k takes source index perm[k] (0-based).
+ *  - Streams are hipStream_t passed as void*; NULL is the default stream.
+ *  - Plans are immutable after creation and may be used concurrently from
+ *    several host threads on different streams.
+ */
+#ifndef TKMI_H
+#define TKMI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tkmi_status {
+    TKMI_OK = 0,
+    TKMI_ERR_ARGUMENT = 1,       /* ArgumentError in the reference          */
+    TKMI_ERR_SPACE_MISMATCH = 2, /* SpaceMismatch (sizes/lengths disagree)   */
+    TKMI_ERR_INDEX = 3,          /* IndexError (bad permutation)            */
+    TKMI_ERR_HIP = 4,            /* HIP runtime failure                     */
+    TKMI_ERR_ALLOC = 5,          /* device allocation failed                */
+    TKMI_ERR_UNSUPPORTED = 6     /* dtype / rank not supported              */
+};
+
+enum tkmi_dtype { TKMI_F64 = 0, TKMI_F32 = 1 };
+
+/* Arithmetic forms of the grouped GEMM. */
+enum tkmi_precision {
+    TKMI_PREC_F64 = 0,     /* FP64 MFMA, FP64 accumulate (drop-in path)               */
+    TKMI_PREC_F32 = 1,     /* FP32 MFMA on down-converted inputs, FP32 accumulate       */
+    TKMI_PREC_BF16X3 = 2   /* 3-way BF16 split products on MFMA, FP32 panels, FP64 fold */
+};
+
+typedef struct tkmi_transform_plan_s *tkmi_transform_plan;
+typedef struct tkmi_gemm_plan_s *tkmi_gemm_plan;
+typedef struct tkmi_graph_s *tkmi_graph;
+
+/* ---- runtime ------------------------------------------------------------ */
+int tkmi_init(int device);
+const char *tkmi_last_error(void);
+int tkmi_version(void);
+int tkmi_alloc(void **ptr, size_t bytes);
+int tkmi_free(void *ptr);
+int tkmi_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int tkmi_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int tkmi_stream_synchronize(void *stream);
+
+/* ---- permute / braid (add_transform_kernel!) ------------------------------
+ * Abelian tree transformer upload.  Replaces the per-entry loop of
+ *   add_transform_kernel!(data_dst, data_src, p, ::AbelianTreeTransformer, α, β, …)
+ *   (src/tensors/indexmanipulations.jl:658-669)
+ * over the table built by AbelianTreeTransformer
+ *   (src/tensors/treetransformers.jl:16-42): one entry per source tree pair,
+ *   (coeff, dst StridedStructure, src StridedStructure).
+ * coeffs[nentries]; dst_structs / src_structs: nentries × (2N+1) int64.      */
+int tkmi_abelian_plan_create(tkmi_transform_plan *plan, int32_t numind, const int32_t *perm,
+                             int64_t nentries, const double *coeffs,
+                             const int64_t *dst_structs, const int64_t *src_structs,
+                             int64_t dst_len, int64_t src_len);
+
+/* Generic tree transformer upload.  Replaces
+ *   add_transform_kernel!(data_dst, data_src, p, ::GenericTreeTransformer, α, β, …)
+ *   (src/tensors/indexmanipulations.jl:670-726)
+ * over the table built by GenericTreeTransformer (treetransformers.jl:54-120):
+ * per fusion block b: a (rows_b × cols_b) recoupling matrix U_b (column-major,
+ * concatenated in `U`), the common sub-block sizes (N int64, sz_dst / sz_src)
+ * and per tree (strides[N], offset) (N+1 int64, concatenated per block).
+ * The kernel computes dst_j = β dst_j + α Σ_i U_b[j,i] permute(src_i, perm)
+ * for every block, i.e. pack → src·Uᵀ → unpack fused in one launch.         */
+int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const int32_t *perm,
+                             int64_t nblocks, const int64_t *rows, const int64_t *cols,
+                             const double *U, const int64_t *sz_dst, const int64_t *sz_src,
+                             const int64_t *strides_offsets_dst,
+                             const int64_t *strides_offsets_src,
+                             int64_t dst_len, int64_t src_len);
+
+/* dst = β·dst + α·transform(src).  dtype: TKMI_F64 or TKMI_F32 (both buffers).
+ * conj is accepted for API symmetry with TO.tensoradd! and is a no-op on
+ * real data.  src and dst must not overlap.                                  */
+int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
+                        double alpha, double beta, void *stream);
+
+/* Same transform with the adjoint table: src += conj(α)·transformᵀ(dst-shaped
+ * cotangent), i.e. the pullback of tkmi_transform_exec (inverse permutation,
+ * transposed recoupling matrices), used by the permute/braid rrules
+ * (src/pullbacks/indexmanipulations.jl:1-31).                               */
+int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const void *ddst,
+                                void *dsrc, double alpha, double beta, void *stream);
+
+int tkmi_transform_plan_info(tkmi_transform_plan plan, int64_t *ntiles, int64_t *nentries,
+                             int64_t *elements);
+int tkmi_transform_plan_destroy(tkmi_transform_plan plan);
+
+/* ---- grouped GEMM (mul! per coupled sector) --------------------------------
+ * Replaces the merge-walk of LinearAlgebra.mul!(tC, tA, tB, α, β)
+ * (src/tensors/linalg.jl:330-370): one C_g = α·op(A_g)·op(B_g) + β·C_g per
+ * group g, all groups in one launch.  Blocks are column-major
+ * (reshape(view(data, r), d₁, d₂), src/tensors/tensor.jl:474-479).
+ * op(A_g) is m×k: transa=0 → A[i + l·lda]; transa=1 → A[l + i·lda]
+ * (the adjoint block of an AdjointTensorMap, src/tensors/adjoint.jl:30).
+ * op(B_g) is k×n likewise.  k = 0 gives C_g = β·C_g (the rmul! branch).     */
+int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *m,
+                          const int64_t *n, const int64_t *k, const int64_t *a_off,
+                          const int64_t *lda, const int32_t *transa, const int64_t *b_off,
+                          const int64_t *ldb, const int32_t *transb, const int64_t *c_off,
+                          const int64_t *ldc);
+/* tile: 0 = automatic (plan's autotuned choice), else 64 or 128 */
+int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta,
+                   const double *A, const double *B, double *C, void *stream);
+int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile);
+int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops);
+int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
+
+/* ---- hipGraph capture of a chain of launches (one sweep step) ------------ */
+int tkmi_graph_begin(void *stream);
+int tkmi_graph_end(void *stream, tkmi_graph *graph);
+int tkmi_graph_launch(tkmi_graph graph, void *stream);
+int tkmi_graph_destroy(tkmi_graph graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TKMI_H */

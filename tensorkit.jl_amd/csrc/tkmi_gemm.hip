@@ -1,0 +1,523 @@
+// Grouped per-coupled-sector GEMM on CDNA4 matrix cores (K3 FP64, K4 FP32, K5 3×BF16).
+//
+// One launch computes C_g = α·op(A_g)·op(B_g) + β·C_g for every coupled sector g of a
+// contraction (the merge-walk of mul!, src/tensors/linalg.jl:330-370).  Tiles of all
+// groups are numbered consecutively; workgroup → tile goes through an XCD-aware remap
+// (consecutive tiles of one group land on the same XCD, so they share A panels in that
+// XCD's L2) and a binary search over the groups' tile_start prefix sums.
+//
+// Workgroup = 256 threads = 4 wave64s in a 2×2 arrangement; a BM×BN output tile is
+// computed as (BM/2)×(BN/2) per wave from 16×16 MFMA fragments.  K advances in BK
+// slabs staged through double-buffered LDS; the next slab's global loads are issued
+// before the current slab's MFMAs.
+//
+// MFMA lane layouts used (wave64, i = row, j = col, l = lane):
+//   v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32:
+//       A[i][k]: i = l%16, k = l/16;  B[k][j]: j = l%16, k = l/16
+//       D[i][j], 4 values v per lane:  i = 4*(l/16) + v, j = l%16
+//   v_mfma_f32_16x16x32_bf16: A[i][k], 8 values e per lane: i = l%16, k = 8*(l/16) + e
+//       (B likewise with j), D as above.
+#include "tkmi_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int NT = 256;
+
+struct GGroup {
+    int64_t m, n, k;
+    int64_t a_off, lda, b_off, ldb, c_off, ldc;
+    int32_t ta, tb;
+    int64_t tiles_m, tiles_n, tile_start;
+};
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t ntiles) {
+    // hardware dispatches workgroup ids round-robin over 8 XCDs; give each XCD a
+    // contiguous range of tiles instead.
+    constexpr int64_t NX = 8;
+    const int64_t q = ntiles / NX, r = ntiles % NX;
+    const int64_t x = bid % NX, pos = bid / NX;
+    // XCD x owns tiles [start_x, start_x + cnt_x); cnt_x = q + (x < r)
+    const int64_t start = x * q + min(x, r);
+    const int64_t cnt = q + (x < r ? 1 : 0);
+    if (pos < cnt) return start + pos;
+    return bid;  // unreachable when gridDim.x == ntiles
+}
+
+__device__ inline int find_group(const GGroup *__restrict__ g, int64_t ng, int64_t t) {
+    int64_t lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (g[mid].tile_start <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return (int)lo;
+}
+
+// --------------------------------------------------------------------------------
+// FP64 / FP32 kernel (same data flow; T_LDS = compute type of the MFMA)
+// --------------------------------------------------------------------------------
+template <typename TC>
+struct MfmaK4;
+template <>
+struct MfmaK4<double> {
+    typedef d4_t acc_t;
+    __device__ static inline acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MfmaK4<float> {
+    typedef f4_t acc_t;
+    __device__ static inline acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename TC, int BM, int BN>
+__global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                     int64_t ntiles, const double *__restrict__ A,
+                                                     const double *__restrict__ Bm, double *__restrict__ C,
+                                                     double alpha, double beta) {
+    constexpr int BK = 16;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int AE = BM * BK / NT;  // A elements per thread per slab
+    constexpr int BE = BN * BK / NT;
+    constexpr int PAD = 0;
+    __shared__ TC As[2][BK][BM + PAD];
+    __shared__ TC Bs[2][BK][BN + PAD];
+    using acc_t = typename MfmaK4<TC>::acc_t;
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tm = lt / G.tiles_n, tn = lt % G.tiles_n;
+    const int64_t m = G.m, n = G.n, K = G.k;
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const double *Ag = A + G.a_off;
+    const double *Bg = Bm + G.b_off;
+    const int64_t lda = G.lda, ldb = G.ldb;
+    const int ta = G.ta, tb = G.tb;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    double ra[AE], rb[BE];
+    // element e of this thread in the A slab: (i, l) with coalesced global order
+    auto a_coord = [&](int e, int &i, int &l) {
+        const int idx = e * NT + tid;
+        if (!ta) { i = idx % BM; l = idx / BM; }   // column-major: i contiguous
+        else     { l = idx % BK; i = idx / BK; }   // transposed: l contiguous
+    };
+    auto b_coord = [&](int e, int &l, int &j) {
+        const int idx = e * NT + tid;
+        if (!tb) { l = idx % BK; j = idx / BK; }   // column-major k×n: l contiguous
+        else     { j = idx % BN; l = idx / BN; }   // transposed: j contiguous
+    };
+    auto load_slab = [&](int64_t k0) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, l;
+            a_coord(e, i, l);
+            const int64_t gi = row0 + i, gl = k0 + l;
+            ra[e] = (gi < m && gl < K) ? (ta ? Ag[gl + gi * lda] : Ag[gi + gl * lda]) : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int l, j;
+            b_coord(e, l, j);
+            const int64_t gj = col0 + j, gl = k0 + l;
+            rb[e] = (gj < n && gl < K) ? (tb ? Bg[gj + gl * ldb] : Bg[gl + gj * ldb]) : 0.0;
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, l;
+            a_coord(e, i, l);
+            As[buf][l][i] = (TC)ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int l, j;
+            b_coord(e, l, j);
+            Bs[buf][l][j] = (TC)rb[e];
+        }
+    };
+
+    const int64_t nk = (K + BK - 1) / BK;
+    if (nk > 0) {
+        load_slab(0);
+        store_slab(0);
+    }
+    __syncthreads();
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int buf = (int)(kt & 1);
+        if (kt + 1 < nk) load_slab((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            TC af[FM], bfr[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) bfr[j] = Bs[buf][kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i], bfr[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue
+    double *Cg = C + G.c_off;
+    const int64_t ldc = G.ldc;
+    const bool has_beta = (beta != 0.0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
+            if (col >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = row0 + wm * WM + i * 16 + 4 * (lane >> 4) + v;
+                if (row < m) {
+                    double r = alpha * (double)acc[i][j][v];
+                    double *p = Cg + row + col * ldc;
+                    *p = has_beta ? r + beta * *p : r;
+                }
+            }
+        }
+}
+
+// --------------------------------------------------------------------------------
+// 3×BF16 split-product kernel: x = x0 + x1 + x2 (bf16 each), products
+// x0y0 (high), x0y1 + x1y0 + x0y2 + x1y1 + x2y0 (low) on v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation per BK=32 slab, high part folded into FP64 after every slab.
+// --------------------------------------------------------------------------------
+__device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
+
+template <int BM, int BN>
+__global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                         int64_t ntiles, const double *__restrict__ A,
+                                                         const double *__restrict__ Bm, double *__restrict__ C,
+                                                         double alpha, double beta) {
+    constexpr int BK = 32;
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int AE = BM * BK / NT;
+    constexpr int BE = BN * BK / NT;
+    constexpr int KP = BK + 8;  // row pitch (bf16 elements), keeps 16-byte alignment
+    // split parts: [part][row][k]
+    __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][KP];
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tm = lt / G.tiles_n, tn = lt % G.tiles_n;
+    const int64_t m = G.m, n = G.n, K = G.k;
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const double *Ag = A + G.a_off;
+    const double *Bg = Bm + G.b_off;
+    const int64_t lda = G.lda, ldb = G.ldb;
+    const int ta = G.ta, tb = G.tb;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    d4_t acc64[FM][FN];
+    f4_t lo[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            acc64[i][j] = d4_t{0, 0, 0, 0};
+            lo[i][j] = f4_t{0, 0, 0, 0};
+        }
+
+    double ra[AE], rb[BE];
+    auto a_coord = [&](int e, int &i, int &l) {
+        const int idx = e * NT + tid;
+        if (!ta) { i = idx % BM; l = idx / BM; }
+        else     { l = idx % BK; i = idx / BK; }
+    };
+    auto b_coord = [&](int e, int &l, int &j) {
+        const int idx = e * NT + tid;
+        if (!tb) { l = idx % BK; j = idx / BK; }
+        else     { j = idx % BN; l = idx / BN; }
+    };
+    auto load_slab = [&](int64_t k0) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, l;
+            a_coord(e, i, l);
+            const int64_t gi = row0 + i, gl = k0 + l;
+            ra[e] = (gi < m && gl < K) ? (ta ? Ag[gl + gi * lda] : Ag[gi + gl * lda]) : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int l, j;
+            b_coord(e, l, j);
+            const int64_t gj = col0 + j, gl = k0 + l;
+            rb[e] = (gj < n && gl < K) ? (tb ? Bg[gj + gl * ldb] : Bg[gl + gj * ldb]) : 0.0;
+        }
+    };
+    auto split3 = [](double x, __bf16 &x0, __bf16 &x1, __bf16 &x2) {
+        x0 = to_bf16((float)x);
+        const double r1 = x - (double)(float)x0;
+        x1 = to_bf16((float)r1);
+        const double r2 = r1 - (double)(float)x1;
+        x2 = to_bf16((float)r2);
+    };
+    auto store_slab = [&]() {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, l;
+            a_coord(e, i, l);
+            __bf16 x0, x1, x2;
+            split3(ra[e], x0, x1, x2);
+            As[0][i][l] = x0;
+            As[1][i][l] = x1;
+            As[2][i][l] = x2;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int l, j;
+            b_coord(e, l, j);
+            __bf16 x0, x1, x2;
+            split3(rb[e], x0, x1, x2);
+            Bs[0][j][l] = x0;
+            Bs[1][j][l] = x1;
+            Bs[2][j][l] = x2;
+        }
+    };
+
+    const int64_t nk = (K + BK - 1) / BK;
+    if (nk > 0) load_slab(0);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        __syncthreads();  // previous slab fully consumed
+        store_slab();
+        __syncthreads();
+        if (kt + 1 < nk) load_slab((kt + 1) * BK);
+        const int kq = 8 * (lane >> 4);
+        bf8_t a[3][FM], b[3][FN];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+                a[p][i] = *reinterpret_cast<const bf8_t *>(&As[p][wm * WM + i * 16 + (lane & 15)][kq]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                b[p][j] = *reinterpret_cast<const bf8_t *>(&Bs[p][wn * WN + j * 16 + (lane & 15)][kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                f4_t h = f4_t{0, 0, 0, 0};
+                h = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], h, 0, 0, 0);
+                f4_t l = lo[i][j];
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], l, 0, 0, 0);
+                lo[i][j] = l;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)h[v];
+            }
+    }
+
+    double *Cg = C + G.c_off;
+    const int64_t ldc = G.ldc;
+    const bool has_beta = (beta != 0.0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
+            if (col >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = row0 + wm * WM + i * 16 + 4 * (lane >> 4) + v;
+                if (row < m) {
+                    double r = alpha * (acc64[i][j][v] + (double)lo[i][j][v]);
+                    double *p = Cg + row + col * ldc;
+                    *p = has_beta ? r + beta * *p : r;
+                }
+            }
+        }
+}
+
+struct TileSet {
+    int tile = 0;  // 64 or 128
+    std::vector<GGroup> groups;
+    int64_t ntiles = 0;
+    GGroup *d_groups = nullptr;
+};
+
+}  // namespace
+
+struct tkmi_gemm_plan_s {
+    std::vector<GGroup> base;  // groups without tiling
+    double flops = 0;
+    TileSet t64, t128;
+    int choice[3] = {0, 0, 0};  // per precision: 0 auto
+};
+
+namespace {
+
+int build_tiles(tkmi_gemm_plan_s *p, TileSet &ts, int tile) {
+    ts.tile = tile;
+    ts.groups.clear();
+    int64_t start = 0;
+    for (auto g : p->base) {
+        g.tiles_m = (g.m + tile - 1) / tile;
+        g.tiles_n = (g.n + tile - 1) / tile;
+        if (g.m == 0 || g.n == 0) continue;
+        g.tile_start = start;
+        start += g.tiles_m * g.tiles_n;
+        ts.groups.push_back(g);
+    }
+    ts.ntiles = start;
+    if (ts.groups.empty()) return TKMI_OK;
+    return tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
+}
+
+int auto_tile(const tkmi_gemm_plan_s *p) {
+    // prefer 128×128 tiles when that still gives at least two tiles per CU
+    return (p->t128.ntiles >= 512) ? 128 : 64;
+}
+
+template <int BM>
+int launch(const TileSet &ts, int prec, const double *A, const double *B, double *C, double alpha, double beta,
+           hipStream_t s) {
+    if (ts.ntiles == 0) return TKMI_OK;
+    dim3 grid((unsigned)ts.ntiles), block(NT);
+    const int64_t ng = (int64_t)ts.groups.size();
+    if (prec == TKMI_PREC_F64)
+        hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BM>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B,
+                           C, alpha, beta);
+    else if (prec == TKMI_PREC_F32)
+        hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BM>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B,
+                           C, alpha, beta);
+    else
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BM>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B, C,
+                           alpha, beta);
+    TKMI_HIP(hipGetLastError());
+    return TKMI_OK;
+}
+
+void free_gemm(tkmi_gemm_plan_s *p) {
+    if (!p) return;
+    (void)hipFree(p->t64.d_groups);
+    (void)hipFree(p->t128.d_groups);
+    delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *m, const int64_t *n,
+                          const int64_t *k, const int64_t *a_off, const int64_t *lda, const int32_t *transa,
+                          const int64_t *b_off, const int64_t *ldb, const int32_t *transb, const int64_t *c_off,
+                          const int64_t *ldc) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_gemm_plan_create: null plan pointer");
+    *plan = nullptr;
+    if (ngroups < 0) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_gemm_plan_create: negative group count");
+    auto *p = new tkmi_gemm_plan_s();
+    for (int64_t g = 0; g < ngroups; ++g) {
+        GGroup G;
+        std::memset(&G, 0, sizeof(G));
+        G.m = m[g];
+        G.n = n[g];
+        G.k = k[g];
+        G.a_off = a_off[g];
+        G.lda = lda[g];
+        G.ta = transa[g] ? 1 : 0;
+        G.b_off = b_off[g];
+        G.ldb = ldb[g];
+        G.tb = transb[g] ? 1 : 0;
+        G.c_off = c_off[g];
+        G.ldc = ldc[g];
+        if (G.m < 0 || G.n < 0 || G.k < 0 || G.ldc < std::max<int64_t>(1, G.m)) {
+            free_gemm(p);
+            TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_gemm_plan_create: inconsistent group dimensions");
+        }
+        if (G.k > 0) {
+            const int64_t need_a = G.ta ? G.k : G.m, need_b = G.tb ? G.n : G.k;
+            if (G.lda < std::max<int64_t>(1, need_a) || G.ldb < std::max<int64_t>(1, need_b)) {
+                free_gemm(p);
+                TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_gemm_plan_create: leading dimension too small");
+            }
+        }
+        p->flops += 2.0 * (double)G.m * (double)G.n * (double)G.k;
+        p->base.push_back(G);
+    }
+    // largest groups first: their tiles start earliest
+    std::stable_sort(p->base.begin(), p->base.end(), [](const GGroup &a, const GGroup &b) {
+        return a.m * a.n * std::max<int64_t>(a.k, 1) > b.m * b.n * std::max<int64_t>(b.k, 1);
+    });
+    int rc = build_tiles(p, p->t64, 64);
+    if (!rc) rc = build_tiles(p, p->t128, 128);
+    if (rc) {
+        free_gemm(p);
+        return rc;
+    }
+    *plan = p;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
+    if (tile != 0 && tile != 64 && tile != 128) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: tile must be 0, 64 or 128");
+    plan->choice[precision] = tile;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta, const double *A,
+                   const double *B, double *C, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
+    int tile = plan->choice[precision] ? plan->choice[precision] : auto_tile(plan);
+    hipStream_t s = tkmi::as_stream(stream);
+    if (tile == 128) return launch<128>(plan->t128, precision, A, B, C, alpha, beta, s);
+    return launch<64>(plan->t64, precision, A, B, C, alpha, beta, s);
+}
+
+int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (ntiles) *ntiles = plan->t64.ntiles;
+    if (flops) *flops = plan->flops;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan) {
+    free_gemm(plan);
+    return TKMI_OK;
+}
+
+}  // extern "C"

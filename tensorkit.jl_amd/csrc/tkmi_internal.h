@@ -1,0 +1,36 @@
+// Internal helpers shared by the tkmi translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/tkmi.h"
+
+namespace tkmi {
+
+void set_error(const std::string &msg);
+
+#define TKMI_FAIL(code, msg)          \
+    do {                              \
+        ::tkmi::set_error(msg);       \
+        return (code);                \
+    } while (0)
+
+#define TKMI_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            ::tkmi::set_error(std::string(#call) + ": " + hipGetErrorString(_e));        \
+            return TKMI_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+
+// Upload a host array into a fresh device buffer (synchronous; plans only).
+int upload(void **dptr, const void *host, size_t bytes);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace tkmi

@@ -1,0 +1,29 @@
+"""tkmi — MI355X (gfx950) backend for TensorKit's symmetric-tensor hot path.
+
+Host-side mirror of TensorKit's permute/braid and contraction API.  Spaces,
+fusion trees and transform tables are built and cached on the host; all
+tensor arithmetic runs in libtkmi.so (hand-written HIP kernels) through its
+C ABI (include/tkmi.h).
+"""
+from .sectors import (U1Irrep, SU2Irrep, FermionParity, ProductSector, fZ2U1, Sector,
+                      UNIQUE, SIMPLE, BOSONIC, FERMIONIC)
+from .spaces import (GradedSpace, ProductSpace, HomSpace, U1Space, SU2Space, Vect, otimes,
+                     SpaceMismatch, SectorMismatch)
+from .fusiontrees import FusionTree, FusionTreeBlock, fusiontrees
+from .treetransformers import treebraider, empty_globalcaches, global_cache_info
+from . import _lib
+
+__all__ = [
+    "U1Irrep", "SU2Irrep", "FermionParity", "ProductSector", "fZ2U1", "Sector",
+    "GradedSpace", "ProductSpace", "HomSpace", "U1Space", "SU2Space", "Vect", "otimes",
+    "SpaceMismatch", "SectorMismatch", "FusionTree", "FusionTreeBlock", "fusiontrees",
+    "treebraider", "empty_globalcaches", "global_cache_info",
+]
+
+
+def __getattr__(name):
+    # device-side API (imports torch lazily so host-only use stays light)
+    if name in ("tensors",):
+        from . import tensors
+        return tensors
+    raise AttributeError(name)

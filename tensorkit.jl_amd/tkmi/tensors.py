@@ -1,0 +1,572 @@
+"""Device TensorMap and the index-manipulation / contraction entry points.
+
+Host-side mirror of TensorKit's L4 layer for the hot path.  A `TensorMap`
+owns one flat device vector (a torch tensor on the HIP device, used purely as
+device memory) plus its HomSpace (`src/tensors/tensor.jl:10-20`); blocks and
+sub-blocks are views defined by `degeneracystructure`.  Every arithmetic
+operation goes through libtkmi.so.
+
+Entry points and the reference functions they mirror:
+  permute / permute_ / braid_ / add_transform_   src/tensors/indexmanipulations.jl:217-318,557-580
+  tensoradd_                                     src/tensors/tensoroperations.jl:44-62
+  mul_                                           src/tensors/linalg.jl:330-370
+  tensorcontract_ / contract_ / blas_contract_   src/tensors/tensoroperations.jl:130-161,318-462
+  twist_                                         src/tensors/indexmanipulations.jl:62-78
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .spaces import HomSpace, SpaceMismatch, IndexError_
+from .sectors import FERMIONIC
+from .treetransformers import treebraider, flat_plan, twist_factor
+
+_DT = {torch.float64: _lib.TKMI_F64, torch.float32: _lib.TKMI_F32}
+
+DEFAULT_PRECISION = _lib.PREC_F64
+
+
+def _device():
+    _lib.ensure_device(0)
+    if not torch.cuda.is_available():
+        raise RuntimeError("tkmi: no HIP device available")
+    return torch.device("cuda", 0)
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+class TensorMap:
+    """Block-sparse tensor: flat device data vector + HomSpace."""
+
+    __slots__ = ("data", "space")
+
+    def __init__(self, data, space: HomSpace):
+        if data.dim() != 1 or data.numel() != space.dim():
+            raise SpaceMismatch(f"data length {data.numel()} does not match dim {space.dim()} of {space}")
+        if data.dtype not in _DT:
+            raise TypeError("tkmi TensorMap data must be float64 or float32")
+        self.data = data
+        self.space = space
+
+    # ---------------- construction ----------------
+    @staticmethod
+    def empty(space, dtype=torch.float64):
+        return TensorMap(torch.empty(space.dim(), dtype=dtype, device=_device()), space)
+
+    @staticmethod
+    def zeros(space, dtype=torch.float64):
+        t = TensorMap.empty(space, dtype)
+        scale_(t, 0.0)
+        return t
+
+    @staticmethod
+    def from_numpy(arr, space, dtype=torch.float64):
+        arr = np.ascontiguousarray(arr, dtype=np.float64 if dtype == torch.float64 else np.float32)
+        return TensorMap(torch.from_numpy(arr).to(_device()), space)
+
+    @staticmethod
+    def randn(space, rng: np.random.Generator, dtype=torch.float64):
+        """Seeded standard-normal data drawn on the host (the seed stream is
+        this package's, not Julia's Xoshiro)."""
+        return TensorMap.from_numpy(rng.standard_normal(space.dim()), space, dtype)
+
+    def numpy(self):
+        return self.data.detach().cpu().numpy()
+
+    # ---------------- properties ----------------
+    codomain = property(lambda self: self.space.codomain)
+    domain = property(lambda self: self.space.domain)
+    numout = property(lambda self: self.space.numout)
+    numin = property(lambda self: self.space.numin)
+    numind = property(lambda self: self.space.numind)
+    dtype = property(lambda self: self.data.dtype)
+
+    def dim(self):
+        return self.space.dim()
+
+    def adjoint(self):
+        return AdjointTensorMap(self)
+
+    def codomainind(self):
+        return tuple(range(self.numout))
+
+    def domainind(self):
+        return tuple(range(self.numout, self.numind))
+
+    def similar(self, space=None, dtype=None):
+        return TensorMap.empty(space or self.space, dtype or self.dtype)
+
+    def copy(self):
+        t = self.similar()
+        add_(t, self, 1.0, 0.0)
+        return t
+
+    def blocks_host(self):
+        """Host copies of all blocks, keyed by coupled sector (column-major)."""
+        d = self.numpy()
+        out = {}
+        for c, (d1, d2), off in self.space.degeneracystructure()[1]:
+            out[c] = d[off:off + d1 * d2].reshape((d1, d2), order="F")
+        return out
+
+    def __repr__(self):
+        return f"TensorMap({self.space}, dtype={self.dtype})"
+
+
+class AdjointTensorMap:
+    """Lazy adjoint (`src/tensors/adjoint.jl:9-53`): shares the parent's data."""
+
+    __slots__ = ("parent",)
+
+    def __init__(self, parent: TensorMap):
+        self.parent = parent
+
+    space = property(lambda self: self.parent.space.adjoint())
+    numout = property(lambda self: self.parent.numin)
+    numin = property(lambda self: self.parent.numout)
+    numind = property(lambda self: self.parent.numind)
+    dtype = property(lambda self: self.parent.dtype)
+    data = property(lambda self: self.parent.data)
+
+    def dim(self):
+        return self.parent.dim()
+
+    def adjoint(self):
+        return self.parent
+
+    def codomainind(self):
+        return tuple(range(self.numout))
+
+    def domainind(self):
+        return tuple(range(self.numout, self.numind))
+
+
+def adjointtensorindices(t, p):
+    """`adjointtensorindex`: index i of t ↦ index of t′ (0-based)."""
+    n1, n2 = t.numout, t.numin
+    return tuple(n2 + i if i < n1 else i - n1 for i in p)
+
+
+def _is_identity(t, p1, p2):
+    return tuple(p1) == t.codomainind() and tuple(p2) == t.domainind()
+
+
+# ---------------------------------------------------------------------------
+# elementwise
+# ---------------------------------------------------------------------------
+def _exec_flat(dst, src, alpha, beta):
+    n = dst.data.numel()
+    if n == 0:
+        return
+    plan = flat_plan(n)
+    lib = _lib.load()
+    _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[dst.dtype], src.data.data_ptr(), dst.data.data_ptr(),
+                                       float(alpha), float(beta), _stream()), "add!")
+
+
+def add_(tdst, tsrc, alpha=1.0, beta=1.0):
+    """`add!(tdst, tsrc, α, β)`: tdst = β·tdst + α·tsrc (same space)."""
+    if tdst.space != tsrc.space:
+        raise SpaceMismatch(f"{tdst.space} ≠ {tsrc.space}")
+    if tdst.dtype != tsrc.dtype:
+        raise TypeError("add!: dtype mismatch")
+    _exec_flat(tdst, tsrc, alpha, beta)
+    return tdst
+
+
+def scale_(t, alpha):
+    """`scale!(t, α)` in place (α = 0 writes zeros without reading t)."""
+    _exec_flat(t, t, alpha, 0.0)
+    return t
+
+
+# ---------------------------------------------------------------------------
+# permute / braid
+# ---------------------------------------------------------------------------
+def spacecheck_transform(tdst, tsrc, p1, p2):
+    p = tuple(p1) + tuple(p2)
+    if sorted(p) != list(range(tsrc.numind)):
+        raise IndexError_(f"{(p1, p2)} is not a valid permutation for {tsrc.space}")
+    if tdst.space != tsrc.space.permute(p1, p2):
+        raise SpaceMismatch(f"incompatible spaces for permute: {tdst.space} ≠ permute({tsrc.space}, {(p1, p2)})")
+
+
+def add_transform_(tdst, tsrc, p1, p2, alpha, beta, levels=None, dst_twist=()):
+    """`add_transform!` (`indexmanipulations.jl:557-580`) with the device kernel."""
+    spacecheck_transform(tdst, tsrc, p1, p2)
+    if tdst.dtype != tsrc.dtype:
+        raise TypeError("permute!: dtype mismatch")
+    is_adj = isinstance(tsrc, AdjointTensorMap)
+    if not is_adj and not dst_twist and _is_identity(tsrc, p1, p2):
+        return add_(tdst, tsrc, alpha, beta)
+    plan = treebraider(tdst.space, tsrc.space, tuple(p1), tuple(p2), levels, adjoint_src=is_adj,
+                       dst_twist=tuple(dst_twist))
+    lib = _lib.load()
+    _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(), tdst.data.data_ptr(),
+                                       float(alpha), float(beta), _stream()), "add_transform!")
+    return tdst
+
+
+def braid_(tdst, tsrc, p1, p2, levels, alpha=1.0, beta=0.0):
+    """`braid!(tdst, tsrc, (p₁, p₂), levels, α, β)` (`indexmanipulations.jl:304-318`)."""
+    n1 = tsrc.numout
+    lv = (tuple(levels[:n1]), tuple(levels[n1:]))
+    if len(set(levels)) != len(levels) and tsrc.space.unit.braiding_style not in ("bosonic", "fermionic"):
+        raise ValueError("ambiguous braid levels")
+    return add_transform_(tdst, tsrc, p1, p2, alpha, beta, levels=lv)
+
+
+def permute_(tdst, tsrc, p1, p2, alpha=1.0, beta=0.0):
+    """`permute!(tdst, tsrc, p, α, β)` (`indexmanipulations.jl:217-225`)."""
+    return braid_(tdst, tsrc, p1, p2, tuple(range(tsrc.numind)), alpha, beta)
+
+
+def permute(t, p1, p2=(), copy=False):
+    """`permute(t, (p₁, p₂); copy)` (`indexmanipulations.jl:242-258`): shares
+    data for the identity permutation unless `copy`."""
+    p1, p2 = tuple(p1), tuple(p2)
+    if isinstance(t, AdjointTensorMap):
+        q1 = adjointtensorindices(t, p2)
+        q2 = adjointtensorindices(t, p1)
+        return permute(t.parent, q1, q2, copy=copy).adjoint()
+    if not copy and _is_identity(t, p1, p2):
+        return t
+    tdst = TensorMap.empty(t.space.permute(p1, p2), t.dtype)
+    return permute_(tdst, t, p1, p2, 1.0, 0.0)
+
+
+def braid(t, p1, p2, levels, copy=False):
+    p1, p2 = tuple(p1), tuple(p2)
+    if not copy and _is_identity(t, p1, p2):
+        return t
+    tdst = TensorMap.empty(t.space.permute(p1, p2), t.dtype)
+    return braid_(tdst, t, p1, p2, levels)
+
+
+def transpose(t, p1=None, p2=None):
+    """Planar (cyclic) permutation; for symmetric braiding its coefficients
+    coincide with `permute` on the same cyclic index order."""
+    if p1 is None:
+        p1 = tuple(reversed(t.domainind()))
+        p2 = tuple(reversed(t.codomainind()))
+    return permute(t, p1, p2, copy=True)
+
+
+def twist_(t, inds):
+    """`twist!(t, inds)`: scale each sub-block by its twist θ (fermionic only;
+    bosonic sectors have θ=1 and nothing is launched)."""
+    if t.space.unit.braiding_style != FERMIONIC or not inds:
+        return t
+    tmp = t.copy()
+    add_transform_(t, tmp, t.codomainind(), t.domainind(), 1.0, 0.0, dst_twist=tuple(inds))
+    return t
+
+
+# ---------------------------------------------------------------------------
+# tensoradd!
+# ---------------------------------------------------------------------------
+def tensoradd_(C, A, pA, conjA, alpha=1.0, beta=0.0):
+    """`TO.tensoradd!(C, A, pA, conjA, α, β)` (`tensoroperations.jl:44-62`)."""
+    p1, p2 = _canonicalize(pA, C)
+    if conjA:
+        Ap = A.adjoint()
+        q1 = adjointtensorindices(A, p1)
+        q2 = adjointtensorindices(A, p2)
+        return permute_(C, Ap, q1, q2, alpha, beta)
+    return permute_(C, A, p1, p2, alpha, beta)
+
+
+def _canonicalize(p, C):
+    p1, p2 = p
+    if len(p1) == C.numout and len(p2) == C.numin:
+        return tuple(p1), tuple(p2)
+    lin = tuple(p1) + tuple(p2)
+    return lin[:C.numout], lin[C.numout:]
+
+
+# ---------------------------------------------------------------------------
+# mul! (grouped GEMM per coupled sector)
+# ---------------------------------------------------------------------------
+class GemmPlan:
+    def __init__(self, handle, flops, ngroups):
+        self.handle = handle
+        self.flops = flops
+        self.ngroups = ngroups
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib._lib is not None:
+            _lib._lib.tkmi_gemm_plan_destroy(h)
+            self.handle = None
+
+
+_GEMM_CACHE = {}
+
+
+def _block_layout(t):
+    """(c -> (rows, cols, offset, ld, trans)) of op(block) as seen by mul!."""
+    if isinstance(t, AdjointTensorMap):
+        out = {}
+        for c, (d1, d2), off in t.parent.space.degeneracystructure()[1]:
+            out[c] = (d2, d1, off, max(d1, 1), 1)
+        return out
+    return {c: (d1, d2, off, max(d1, 1), 0) for c, (d1, d2), off in t.space.degeneracystructure()[1]}
+
+
+def gemm_plan(tC, tA, tB, with_beta_groups=True):
+    key = (tC.space, tA.space, tB.space, isinstance(tA, AdjointTensorMap), isinstance(tB, AdjointTensorMap))
+    plan = _GEMM_CACHE.get(key)
+    if plan is not None:
+        return plan
+    LA, LB = _block_layout(tA), _block_layout(tB)
+    rows = []
+    for c, (d1, d2), off in tC.space.degeneracystructure()[1]:
+        if c in LA and c in LB:
+            ma, ka, oa, lda, ta = LA[c]
+            kb, nb, ob, ldb, tb = LB[c]
+            if ma != d1 or nb != d2 or ka != kb:
+                raise SpaceMismatch("mul!: block sizes disagree")
+            rows.append((d1, d2, ka, oa, lda, ta, ob, ldb, tb, off, max(d1, 1)))
+        else:
+            # no matching block: C_c = β·C_c (rmul!)
+            rows.append((d1, d2, 0, 0, 1, 0, 0, 1, 0, off, max(d1, 1)))
+    arr = np.array(rows, dtype=np.int64).reshape(-1, 11)
+    lib = _lib.ensure_device()
+    h = _lib.C.c_void_p()
+    cols = [_lib.i64(arr[:, i]) for i in range(11)]
+    ta = _lib.i32(arr[:, 5])
+    tb = _lib.i32(arr[:, 8])
+    _lib.check(lib.tkmi_gemm_plan_create(_lib.C.byref(h), len(rows), cols[0][1], cols[1][1], cols[2][1],
+                                         cols[3][1], cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1],
+                                         cols[9][1], cols[10][1]), "tkmi_gemm_plan_create")
+    flops = float(np.sum(2.0 * arr[:, 0] * arr[:, 1] * arr[:, 2])) if len(rows) else 0.0
+    plan = GemmPlan(h, flops, len(rows))
+    _GEMM_CACHE[key] = plan
+    return plan
+
+
+def mul_(tC, tA, tB, alpha=1.0, beta=0.0, precision=None):
+    """`mul!(tC, tA, tB, α, β)` (`linalg.jl:330-370`): one grouped launch."""
+    if tA.space.compose(tB.space) != tC.space:
+        raise SpaceMismatch(f"{tC.space} ≠ {tA.space} * {tB.space}")
+    if not (tA.dtype == tB.dtype == tC.dtype == torch.float64):
+        raise TypeError("mul!: the grouped GEMM takes float64 tensors")
+    plan = gemm_plan(tC, tA, tB)
+    prec = DEFAULT_PRECISION if precision is None else precision
+    lib = _lib.load()
+    _lib.check(lib.tkmi_gemm_exec(plan.handle, int(prec), float(alpha), float(beta), tA.data.data_ptr(),
+                                  tB.data.data_ptr(), tC.data.data_ptr(), _stream()), "mul!")
+    return tC
+
+
+# ---------------------------------------------------------------------------
+# contraction
+# ---------------------------------------------------------------------------
+def _sortperm(t):
+    return tuple(sorted(range(len(t)), key=lambda i: t[i]))
+
+
+def _contract_candidates(pA, pB, pAB):
+    """`_contract_candidates` (`tensoroperations.jl:360-376`), 0-based."""
+    n1, n2 = len(pA[0]), len(pB[1])
+    qA = _sortperm(pA[1])
+    pA1 = (pA[0], tuple(pA[1][i] for i in qA))
+    pB1 = (tuple(pB[0][i] for i in qA), pB[1])
+    qB = _sortperm(pB[0])
+    pA2 = (pA[0], tuple(pA[1][i] for i in qB))
+    pB2 = (tuple(pB[0][i] for i in qB), pB[1])
+    f = (lambda x: x - n1 if x >= n1 else x + n2)
+    pABp = (tuple(f(x) for x in pAB[0]), tuple(f(x) for x in pAB[1]))
+    return pA1, pB1, pA2, pB2, pABp
+
+
+def _invperm(p):
+    inv = [0] * len(p)
+    for i, x in enumerate(p):
+        inv[x] = i
+    return tuple(inv)
+
+
+def oindABinC(pAB, pA, pB):
+    """TensorOperations `oindABinC`: positions in C of A's and B's open indices."""
+    ip = _invperm(tuple(pAB[0]) + tuple(pAB[1]))
+    na, nb = len(pA[0]), len(pB[1])
+    return tuple(ip[i] for i in range(na)), tuple(ip[na + i] for i in range(nb))
+
+
+def _has_shared_permute(t, p):
+    if isinstance(t, AdjointTensorMap):
+        q1 = adjointtensorindices(t, p[1])
+        q2 = adjointtensorindices(t, p[0])
+        return _has_shared_permute(t.parent, (q1, q2))
+    return _is_identity(t, p[0], p[1])
+
+
+def _contract_memcost(dA, dB, dC, C, A, pA, B, pB, pAB):
+    ipAB = oindABinC(pAB, pA, pB)
+    return (dA * (not _has_shared_permute(A, pA) or A.dtype != C.dtype)
+            + dB * (not _has_shared_permute(B, pB) or B.dtype != C.dtype)
+            + dC * (not _has_shared_permute(C, ipAB)))
+
+
+def _rev(p):
+    return (p[1], p[0])
+
+
+def contract_(C, A, pA, B, pB, pAB, alpha=1.0, beta=0.0, precision=None):
+    """`contract!` (`tensoroperations.jl:318-357`): pick operand order and index
+    sorting by memory cost, then `blas_contract!`."""
+    if len(pA[1]) != len(pB[0]):
+        raise IndexError_("number of contracted indices does not match")
+    pA1, pB1, pA2, pB2, pABp = _contract_candidates(pA, pB, pAB)
+    dA, dB, dC = A.dim(), B.dim(), C.dim()
+    m1 = _contract_memcost(dA, dB, dC, C, A, pA1, B, pB1, pAB)
+    m2 = _contract_memcost(dA, dB, dC, C, A, pA2, B, pB2, pAB)
+    m3 = _contract_memcost(dB, dA, dC, C, B, _rev(pB1), A, _rev(pA1), pABp)
+    m4 = _contract_memcost(dB, dA, dC, C, B, _rev(pB2), A, _rev(pA2), pABp)
+    if min(m1, m2) <= min(m3, m4):
+        if m1 <= m2:
+            return blas_contract_(C, A, pA1, B, pB1, pAB, alpha, beta, precision)
+        return blas_contract_(C, A, pA2, B, pB2, pAB, alpha, beta, precision)
+    if m3 <= m4:
+        return blas_contract_(C, B, _rev(pB1), A, _rev(pA1), pABp, alpha, beta, precision)
+    return blas_contract_(C, B, _rev(pB2), A, _rev(pA2), pABp, alpha, beta, precision)
+
+
+def _space_isdual(t, i):
+    return t.space[i].isdual
+
+
+def blas_contract_(C, A, pA, B, pB, pAB, alpha, beta, precision=None):
+    """`blas_contract!` (`tensoroperations.jl:392-462`): permute operands into
+    GEMM form (fermionic twists folded into the permute), grouped GEMM, and a
+    final permute of the result when C is not in GEMM layout."""
+    TC = C.dtype
+    copyA = not (_has_shared_permute(A, pA) and A.dtype == TC)
+    copyB = not (_has_shared_permute(B, pB) and B.dtype == TC)
+    fermionic = C.space.unit.braiding_style == FERMIONIC
+    if fermionic and any(_space_isdual(B, i) for i in pB[0]):
+        if not (copyA ^ copyB):
+            twistA = A.dim() < B.dim()
+        else:
+            twistA = copyA
+        twistB = not twistA
+        copyA |= twistA
+        copyB |= twistB
+    else:
+        twistA = twistB = False
+
+    if copyA:
+        Anew = TensorMap.empty(A.space.permute(*pA), TC)
+        tw = ()
+        if twistA:
+            tw = tuple(i for i in Anew.domainind() if not Anew.space[i].isdual)
+        _transform_into(Anew, A, pA, tw)
+    else:
+        Anew = permute(A, *pA)
+    if copyB:
+        Bnew = TensorMap.empty(B.space.permute(*pB), TC)
+        tw = ()
+        if twistB:
+            tw = tuple(i for i in Bnew.codomainind() if Bnew.space[i].isdual)
+        _transform_into(Bnew, B, pB, tw)
+    else:
+        Bnew = permute(B, *pB)
+
+    ipAB = oindABinC(pAB, (Anew.codomainind(), Anew.domainind()), (Bnew.codomainind(), Bnew.domainind()))
+    copyC = not _has_shared_permute(C, ipAB)
+    if copyC:
+        Cnew = TensorMap.empty(C.space.permute(*ipAB), TC)
+        mul_(Cnew, Anew, Bnew, 1.0, 0.0, precision)
+        tensoradd_(C, Cnew, pAB, False, alpha, beta)
+    else:
+        Cnew = permute(C, *ipAB)
+        mul_(Cnew, Anew, Bnew, alpha, beta, precision)
+    return C
+
+
+def _transform_into(dst, src, p, twist_legs):
+    p1, p2 = tuple(p[0]), tuple(p[1])
+    if isinstance(src, AdjointTensorMap) or twist_legs or not _is_identity(src, p1, p2):
+        add_transform_(dst, src, p1, p2, 1.0, 0.0, dst_twist=twist_legs)
+    else:
+        add_(dst, src, 1.0, 0.0)
+
+
+def spacecheck_contract(C, A, pA, conjA, B, pB, conjB, pAB):
+    sA = A.space.adjoint() if conjA else A.space
+    sB = B.space.adjoint() if conjB else B.space
+    qA = adjointtensorindices(A, pA[0]), adjointtensorindices(A, pA[1])
+    qB = adjointtensorindices(B, pB[0]), adjointtensorindices(B, pB[1])
+    VA = sA.permute(*(qA if conjA else pA))
+    VB = sB.permute(*(qB if conjB else pB))
+    VC = VA.compose(VB).permute(*pAB)
+    if VC != C.space:
+        raise SpaceMismatch(f"incompatible spaces for tensorcontract: {VC} ≠ {C.space}")
+
+
+def tensorcontract_(C, A, pA, conjA, B, pB, conjB, pAB, alpha=1.0, beta=0.0, precision=None):
+    """`TO.tensorcontract!` (`tensoroperations.jl:130-161`).  Index tuples are
+    0-based; conj operands become lazy adjoints."""
+    pAB = _canonicalize(pAB, C)
+    spacecheck_contract(C, A, pA, conjA, B, pB, conjB, pAB)
+    if conjA:
+        A2, pA2 = A.adjoint(), (adjointtensorindices(A, pA[0]), adjointtensorindices(A, pA[1]))
+    else:
+        A2, pA2 = A, (tuple(pA[0]), tuple(pA[1]))
+    if conjB:
+        B2, pB2 = B.adjoint(), (adjointtensorindices(B, pB[0]), adjointtensorindices(B, pB[1]))
+    else:
+        B2, pB2 = B, (tuple(pB[0]), tuple(pB[1]))
+    return contract_(C, A2, pA2, B2, pB2, pAB, alpha, beta, precision)
+
+
+def tensorcontract_space(A, pA, conjA, B, pB, conjB, pAB):
+    """`TO.tensorcontract(HomSpace...)` (`homspace.jl:273-295`)."""
+    sA = A.space.adjoint() if conjA else A.space
+    sB = B.space.adjoint() if conjB else B.space
+    qA = (adjointtensorindices(A, pA[0]), adjointtensorindices(A, pA[1])) if conjA else pA
+    qB = (adjointtensorindices(B, pB[0]), adjointtensorindices(B, pB[1])) if conjB else pB
+    return sA.permute(*qA).compose(sB.permute(*qB)).permute(*pAB)
+
+
+def tensorcontract(A, pA, conjA, B, pB, conjB, pAB, alpha=1.0, precision=None):
+    """Allocating contraction: returns a new TensorMap."""
+    VC = tensorcontract_space(A, pA, conjA, B, pB, conjB, pAB)
+    C = TensorMap.empty(VC, A.dtype)
+    return tensorcontract_(C, A, pA, conjA, B, pB, conjB, pAB, alpha, 0.0, precision)
+
+
+# ---------------------------------------------------------------------------
+# label-based pairwise contraction (the index bookkeeping @tensor lowers to)
+# ---------------------------------------------------------------------------
+def contraction_indices(la, lb, lc, nout):
+    """TensorOperations-style (pA, pB, pAB) for C[lc] = A[la] * B[lb] with the
+    first `nout` labels of `lc` in C's codomain.  Open indices keep their order
+    of appearance in A and B; contracted ones follow A's order."""
+    la, lb, lc = list(la), list(lb), list(lc)
+    shared = [x for x in la if x in lb]
+    oA = [i for i, x in enumerate(la) if x not in lb]
+    cA = [la.index(x) for x in shared]
+    cB = [lb.index(x) for x in shared]
+    oB = [i for i, x in enumerate(lb) if x not in la]
+    ab = [la[i] for i in oA] + [lb[i] for i in oB]
+    if sorted(ab) != sorted(lc):
+        raise IndexError_(f"open labels {ab} do not match output labels {lc}")
+    pos = [ab.index(x) for x in lc]
+    return (tuple(oA), tuple(cA)), (tuple(cB), tuple(oB)), (tuple(pos[:nout]), tuple(pos[nout:]))
+
+
+def contract_labels(A, la, B, lb, lc, nout, conjA=False, conjB=False, out=None, alpha=1.0, beta=0.0,
+                    precision=None):
+    """C[lc] = α·A[la]·B[lb] (+ β·C): one `tensorcontract!` call."""
+    pA, pB, pAB = contraction_indices(la, lb, lc, nout)
+    if out is None:
+        VC = tensorcontract_space(A, pA, conjA, B, pB, conjB, pAB)
+        out = TensorMap.empty(VC, A.dtype)
+        beta = 0.0
+    return tensorcontract_(out, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, precision)

@@ -1,0 +1,241 @@
+"""Tree transformers: the tables a permute/braid kernel consumes, built and
+cached on the host and uploaded to the device once.
+
+Mirrors `src/tensors/treetransformers.jl`:
+- `AbelianTreeTransformer` (`:10-42`): one (coeff, dst structure, src structure)
+  entry per source tree pair, in the source's tree order.
+- `GenericTreeTransformer` (`:44-120`): per fusion block of the source, the
+  recoupling matrix U (dst trees × src trees), the common sub-block size and
+  the (strides, offset) of every tree; sorted by descending
+  `length(U)*prod(sz)` (`:186-205`).
+- `treebraider`/`treetransposer` LRU caches (`:156-180`, `src/auxiliary/caches.jl:135-173`).
+
+The serialized table is handed to `tkmi_abelian_plan_create` /
+`tkmi_generic_plan_create`, which return an immutable device plan.
+"""
+from __future__ import annotations
+
+from collections import OrderedDict
+
+import numpy as np
+
+from . import _lib
+from .fusiontrees import braid_pair, braid_block
+from .sectors import UNIQUE
+
+DEFAULT_GLOBALCACHE_SIZE = 10 ** 4  # src/auxiliary/caches.jl:19
+
+
+class TransformPlan:
+    """Owner of one device transform plan (C handle) plus its host table."""
+
+    def __init__(self, handle, kind, nentries, elements, table):
+        self.handle = handle
+        self.kind = kind
+        self.nentries = nentries
+        self.elements = elements
+        self.table = table
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib._lib is not None:
+            _lib._lib.tkmi_transform_plan_destroy(h)
+            self.handle = None
+
+
+class _LRU:
+    def __init__(self, maxsize):
+        self.maxsize = maxsize
+        self.d = OrderedDict()
+        self.hits = 0
+        self.misses = 0
+
+    def get(self, key):
+        v = self.d.get(key)
+        if v is not None:
+            self.d.move_to_end(key)
+            self.hits += 1
+        else:
+            self.misses += 1
+        return v
+
+    def put(self, key, v):
+        self.d[key] = v
+        self.d.move_to_end(key)
+        while len(self.d) > self.maxsize:
+            self.d.popitem(last=False)
+
+    def clear(self):
+        self.d.clear()
+
+
+_TRANSFORMER_CACHE = _LRU(DEFAULT_GLOBALCACHE_SIZE)
+
+
+def empty_globalcaches():
+    """`empty_globalcaches!()` (`src/auxiliary/caches.jl:2-5`)."""
+    _TRANSFORMER_CACHE.clear()
+
+
+def global_cache_info():
+    c = _TRANSFORMER_CACHE
+    return {"size": len(c.d), "maxsize": c.maxsize, "hits": c.hits, "misses": c.misses}
+
+
+def _perm0(p1, p2):
+    return tuple(p1) + tuple(p2)
+
+
+def twist_factor(tree_pair, legs, numout):
+    """θ = Π_{i∈legs} twist(uncoupled sector at leg i) for a tree pair
+    (`twist!`, `src/tensors/indexmanipulations.jl:62-78`)."""
+    f1, f2 = tree_pair
+    th = 1
+    for i in legs:
+        s = f1.uncoupled[i] if i < numout else f2.uncoupled[i - numout]
+        th *= s.twist()
+    return th
+
+
+def abelian_table(Vdst, Vsrc, p1, p2, src_structure=None, dst_twist=()):
+    """Rows of `AbelianTreeTransformer(braid, p, Vdst, Vsrc)`.
+
+    `src_structure` overrides the source's sub-block structure (used for the
+    lazy adjoint of a tensor); `dst_twist` lists destination legs whose twist
+    is folded into the coefficient (the fermionic `twist!` after the copy in
+    `blas_contract!`, `src/tensors/tensoroperations.jl:425-441`)."""
+    fts_src = src_structure if src_structure is not None else Vsrc.subblockstructure()
+    fts_dst = Vdst.subblockstructure()
+    n = Vsrc.numind
+    coeffs, dsts, srcs, pairs = [], [], [], []
+    for pair in Vsrc.fusiontrees():
+        f1, f2 = pair
+        (g1, g2), coeff = braid_pair(f1, f2, p1, p2)
+        if dst_twist:
+            coeff *= twist_factor((g1, g2), dst_twist, Vdst.numout)
+        sd = fts_dst[(g1, g2)]
+        ss = fts_src[pair]
+        coeffs.append(float(coeff))
+        dsts.append(list(sd[0]) + list(sd[1]) + [sd[2]])
+        srcs.append(list(ss[0]) + list(ss[1]) + [ss[2]])
+        pairs.append((pair, (g1, g2)))
+    return {
+        "coeffs": np.array(coeffs, dtype=np.float64),
+        "dst": np.array(dsts, dtype=np.int64).reshape(len(dsts), 2 * n + 1),
+        "src": np.array(srcs, dtype=np.int64).reshape(len(srcs), 2 * n + 1),
+        "pairs": pairs,
+    }
+
+
+def generic_table(Vdst, Vsrc, p1, p2, levels1=None, levels2=None, src_structure=None, dst_twist=()):
+    """Rows of `GenericTreeTransformer(braid, p, Vdst, Vsrc)` (one per fusion
+    block), sorted by descending weight like `sort!(transformer)`."""
+    fts_src = src_structure if src_structure is not None else Vsrc.subblockstructure()
+    fts_dst = Vdst.subblockstructure()
+    n1, n2 = Vsrc.numout, Vsrc.numin
+    if levels1 is None:
+        levels1 = tuple(range(n1))
+    if levels2 is None:
+        levels2 = tuple(range(n1, n1 + n2))
+    unit = Vsrc.unit
+    rows = []
+    for fs_src in Vsrc.fusionblocks():
+        fs_dst, U = braid_block(fs_src, p1, p2, levels1, levels2, unit)
+        if dst_twist:
+            th = np.array([twist_factor(t, dst_twist, Vdst.numout) for t in fs_dst.trees], dtype=np.float64)
+            U = th[:, None] * U
+        sz_src = fts_src[fs_src.trees[0]][0]
+        sz_dst = fts_dst[fs_dst.trees[0]][0]
+        so_src = [list(fts_src[t][1]) + [fts_src[t][2]] for t in fs_src.trees]
+        so_dst = [list(fts_dst[t][1]) + [fts_dst[t][2]] for t in fs_dst.trees]
+        w = U.size * int(np.prod(sz_dst))
+        rows.append((w, U, sz_dst, so_dst, sz_src, so_src, fs_src, fs_dst))
+    rows.sort(key=lambda r: r[0], reverse=True)
+    return rows
+
+
+def _make_abelian_plan(tab, perm, dst_len, src_len):
+    lib = _lib.ensure_device()
+    n = len(perm)
+    h = _lib.C.c_void_p()
+    pa, pp = _lib.i32(perm)
+    ca, cp = _lib.f64(tab["coeffs"])
+    da, dp = _lib.i64(tab["dst"].reshape(-1))
+    sa, sp = _lib.i64(tab["src"].reshape(-1))
+    _lib.check(lib.tkmi_abelian_plan_create(_lib.C.byref(h), n, pp, len(tab["coeffs"]), cp, dp, sp,
+                                            int(dst_len), int(src_len)), "tkmi_abelian_plan_create")
+    return TransformPlan(h, "abelian", len(tab["coeffs"]), None, tab)
+
+
+def _make_generic_plan(rows, perm, dst_len, src_len):
+    lib = _lib.ensure_device()
+    n = len(perm)
+    rws = np.array([r[1].shape[0] for r in rows], dtype=np.int64)
+    cls = np.array([r[1].shape[1] for r in rows], dtype=np.int64)
+    U = np.concatenate([np.asarray(r[1], dtype=np.float64).reshape(-1, order="F") for r in rows]) \
+        if rows else np.zeros(0)
+    szd = np.array([r[2] for r in rows], dtype=np.int64).reshape(-1)
+    szs = np.array([r[4] for r in rows], dtype=np.int64).reshape(-1)
+    sod = np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1)
+    sos = np.array([x for r in rows for x in r[5]], dtype=np.int64).reshape(-1)
+    h = _lib.C.c_void_p()
+    pa, pp = _lib.i32(perm)
+    keep = [_lib.i64(rws), _lib.i64(cls), _lib.f64(U), _lib.i64(szd), _lib.i64(szs), _lib.i64(sod),
+            _lib.i64(sos)]
+    _lib.check(lib.tkmi_generic_plan_create(_lib.C.byref(h), n, pp, len(rows), keep[0][1], keep[1][1],
+                                            keep[2][1], keep[3][1], keep[4][1], keep[5][1], keep[6][1],
+                                            int(dst_len), int(src_len)), "tkmi_generic_plan_create")
+    return TransformPlan(h, "generic", len(rows), None, rows)
+
+
+def treebraider(Vdst, Vsrc, p1, p2, levels=None, *, adjoint_src=False, dst_twist=()):
+    """Cached device plan for `braid!/permute!` from `Vsrc` to `Vdst`
+    (`treebraider`, `treetransformers.jl:159-167`).  With `adjoint_src`, the
+    source is the lazy adjoint of a tensor with space `Vsrc.adjoint()` (its data
+    is the parent's, sub-blocks permuted as in `src/tensors/adjoint.jl:44-48`)."""
+    key = ("braid", Vdst, Vsrc, tuple(p1), tuple(p2), levels, adjoint_src, tuple(dst_twist))
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is not None:
+        return plan
+    if Vsrc.permute(p1, p2) != Vdst:
+        from .spaces import SpaceMismatch
+        raise SpaceMismatch("Incompatible spaces for permuting.")
+    src_structure = adjoint_structure(Vsrc) if adjoint_src else None
+    perm = _perm0(p1, p2)
+    if Vsrc.unit.fusion_style == UNIQUE:
+        tab = abelian_table(Vdst, Vsrc, p1, p2, src_structure, dst_twist)
+        plan = _make_abelian_plan(tab, perm, Vdst.dim(), Vsrc.dim())
+    else:
+        lv1 = levels[0] if levels else None
+        lv2 = levels[1] if levels else None
+        rows = generic_table(Vdst, Vsrc, p1, p2, lv1, lv2, src_structure, dst_twist)
+        plan = _make_generic_plan(rows, perm, Vdst.dim(), Vsrc.dim())
+    _TRANSFORMER_CACHE.put(key, plan)
+    return plan
+
+
+def adjoint_structure(Vadj):
+    """Sub-block structure of the lazy adjoint t′ of a tensor t with space
+    `Vadj.adjoint()`: (f₁, f₂) of t′ is t's (f₂, f₁) sub-block viewed with the
+    domain legs first (`subblock(::AdjointTensorMap)`, `src/tensors/adjoint.jl:44-48`)."""
+    parent = Vadj.adjoint()
+    ps = parent.subblockstructure()
+    n1p = parent.numout
+    out = {}
+    for (f1, f2), (sz, st, off) in ps.items():
+        out[(f2, f1)] = (tuple(sz[n1p:]) + tuple(sz[:n1p]), tuple(st[n1p:]) + tuple(st[:n1p]), off)
+    return out
+
+
+def flat_plan(length):
+    """Plan for an elementwise y = β·y + α·x over a flat vector (the identity
+    transform used by add!/scale!)."""
+    key = ("flat", int(length))
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is not None:
+        return plan
+    tab = {"coeffs": np.ones(1 if length else 0), "dst": np.array([[length, 1, 0]] if length else np.zeros((0, 3))),
+           "src": np.array([[length, 1, 0]] if length else np.zeros((0, 3)))}
+    plan = _make_abelian_plan(tab, (0,), length, length)
+    _TRANSFORMER_CACHE.put(key, plan)
+    return plan

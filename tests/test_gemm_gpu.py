@@ -1,0 +1,117 @@
+"""GPU parity: grouped per-sector GEMM (K3 FP64, K4 FP32, K5 3×BF16) through
+the C ABI vs numpy.  Integer-valued inputs make FP64 results exact (any
+summation order), which pins the MFMA fragment layouts bit-for-bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from tkmi import _lib, tensors
+    tensors._device()
+    return _lib
+
+
+def _make_groups(rng, shapes, transa, transb):
+    """Pack groups into flat A, B, C host arrays; returns (arrays, tables)."""
+    offs_a = offs_b = offs_c = 0
+    tab = []
+    for (m, n, k), ta, tb in zip(shapes, transa, transb):
+        lda = (k if ta else m) + int(rng.integers(0, 3))
+        ldb = (n if tb else k) + int(rng.integers(0, 3))
+        ldc = m + int(rng.integers(0, 3))
+        lda, ldb, ldc = max(lda, 1), max(ldb, 1), max(ldc, 1)
+        asz = lda * (m if ta else k)
+        bsz = ldb * (k if tb else n)
+        csz = ldc * n
+        tab.append((m, n, k, offs_a, lda, ta, offs_b, ldb, tb, offs_c, ldc))
+        offs_a += asz
+        offs_b += bsz
+        offs_c += csz
+    return tab, offs_a, offs_b, offs_c
+
+
+def _run(lib, tab, A, B, Cm, prec, alpha, beta, tile=0):
+    arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+    cols = [lib.i64(arr[:, i]) for i in range(11)]
+    ta = lib.i32(arr[:, 5])
+    tb = lib.i32(arr[:, 8])
+    L = lib.load()
+    h = C.c_void_p()
+    lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                      cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1], cols[10][1]),
+              "plan")
+    lib.check(L.tkmi_gemm_plan_set_tile(h, prec, tile), "tile")
+    dA = torch.from_numpy(A).cuda()
+    dB = torch.from_numpy(B).cuda()
+    dC = torch.from_numpy(Cm).cuda()
+    s = torch.cuda.current_stream().cuda_stream
+    lib.check(L.tkmi_gemm_exec(h, prec, alpha, beta, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s), "exec")
+    out = dC.cpu().numpy()
+    L.tkmi_gemm_plan_destroy(h)
+    return out
+
+
+def _ref(tab, A, B, Cm, alpha, beta):
+    out = Cm.copy()
+    for (m, n, k, oa, lda, ta, ob, ldb, tb, oc, ldc) in tab:
+        if m == 0 or n == 0:
+            continue
+        if k > 0:
+            # op(A) is m×k: column-major (ld=lda) or its transpose
+            a = A[oa:oa + lda * m].reshape(m, lda)[:, :k] if ta else A[oa:oa + lda * k].reshape(k, lda).T[:m, :]
+            # op(B) is k×n
+            b = B[ob:ob + ldb * k].reshape(k, ldb)[:, :n] if tb else B[ob:ob + ldb * n].reshape(n, ldb).T[:k, :]
+            prod = a @ b
+        else:
+            prod = np.zeros((m, n))
+        c = out[oc:oc + ldc * n].reshape(n, ldc).T[:m, :]
+        c[...] = alpha * prod + (beta * c if beta != 0 else 0)
+    return out
+
+
+SHAPES = [(1, 1, 1), (16, 16, 4), (29, 24, 5), (64, 64, 16), (65, 63, 17), (130, 70, 33), (200, 129, 290),
+          (5, 300, 7), (300, 4, 4), (17, 33, 0), (0, 5, 3)]
+
+
+@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_f64_exact_integer(lib, ta, tb, tile):
+    rng = np.random.default_rng(10 * ta + tb)
+    tab, na, nb, nc = _make_groups(rng, SHAPES, [ta] * len(SHAPES), [tb] * len(SHAPES))
+    A = rng.integers(-8, 9, na).astype(np.float64)
+    B = rng.integers(-8, 9, nb).astype(np.float64)
+    Cm = rng.integers(-8, 9, nc).astype(np.float64)
+    for alpha, beta in ((1.0, 0.0), (2.0, -1.0), (1.0, 1.0)):
+        out = _run(lib, tab, A, B, Cm, 0, alpha, beta, tile)
+        ref = _ref(tab, A, B, Cm, alpha, beta)
+        np.testing.assert_array_equal(out, ref)
+
+
+def test_f64_random(lib):
+    rng = np.random.default_rng(7)
+    shapes = [(int(rng.integers(1, 400)), int(rng.integers(1, 400)), int(rng.integers(1, 300))) for _ in range(12)]
+    tab, na, nb, nc = _make_groups(rng, shapes, list(rng.integers(0, 2, 12)), list(rng.integers(0, 2, 12)))
+    A, B, Cm = rng.standard_normal(na), rng.standard_normal(nb), rng.standard_normal(nc)
+    out = _run(lib, tab, A, B, Cm, 0, 1.0, 0.5)
+    ref = _ref(tab, A, B, Cm, 1.0, 0.5)
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 2e-5), (2, 1e-6)])
+def test_reduced_precision_forms(lib, prec, tol):
+    """FP32 MFMA and 3×BF16 split forms: relative Frobenius error vs FP64."""
+    rng = np.random.default_rng(11)
+    shapes = [(300, 280, 290), (64, 64, 64), (33, 17, 1000), (128, 129, 5)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0, 1], [0, 0, 1, 1])
+    A, B, Cm = rng.standard_normal(na), rng.standard_normal(nb), np.zeros(nc)
+    ref = _ref(tab, A, B, Cm, 1.0, 0.0)
+    for tile in (64, 128):
+        out = _run(lib, tab, A, B, Cm, prec, 1.0, 0.0, tile)
+        err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+        assert err <= tol, (prec, tile, err)
