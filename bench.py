@@ -1,0 +1,278 @@
+"""Benchmark: U1 two-site H_eff application (D≈2048) on one MI355X.
+
+One step = one application y = H_eff(x) of the two-site effective Hamiltonian
+(four pairwise tensorcontract! calls: operand permutes + one grouped GEMM each,
+captured in one HIP graph), FP64 — the drop-in arithmetic.
+
+Reported:
+  value        contract GFLOP/s = Σ_c 2·m·n·k of the chain's grouped GEMMs / step time
+  permute      aggregate GB/s of the chain's permutes (16 B/element, 24 with β≠0)
+  precision    FP32-MFMA and 3×BF16 split forms: step time, speed-up over FP64 and
+               relative Frobenius error of y against the FP64 result
+  roofline     dominant kernel (FP64 grouped GEMM) achieved TFLOP/s vs 78.6 peak,
+               from HIP events around each of its launches on the launch stream
+  cpu_baseline the oracle's CPU restatement (table-driven C permute loop + numpy/
+               OpenBLAS dgemm per coupled sector) replaying the same kernel sequence
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "tensorkit.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "U1 D=2048 contract GFLOP/s + permute GB/s; split-BF16 vs FP64 at stated err"
+PEAK_FP64_TF = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gemm_kernel_name(prec, tile):
+    return {0: f"gemm_k4_kernel<double, {tile}, {tile}>", 1: f"gemm_k4_kernel<float, {tile}, {tile}>",
+            2: f"gemm_bf16x3_kernel<{tile}, {tile}>"}[prec]
+
+
+def time_launches(fn, reps, torch):
+    """Average duration (ms) of `fn()` launches on the current stream, HIP events."""
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def kernel_breakdown(trace, T, lib, torch, reps=20):
+    """Per-launch timing of every kernel in one chain (eager re-issue)."""
+    import ctypes as C
+    L = lib.load()
+    s = torch.cuda.current_stream().cuda_stream
+    gemms, perms = [], []
+    for op in trace:
+        if op[0] == "gemm":
+            _, plan, prec, (pa, _), (pb, _), (pc, nc), a, b = op
+            tile, nt = C.c_int32(), C.c_int64()
+            L.tkmi_gemm_plan_query(plan.handle, prec, C.byref(tile), C.byref(nt))
+            fn = lambda plan=plan, prec=prec, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec(
+                plan.handle, prec, a, b, pa, pb, pc, s)
+            ms = time_launches(fn, reps, torch)
+            gemms.append({"flops": plan.flops, "ms": ms, "kernel": gemm_kernel_name(prec, tile.value),
+                          "groups": plan.ngroups, "tiles": nt.value})
+        else:
+            _, plan, perm, (ps, ns), (pd, nd), a, b = op
+            fn = lambda plan=plan, ps=ps, pd=pd, a=a, b=b: L.tkmi_transform_exec(plan.handle, 0, ps, pd, a, b, s)
+            ms = time_launches(fn, reps, torch)
+            perms.append({"elements": nd, "bytes": nd * (24 if b != 0 else 16), "ms": ms,
+                          "entries": plan.nentries})
+    return gemms, perms
+
+
+def cpu_replay(trace, init, nthreads):
+    """Replay the recorded kernel sequence on the host with the oracle's C
+    permute loop and numpy (OpenBLAS) GEMMs per coupled sector."""
+    import numpy as np
+    from oracle import kernels as OK
+    bufs = {}
+
+    def get(ptrn):
+        p, n = ptrn
+        if p not in bufs:
+            bufs[p] = init[p].copy() if p in init else np.zeros(n)
+        return bufs[p]
+
+    for op in trace:
+        if op[0] == "transform":
+            _, plan, perm, src, dst, a, b = op
+            s, d = get(src), get(dst)
+            if plan.kind == "abelian":
+                OK.abelian_permute(plan.table, perm, s, d, a, b, nthreads)
+            else:
+                OK.generic_permute(plan.table, perm, s, d, a, b, nthreads)
+        else:
+            _, plan, prec, pa, pb, pc, a, b = op
+            A, B, Cm = get(pa), get(pb), get(pc)
+            for (m, n, k, oa, lda, ta, ob, ldb, tb, oc, ldc) in plan.table:
+                if m == 0 or n == 0:
+                    continue
+                c = Cm[oc:oc + ldc * n].reshape(n, ldc).T[:m, :]
+                if k == 0:
+                    c *= b
+                    continue
+                am = A[oa:oa + lda * m].reshape(m, lda)[:, :k] if ta else A[oa:oa + lda * k].reshape(k, lda).T[:m]
+                bm = B[ob:ob + ldb * k].reshape(k, ldb)[:, :n] if tb else B[ob:ob + ldb * n].reshape(n, ldb).T[:k]
+                if b == 0:
+                    c[...] = a * (am @ bm)
+                else:
+                    c[...] = a * (am @ bm) + b * c
+    return bufs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--D", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
+              "(see the sharding design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from tkmi import tensors as T
+    from tkmi import _lib
+    from tkmi.heff import TwoSiteHeff, two_site_spaces
+
+    T._device()
+    V, P, W = two_site_spaces(D=args.D, qmax=8, mode="round")
+    log(f"V dim {V.dim()}, x dim {TwoSiteHeff.__name__}")
+    H = TwoSiteHeff(V, P, W, np.random.default_rng(1))
+    log(f"x elements {H.x.dim()}")
+
+    # eager pass under the recorder: plans, flops, kernel sequence
+    T.start_trace()
+    y_eager = H.apply()
+    trace = T.stop_trace()
+    torch.cuda.synchronize()
+    flops = sum(op[1].flops for op in trace if op[0] == "gemm")
+    ngemm_groups = sum(op[1].ngroups for op in trace if op[0] == "gemm")
+    log(f"chain: {sum(1 for o in trace if o[0]=='gemm')} grouped GEMM launches, {ngemm_groups} groups, "
+        f"{flops/1e9:.3f} GFLOP, {sum(1 for o in trace if o[0]=='transform')} permutes")
+
+    # FP64 graph
+    y64 = H.capture(precision=_lib.PREC_F64)
+    for _ in range(args.warmup):
+        H.replay()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(args.steps):
+        H.replay()
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms64 = e0.elapsed_time(e1) / args.steps
+    y_ref = H.graph_y.numpy().copy()
+    log(f"FP64 step {ms64:.4f} ms (wall {wall/args.steps*1e3:.4f} ms), {flops/ms64/1e6:.1f} GFLOP/s")
+    eager_err = float(np.abs(y_ref - y_eager.numpy()).max())
+
+    # reduced-precision forms (separate graphs)
+    prec_report = {}
+    for name, prec in (("fp32", _lib.PREC_F32), ("bf16x3", _lib.PREC_BF16X3)):
+        H2 = TwoSiteHeff.__new__(TwoSiteHeff)
+        H2.__dict__.update({k: v for k, v in H.__dict__.items() if k not in ("graph", "outs")})
+        H2.outs = [None] * 4
+        H2.graph = None
+        H2.capture(precision=prec)
+        for _ in range(args.warmup):
+            H2.replay()
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(args.steps):
+            H2.replay()
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        y = H2.graph_y.numpy()
+        err = float(np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref))
+        prec_report[name] = {"ms_per_step": round(ms, 5), "gflops": round(flops / ms / 1e6, 1),
+                             "speedup_vs_fp64": round(ms64 / ms, 3), "rel_frobenius_err_vs_fp64": err}
+        log(f"{name}: {ms:.4f} ms, x{ms64/ms:.2f} vs FP64, rel err {err:.2e}")
+
+    # per-kernel breakdown and roofline of the dominant kernel
+    gemms, perms = kernel_breakdown(trace, T, _lib, torch)
+    by_kernel = {}
+    for g in gemms:
+        d = by_kernel.setdefault(g["kernel"], {"flops": 0.0, "ms": 0.0, "calls": 0})
+        d["flops"] += g["flops"]
+        d["ms"] += g["ms"]
+        d["calls"] += 1
+    dom_name, dom = max(by_kernel.items(), key=lambda kv: kv[1]["ms"])
+    achieved_tf = dom["flops"] / dom["ms"] / 1e9
+    perm_bytes = sum(p["bytes"] for p in perms)
+    perm_ms = sum(p["ms"] for p in perms)
+    big = max(perms, key=lambda p: p["bytes"]) if perms else None
+    log("gemm launches: " + "; ".join(f"{g['kernel']} {g['flops']/1e9:.3f} GF {g['ms']*1e3:.1f} us "
+                                       f"{g['flops']/g['ms']/1e9:.2f} TF/s" for g in gemms))
+    log("permutes: " + "; ".join(f"{p['elements']} el {p['ms']*1e3:.1f} us {p['bytes']/p['ms']/1e6:.0f} GB/s"
+                                 for p in perms))
+
+    out = {
+        "metric": METRIC,
+        "value": round(flops / ms64 / 1e6, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded randn operands)",
+        "config": {"workload": "U1 two-site H_eff application y=((((L·x)·M1)·M2)·R), graph-captured",
+                   "V": f"U1 q=-8:8, d_q=round(2048·e^(-q²/16)/Σ), D={V.dim()}",
+                   "P": "U1 ±1/2 => 1", "W": "U1 0=>3, ±1=>1",
+                   "x_elements": H.x.dim(), "gemm_gflop_per_step": round(flops / 1e9, 4),
+                   "grouped_gemm_launches": len(gemms), "gemm_groups": ngemm_groups,
+                   "permute_launches": len(perms)},
+        "permute": {"GB/s": round(perm_bytes / perm_ms / 1e6, 1) if perm_ms else None,
+                    "bytes_per_step": perm_bytes,
+                    "largest": ({"elements": big["elements"], "GB/s": round(big["bytes"] / big["ms"] / 1e6, 1)}
+                                if big else None),
+                    "roofline_frac": round(perm_bytes / perm_ms / 1e6 / PEAK_HBM_GBS, 4) if perm_ms else None},
+        "precision": prec_report,
+        "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": round(achieved_tf, 3),
+                     "peak": PEAK_FP64_TF, "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_FP64_TF, 4),
+                     "traffic": None, "calls_per_step": dom["calls"],
+                     "avg_launch_us": round(dom["ms"] / dom["calls"] * 1e3, 2)},
+        "graph_vs_eager_maxabs": eager_err,
+    }
+
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        os.environ.setdefault("OMP_NUM_THREADS", str(nthreads))
+        init = {}
+        for t in [H.L, H.M1, H.M2, H.R, H.x]:
+            init[t.data.data_ptr()] = t.numpy().copy()
+        reps, t_cpu = 0, 0.0
+        bufs = None
+        while t_cpu < args.cpu_seconds and reps < 50:
+            t0 = time.perf_counter()
+            bufs = cpu_replay(trace, init, nthreads)
+            t_cpu += time.perf_counter() - t0
+            reps += 1
+        ycpu = bufs[y_eager.data.data_ptr()]
+        cpu_err = float(np.linalg.norm(ycpu - y_eager.numpy()) / np.linalg.norm(ycpu))
+        cpu_ms = t_cpu / reps * 1e3
+        out["cpu_baseline"] = {"value": round(flops / cpu_ms / 1e6, 2), "unit": "GFLOP/s", "cores": nthreads,
+                               "kind": "port",
+                               "sample": f"{reps} full H_eff applications (same kernel sequence) on the host, "
+                                         f"{cpu_ms:.1f} ms each; C permute loop with {nthreads} OpenMP threads, "
+                                         f"numpy/OpenBLAS dgemm per sector",
+                               "ms_per_step": round(cpu_ms, 3), "gpu_speedup": round(cpu_ms / ms64, 2),
+                               "rel_err_vs_gpu": cpu_err}
+        log(f"cpu baseline {cpu_ms:.1f} ms/step, GPU speed-up x{cpu_ms/ms64:.1f}, rel err {cpu_err:.2e}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

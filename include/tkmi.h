@@ -125,6 +125,8 @@ int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double 
                    const double *A, const double *B, double *C, void *stream);
 int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile);
 int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops);
+/* Tile edge (64 or 128) and tile count the next exec with `precision` uses.   */
+int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *tile, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
 
 /* ---- hipGraph capture of a chain of launches (one sweep step) ------------ */

@@ -515,6 +515,15 @@ int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops) {
     return TKMI_OK;
 }
 
+int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *tile, int64_t *ntiles) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
+    const int t = plan->choice[precision] ? plan->choice[precision] : auto_tile(plan);
+    if (tile) *tile = t;
+    if (ntiles) *ntiles = (t == 128) ? plan->t128.ntiles : plan->t64.ntiles;
+    return TKMI_OK;
+}
+
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan) {
     free_gemm(plan);
     return TKMI_OK;

@@ -47,6 +47,7 @@ SIGNATURES = [
     ("tkmi_gemm_exec", C.c_int, [c_vp, C.c_int32, C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_gemm_plan_set_tile", C.c_int, [c_vp, C.c_int32, C.c_int32]),
     ("tkmi_gemm_plan_info", C.c_int, [c_vp, c_i64p, c_f64p]),
+    ("tkmi_gemm_plan_query", C.c_int, [c_vp, C.c_int32, c_i32p, c_i64p]),
     ("tkmi_gemm_plan_destroy", C.c_int, [c_vp]),
     ("tkmi_graph_begin", C.c_int, [c_vp]),
     ("tkmi_graph_end", C.c_int, [c_vp, C.POINTER(c_vp)]),

@@ -39,6 +39,27 @@ def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
+# Optional operation recorder: when `_TRACE` is a list, every device launch
+# appends (kind, plan, buffers, α, β) so a caller can replay or account for the
+# exact sequence of kernels a high-level call issued.
+_TRACE = None
+
+
+def start_trace():
+    global _TRACE
+    _TRACE = []
+
+
+def stop_trace():
+    global _TRACE
+    tr, _TRACE = _TRACE, None
+    return tr
+
+
+def _buf(t):
+    return (t.data.data_ptr(), t.data.numel())
+
+
 class TensorMap:
     """Block-sparse tensor: flat device data vector + HomSpace."""
 
@@ -164,6 +185,8 @@ def _exec_flat(dst, src, alpha, beta):
         return
     plan = flat_plan(n)
     lib = _lib.load()
+    if _TRACE is not None:
+        _TRACE.append(("transform", plan, (0,), _buf(src), _buf(dst), float(alpha), float(beta)))
     _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[dst.dtype], src.data.data_ptr(), dst.data.data_ptr(),
                                        float(alpha), float(beta), _stream()), "add!")
 
@@ -206,6 +229,9 @@ def add_transform_(tdst, tsrc, p1, p2, alpha, beta, levels=None, dst_twist=()):
     plan = treebraider(tdst.space, tsrc.space, tuple(p1), tuple(p2), levels, adjoint_src=is_adj,
                        dst_twist=tuple(dst_twist))
     lib = _lib.load()
+    if _TRACE is not None:
+        _TRACE.append(("transform", plan, tuple(p1) + tuple(p2), _buf(tsrc), _buf(tdst), float(alpha),
+                       float(beta)))
     _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(), tdst.data.data_ptr(),
                                        float(alpha), float(beta), _stream()), "add_transform!")
     return tdst
@@ -345,6 +371,7 @@ def gemm_plan(tC, tA, tB, with_beta_groups=True):
                                          cols[9][1], cols[10][1]), "tkmi_gemm_plan_create")
     flops = float(np.sum(2.0 * arr[:, 0] * arr[:, 1] * arr[:, 2])) if len(rows) else 0.0
     plan = GemmPlan(h, flops, len(rows))
+    plan.table = arr
     _GEMM_CACHE[key] = plan
     return plan
 
@@ -358,6 +385,8 @@ def mul_(tC, tA, tB, alpha=1.0, beta=0.0, precision=None):
     plan = gemm_plan(tC, tA, tB)
     prec = DEFAULT_PRECISION if precision is None else precision
     lib = _lib.load()
+    if _TRACE is not None:
+        _TRACE.append(("gemm", plan, int(prec), _buf(tA), _buf(tB), _buf(tC), float(alpha), float(beta)))
     _lib.check(lib.tkmi_gemm_exec(plan.handle, int(prec), float(alpha), float(beta), tA.data.data_ptr(),
                                   tB.data.data_ptr(), tC.data.data_ptr(), _stream()), "mul!")
     return tC

@@ -143,3 +143,52 @@ def test_space_mismatch_raises(T):
         T.permute_(C, A, (2, 0), (1,))
     with pytest.raises(IndexError):
         T.permute_(C, A, (0, 0), (1,))
+
+
+# ---------------------------------------------------------------------------
+# generic (SU(2)) transform: K2 vs the oracle's C loop over the same tables and
+# vs the dense CG embedding
+# ---------------------------------------------------------------------------
+def _su2_legs(W):
+    return ([({k.two_j: n for k, n in V._dims.items()}, V.isdual) for V in W.codomain],
+            [({k.two_j: n for k, n in V._dims.items()}, V.isdual) for V in W.domain])
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_su2_permute_dense(T, k):
+    import itertools
+    from tkmi.spaces import SU2Space
+    from oracle import su2 as S
+    rng = np.random.default_rng(40 + k)
+    V = SU2Space((0, 2), (0.5, 1), (1, 2))
+    U = SU2Space((0.5, 2), (1.5, 1))
+    W = HomSpace(ProductSpace(V, U.dual()), ProductSpace(U, V))
+    A = T.TensorMap.randn(W, rng)
+    dA = S.to_dense(A.numpy(), *_su2_legs(W))
+    for p in list(itertools.permutations(range(4)))[::3]:
+        p1, p2 = p[:k], p[k:]
+        B = T.permute(A, p1, p2, copy=True)
+        dB = S.to_dense(B.numpy(), *_su2_legs(B.space))
+        ref = dA.transpose(p)
+        assert np.linalg.norm(dB - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_su2_permute_vs_c_loop_cfg3_scaled(T):
+    """cfg-3 leg space with degeneracies ÷16: K2 against the oracle's table loop."""
+    from tkmi.spaces import SU2Space
+    from oracle import kernels as OK
+    V = SU2Space((0, 4), (0.5, 6), (1, 6), (1.5, 4), (2, 2))
+    W = HomSpace(ProductSpace(V, V), ProductSpace(V, V))
+    rng = np.random.default_rng(2)
+    A = T.TensorMap.randn(W, rng)
+    x = A.numpy()
+    for p1, p2 in [((0, 2), (1, 3)), ((3, 1), (2, 0)), ((1, 0, 3), (2,))]:
+        B = T.permute(A, p1, p2)
+        plan = T.treebraider(B.space, W, p1, p2)
+        y = np.zeros(B.space.dim())
+        OK.generic_permute(plan.table, p1 + p2, x, y)
+        assert np.linalg.norm(B.numpy() - y) <= 1e-14 * np.linalg.norm(y)
+        # round trip
+        inv = [list(p1 + p2).index(i) for i in range(4)]
+        C = T.permute(B, tuple(inv[:2]), tuple(inv[2:]))
+        assert np.linalg.norm(C.numpy() - x) <= 1e-13 * np.linalg.norm(x)
