@@ -129,6 +129,13 @@ int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops);
 int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *tile, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
 
+/* ---- reductions ------------------------------------------------------------
+ * work[0] = Σ_i a[i]·b[i] over n elements (device result, fixed summation order).
+ * `work` is a caller-owned device buffer of at least 1025 doubles.  Used for
+ * TensorKit's inner(A, B) = Σ_c dim(c)·⟨A_c, B_c⟩ (src/tensors/linalg.jl) in the
+ * reverse-mode rules.                                                          */
+int tkmi_dot(int32_t dtype, const void *a, const void *b, int64_t n, double *work, void *stream);
+
 /* ---- hipGraph capture of a chain of launches (one sweep step) ------------ */
 int tkmi_graph_begin(void *stream);
 int tkmi_graph_end(void *stream, tkmi_graph *graph);

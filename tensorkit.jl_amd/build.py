@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "tkmi", "libtkmi.so")
-SOURCES = ["tkmi_runtime.hip", "tkmi_transform.hip", "tkmi_gemm.hip"]
+SOURCES = ["tkmi_runtime.hip", "tkmi_transform.hip", "tkmi_gemm.hip", "tkmi_reduce.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
