@@ -50,6 +50,7 @@ enum tkmi_precision {
 typedef struct tkmi_transform_plan_s *tkmi_transform_plan;
 typedef struct tkmi_gemm_plan_s *tkmi_gemm_plan;
 typedef struct tkmi_graph_s *tkmi_graph;
+typedef struct tkmi_svd_plan_s *tkmi_svd_plan;
 
 /* ---- runtime ------------------------------------------------------------ */
 int tkmi_init(int device);
@@ -128,6 +129,27 @@ int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops);
 /* Tile edge (64 or 128) and tile count the next exec with `precision` uses.   */
 int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *tile, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
+
+/* ---- svd_trunc (FP64 block factorisation + truncation on the device) -------
+ * Per coupled block A_c (m_c × n_c, column-major at a_off[c] with leading
+ * dimension lda[c]): A_c = U_c·diag(S_c)·Vt_c, r_c = min(m_c, n_c).  Outputs are
+ * packed block after block: U_c (m_c×r_c, ld m_c), S_c (r_c), Vt_c (r_c×n_c,
+ * ld r_c) — the layouts of U: codomain←V_r, Vᴴ: V_r←domain as TensorMaps.
+ * Replaces MAK.svd_compact! per block (src/factorizations/matrixalgebrakit.jl:34-51). */
+int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m, const int64_t *n,
+                         const int64_t *a_off, const int64_t *lda);
+int tkmi_svd_plan_sizes(tkmi_svd_plan plan, int64_t *u_len, int64_t *s_len, int64_t *vt_len);
+int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, double *Vt, void *stream);
+int tkmi_svd_plan_destroy(tkmi_svd_plan plan);
+
+/* Keep-mask for truncrank(howmany) ∩ trunctol(atol) over all singular values
+ * (weights = quantum dimension of each value's sector) and the truncation
+ * error ε = sqrt(Σ_discarded w·σ²) (findtruncated / truncation_error!,
+ * src/factorizations/truncation.jl:171-322).  S, weights, keep, disc_work
+ * (≥ nvals doubles) and err (1 double) are device pointers.                   */
+int tkmi_truncate_select(int64_t nvals, const double *S, const double *weights, double howmany,
+                         double atol, unsigned char *keep, double *disc_work, double *err,
+                         void *stream);
 
 /* ---- reductions ------------------------------------------------------------
  * work[0] = Σ_i a[i]·b[i] over n elements (device result, fixed summation order).

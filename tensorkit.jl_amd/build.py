@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "tkmi", "libtkmi.so")
-SOURCES = ["tkmi_runtime.hip", "tkmi_transform.hip", "tkmi_gemm.hip", "tkmi_reduce.hip"]
+SOURCES = ["tkmi_runtime.hip", "tkmi_transform.hip", "tkmi_gemm.hip", "tkmi_reduce.hip", "tkmi_svd.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
@@ -17,10 +17,14 @@ HIPCC_FLAGS = [
 ]
 
 
+# rocSOLVER/rocBLAS only for the FP64 block SVD of svd_trunc
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+
+
 def build_library(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -49,6 +49,11 @@ SIGNATURES = [
     ("tkmi_gemm_plan_info", C.c_int, [c_vp, c_i64p, c_f64p]),
     ("tkmi_gemm_plan_query", C.c_int, [c_vp, C.c_int32, c_i32p, c_i64p]),
     ("tkmi_gemm_plan_destroy", C.c_int, [c_vp]),
+    ("tkmi_svd_plan_create", C.c_int, [C.POINTER(c_vp), C.c_int64, c_i64p, c_i64p, c_i64p, c_i64p]),
+    ("tkmi_svd_plan_sizes", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
+    ("tkmi_svd_exec", C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("tkmi_svd_plan_destroy", C.c_int, [c_vp]),
+    ("tkmi_truncate_select", C.c_int, [C.c_int64, c_vp, c_vp, C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_dot", C.c_int, [C.c_int32, c_vp, c_vp, C.c_int64, c_vp, c_vp]),
     ("tkmi_graph_begin", C.c_int, [c_vp]),
     ("tkmi_graph_end", C.c_int, [c_vp, C.POINTER(c_vp)]),

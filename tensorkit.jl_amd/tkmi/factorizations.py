@@ -1,0 +1,142 @@
+"""svd_compact / svd_trunc for block-sparse tensors, FP64 on the device.
+
+- svd_compact: per coupled block A_c = U_c S_c Vᴴ_c (`MAK.svd_compact!` per block,
+  src/factorizations/matrixalgebrakit.jl:34-51), one `tkmi_svd_exec` call.
+- svd_trunc: `truncrank(howmany) & trunctol(atol)` selection over all blocks with
+  quantum-dimension weights and the truncation error ε (src/factorizations/
+  truncation.jl:68-83,171-322), computed by `tkmi_truncate_select` on the
+  device; the kept counts per block come back to the host to define the
+  truncated space, then the leading columns/rows are copied on the device.
+
+Singular values are returned as a `SectorVector`-like dict {c: device vector}.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import tensors as T
+from .spaces import GradedSpace, HomSpace, ProductSpace
+from .treetransformers import _make_abelian_plan
+
+
+class SVDPlan:
+    def __init__(self, W):
+        blocks = [(c, d1, d2, off) for c, (d1, d2), off in W.degeneracystructure()[1]]
+        self.blocks = blocks
+        arr = np.array([[d1, d2, off, max(d1, 1)] for c, d1, d2, off in blocks], dtype=np.int64).reshape(-1, 4)
+        lib = _lib.ensure_device()
+        h = C.c_void_p()
+        cols = [_lib.i64(arr[:, i]) for i in range(4)]
+        _lib.check(lib.tkmi_svd_plan_create(C.byref(h), len(blocks), cols[0][1], cols[1][1], cols[2][1],
+                                            cols[3][1]), "tkmi_svd_plan_create")
+        self.handle = h
+        self.r = [min(d1, d2) for _, d1, d2, _ in blocks]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib._lib is not None:
+            _lib._lib.tkmi_svd_plan_destroy(h)
+            self.handle = None
+
+
+_SVD_CACHE = {}
+
+
+def _mid_space(W, counts):
+    """V_r: one leg carrying r_c copies of every coupled sector c."""
+    return GradedSpace({c: n for c, n in counts.items() if n > 0})
+
+
+def svd_compact(t):
+    """(U, S, Vᴴ) with U: codomain ← V_r, S: {c: σ_c} (descending), Vᴴ: V_r ← domain."""
+    W = t.space
+    plan = _SVD_CACHE.get(W)
+    if plan is None:
+        plan = SVDPlan(W)
+        _SVD_CACHE[W] = plan
+    counts = {c: r for (c, _, _, _), r in zip(plan.blocks, plan.r)}
+    Vr = _mid_space(W, counts)
+    U = T.TensorMap.empty(HomSpace(W.codomain, ProductSpace(Vr)))
+    Vt = T.TensorMap.empty(HomSpace(ProductSpace(Vr), W.domain))
+    S = torch.empty(sum(plan.r), dtype=torch.float64, device=T._device())
+    lib = _lib.load()
+    _lib.check(lib.tkmi_svd_exec(plan.handle, t.data.data_ptr(), U.data.data_ptr(), S.data_ptr(),
+                                 Vt.data.data_ptr(), T._stream()), "svd")
+    offs, o = {}, 0
+    for (c, _, _, _), r in zip(plan.blocks, plan.r):
+        offs[c] = (o, r)
+        o += r
+    return U, (S, offs), Vt
+
+
+def _copy_leading(dst, src, sizes_src_dst):
+    """Copy strided leading sub-matrices block by block with one transform plan."""
+    rows_d, rows_s = [], []
+    for (m, n, ld_s, off_s, ld_d, off_d) in sizes_src_dst:
+        if m == 0 or n == 0:
+            continue
+        rows_d.append([m, n, 1, ld_d, off_d])
+        rows_s.append([m, n, 1, ld_s, off_s])
+    if not rows_d:
+        return
+    tab = {"coeffs": np.ones(len(rows_d)), "dst": np.array(rows_d, dtype=np.int64),
+           "src": np.array(rows_s, dtype=np.int64)}
+    plan = _make_abelian_plan(tab, (0, 1), dst.data.numel(), src.data.numel())
+    lib = _lib.load()
+    _lib.check(lib.tkmi_transform_exec(plan.handle, _lib.TKMI_F64, src.data.data_ptr(), dst.data.data_ptr(),
+                                       1.0, 0.0, T._stream()), "truncate copy")
+
+
+def svd_trunc(t, howmany=None, atol=0.0):
+    """`svd_trunc(t; trunc = truncrank(howmany) & trunctol(; atol))`.
+
+    Returns (Ũ, S̃, Ṽᴴ, ε) with S̃ = {c: kept σ_c (device)} and ε the
+    dim(c)-weighted 2-norm of the discarded singular values (a device scalar
+    read back as float)."""
+    U, (S, offs), Vt = svd_compact(t)
+    R = S.numel()
+    weights = torch.empty(R, dtype=torch.float64, device=S.device)
+    w_host = np.zeros(R)
+    for c, (o, r) in offs.items():
+        w_host[o:o + r] = c.dim()
+    weights.copy_(torch.from_numpy(w_host))
+    keep = torch.empty(max(R, 1), dtype=torch.uint8, device=S.device)
+    disc = torch.empty(max(R, 1), dtype=torch.float64, device=S.device)
+    err = torch.empty(1, dtype=torch.float64, device=S.device)
+    hm = float(howmany) if howmany is not None else float("inf")
+    lib = _lib.load()
+    _lib.check(lib.tkmi_truncate_select(R, S.data_ptr(), weights.data_ptr(), hm, float(atol), keep.data_ptr(),
+                                        disc.data_ptr(), err.data_ptr(), T._stream()), "truncate")
+    keep_h = keep[:R].cpu().numpy().astype(bool)
+    kept = {}
+    for c, (o, r) in offs.items():
+        k = int(keep_h[o:o + r].sum())
+        # σ_c are sorted descending, so the kept set of a block is its leading k values
+        if k and not keep_h[o:o + k].all():
+            raise RuntimeError("truncation kept a non-leading singular value (ties)")
+        kept[c] = k
+    W = t.space
+    Vk = _mid_space(W, kept)
+    Ut = T.TensorMap.empty(HomSpace(W.codomain, ProductSpace(Vk)))
+    Vtt = T.TensorMap.empty(HomSpace(ProductSpace(Vk), W.domain))
+    uS = U.space.degeneracystructure()[1]
+    vS = Vt.space.degeneracystructure()[1]
+    uD = {c: (d, off) for c, d, off in Ut.space.degeneracystructure()[1]}
+    vD = {c: (d, off) for c, d, off in Vtt.space.degeneracystructure()[1]}
+    ucopy, vcopy = [], []
+    for c, (d1, r), off in uS:
+        if kept[c]:
+            (dd1, k), doff = uD[c]
+            ucopy.append((d1, kept[c], d1, off, dd1, doff))
+    for c, (r, d2), off in vS:
+        if kept[c]:
+            (k, dd2), doff = vD[c]
+            vcopy.append((kept[c], d2, r, off, k, doff))
+    _copy_leading(Ut, U, ucopy)
+    _copy_leading(Vtt, Vt, vcopy)
+    Sk = {c: S[o:o + kept[c]] for c, (o, r) in offs.items() if kept[c]}
+    return Ut, Sk, Vtt, float(err.item())
