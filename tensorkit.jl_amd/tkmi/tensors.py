@@ -273,13 +273,37 @@ def braid(t, p1, p2, levels, copy=False):
     return braid_(tdst, t, p1, p2, levels)
 
 
+def _iscyclic(p):
+    n = len(p)
+    return n == 0 or all(p[(i + 1) % n] == (p[i] + 1) % n for i in range(n))
+
+
 def transpose(t, p1=None, p2=None):
-    """Planar (cyclic) permutation; for symmetric braiding its coefficients
-    coincide with `permute` on the same cyclic index order."""
+    """`transpose(t, (p₁, p₂))` (`indexmanipulations.jl:380-392`): a planar
+    (cyclic, in the linearised leg order) repartition.  With bosonic braiding
+    the planar coefficients coincide with `permute`'s, so the same device
+    transform is used; fermionic transposes (fold moves with twists) are not
+    provided."""
     if p1 is None:
         p1 = tuple(reversed(t.domainind()))
         p2 = tuple(reversed(t.codomainind()))
+    p1, p2 = tuple(p1), tuple(p2)
+    n1, n2 = t.numout, t.numin
+    lin = [x if x < n1 else n2 + 2 * n1 - 1 - x for x in p1] + \
+          [x if x < n1 else n2 + 2 * n1 - 1 - x for x in reversed(p2)]
+    if not _iscyclic(lin):
+        raise ValueError(f"{(p1, p2)} is not a cyclic permutation for {t.space}")
+    if t.space.unit.braiding_style == FERMIONIC:
+        raise NotImplementedError("planar transpose of fermionic tensors")
     return permute(t, p1, p2, copy=True)
+
+
+def planarcontract_(C, A, pA, B, pB, pAB, alpha=1.0, beta=0.0, precision=None):
+    """`planarcontract!` (`src/planar/planaroperations.jl:150-190`): bosonic
+    sectors forward to `contract!`."""
+    if C.space.unit.braiding_style != "bosonic":
+        raise NotImplementedError("planarcontract! for non-bosonic sectors")
+    return contract_(C, A, pA, B, pB, pAB, alpha, beta, precision)
 
 
 def twist_(t, inds):
