@@ -14,7 +14,9 @@
 // MFMA lane layouts used (wave64, i = row, j = col, l = lane):
 //   v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32:
 //       A[i][k]: i = l%16, k = l/16;  B[k][j]: j = l%16, k = l/16
-//       D[i][j], 4 values v per lane:  i = 4*(l/16) + v, j = l%16
+//       D[i][j], 4 values v per lane, j = l%16 and
+//         f32: i = 4*(l/16) + v      (4 consecutive rows per lane)
+//         f64: i = (l/16) + 4*v      (rows interleaved by 4)
 //   v_mfma_f32_16x16x32_bf16: A[i][k], 8 values e per lane: i = l%16, k = 8*(l/16) + e
 //       (B likewise with j), D as above.
 #include "tkmi_internal.h"
@@ -74,6 +76,7 @@ struct MfmaK4<double> {
     __device__ static inline acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct MfmaK4<float> {
@@ -81,6 +84,7 @@ struct MfmaK4<float> {
     __device__ static inline acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 template <typename TC, int BM, int BN>
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
             if (col >= n) continue;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t row = row0 + wm * WM + i * 16 + 4 * (lane >> 4) + v;
+                const int64_t row = row0 + wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
                 if (row < m) {
                     double r = alpha * (double)acc[i][j][v];
                     double *p = Cg + row + col * ldc;

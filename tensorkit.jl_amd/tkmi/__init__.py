@@ -20,10 +20,3 @@ __all__ = [
     "treebraider", "empty_globalcaches", "global_cache_info",
 ]
 
-
-def __getattr__(name):
-    # device-side API (imports torch lazily so host-only use stays light)
-    if name in ("tensors",):
-        from . import tensors
-        return tensors
-    raise AttributeError(name)
