@@ -62,7 +62,8 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
     gemms, perms = [], []
     for op in trace:
         if op[0] == "gemm":
-            _, plan, prec, (pa, _), (pb, _), (pc, nc), a, b = op
+            _, plan, prec, (ta_, _), (tb_, _), (tc_, nc), a, b = op
+            pa, pb, pc = ta_.data_ptr(), tb_.data_ptr(), tc_.data_ptr()
             tile, nt = C.c_int32(), C.c_int64()
             L.tkmi_gemm_plan_query(plan.handle, prec, C.byref(tile), C.byref(nt))
             fn = lambda plan=plan, prec=prec, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec(
@@ -71,7 +72,8 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
             gemms.append({"flops": plan.flops, "ms": ms, "kernel": gemm_kernel_name(prec, tile.value),
                           "groups": plan.ngroups, "tiles": nt.value})
         else:
-            _, plan, perm, (ps, ns), (pd, nd), a, b = op
+            _, plan, perm, (ts_, ns), (td_, nd), a, b = op
+            ps, pd = ts_.data_ptr(), td_.data_ptr()
             fn = lambda plan=plan, ps=ps, pd=pd, a=a, b=b: L.tkmi_transform_exec(plan.handle, 0, ps, pd, a, b, s)
             ms = time_launches(fn, reps, torch)
             perms.append({"elements": nd, "bytes": nd * (24 if b != 0 else 16), "ms": ms,
@@ -86,8 +88,9 @@ def cpu_replay(trace, init, nthreads):
     from oracle import kernels as OK
     bufs = {}
 
-    def get(ptrn):
-        p, n = ptrn
+    def get(tn):
+        t, n = tn
+        p = t.data_ptr()
         if p not in bufs:
             bufs[p] = init[p].copy() if p in init else np.zeros(n)
         return bufs[p]

@@ -83,12 +83,16 @@ int tkmi_abelian_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
  * concatenated in `U`), the common sub-block sizes (N int64, sz_dst / sz_src)
  * and per tree (strides[N], offset) (N+1 int64, concatenated per block).
  * The kernel computes dst_j = β dst_j + α Σ_i U_b[j,i] permute(src_i, perm)
- * for every block, i.e. pack → src·Uᵀ → unpack fused in one launch.         */
+ * for every block, i.e. pack → src·Uᵀ → unpack fused in one launch.
+ * w_dst / w_src: per-tree weights dim(c) of the tree's coupled sector (NULL = 1);
+ * they only enter the adjoint transform, which is the adjoint with respect to
+ * TensorKit's inner product Σ_c dim(c)⟨·,·⟩ (= the inverse permutation).       */
 int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const int32_t *perm,
                              int64_t nblocks, const int64_t *rows, const int64_t *cols,
                              const double *U, const int64_t *sz_dst, const int64_t *sz_src,
                              const int64_t *strides_offsets_dst,
                              const int64_t *strides_offsets_src,
+                             const double *w_dst, const double *w_src,
                              int64_t dst_len, int64_t src_len);
 
 /* dst = β·dst + α·transform(src).  dtype: TKMI_F64 or TKMI_F32 (both buffers).
@@ -97,10 +101,10 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
 int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
                         double alpha, double beta, void *stream);
 
-/* Same transform with the adjoint table: src += conj(α)·transformᵀ(dst-shaped
- * cotangent), i.e. the pullback of tkmi_transform_exec (inverse permutation,
- * transposed recoupling matrices), used by the permute/braid rrules
- * (src/pullbacks/indexmanipulations.jl:1-31).                               */
+/* Adjoint transform: dsrc = β·dsrc + α·transform†(ddst), the adjoint with
+ * respect to Σ_c dim(c)⟨·,·⟩, i.e. the pullback of tkmi_transform_exec and
+ * the inverse permutation (W⁻¹-weighted transposed recoupling matrices), used
+ * by the permute/braid rrules (src/pullbacks/indexmanipulations.jl:1-31).    */
 int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const void *ddst,
                                 void *dsrc, double alpha, double beta, void *stream);
 

@@ -523,8 +523,8 @@ int tkmi_abelian_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
 
 int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const int32_t *perm, int64_t nblocks,
                              const int64_t *rows, const int64_t *cols, const double *U, const int64_t *sz_dst,
-                             const int64_t *sz_src, const int64_t *so_dst, const int64_t *so_src, int64_t dst_len,
-                             int64_t src_len) {
+                             const int64_t *sz_src, const int64_t *so_dst, const int64_t *so_src,
+                             const double *w_dst, const double *w_src, int64_t dst_len, int64_t src_len) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_generic_plan_create: null plan pointer");
     *plan = nullptr;
     int rc = check_perm(numind, perm);
@@ -577,8 +577,12 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
         const int64_t ustart = (int64_t)p->U.size();
         for (int64_t i = 0; i < R * C; ++i) p->U.push_back(U[uoff + i]);
         p->Ut.resize(p->U.size());
+        // adjoint w.r.t. the dim(c)-weighted inner product: Ut = W_src⁻¹ Uᵀ W_dst
         for (int64_t i = 0; i < C; ++i)
-            for (int64_t j = 0; j < R; ++j) p->Ut[ustart + i + j * C] = U[uoff + j + i * R];
+            for (int64_t j = 0; j < R; ++j) {
+                const double wd = w_dst ? w_dst[tdst + j] : 1.0, ws = w_src ? w_src[tsrc + i] : 1.0;
+                p->Ut[ustart + i + j * C] = U[uoff + j + i * R] * wd / ws;
+            }
         add_item(p, items, sizes, strs, offs, (int)C, (int)R, ustart);
         uoff += R * C;
         tsrc += C;

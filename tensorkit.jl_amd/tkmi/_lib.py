@@ -36,7 +36,7 @@ SIGNATURES = [
      [C.POINTER(c_vp), C.c_int32, c_i32p, C.c_int64, c_f64p, c_i64p, c_i64p, C.c_int64, C.c_int64]),
     ("tkmi_generic_plan_create", C.c_int,
      [C.POINTER(c_vp), C.c_int32, c_i32p, C.c_int64, c_i64p, c_i64p, c_f64p, c_i64p, c_i64p,
-      c_i64p, c_i64p, C.c_int64, C.c_int64]),
+      c_i64p, c_i64p, c_f64p, c_f64p, C.c_int64, C.c_int64]),
     ("tkmi_transform_exec", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, C.c_double, C.c_double, c_vp]),
     ("tkmi_transform_adjoint_exec", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, C.c_double, C.c_double, c_vp]),
     ("tkmi_transform_plan_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p]),

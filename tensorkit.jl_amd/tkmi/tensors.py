@@ -57,7 +57,9 @@ def stop_trace():
 
 
 def _buf(t):
-    return (t.data.data_ptr(), t.data.numel())
+    # the recorded entry holds the device tensor itself, so a recorded buffer
+    # stays allocated (and its address valid) for as long as the trace lives
+    return (t.data, t.data.numel())
 
 
 class TensorMap:

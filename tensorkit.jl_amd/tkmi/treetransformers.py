@@ -180,11 +180,15 @@ def _make_generic_plan(rows, perm, dst_len, src_len):
     sos = np.array([x for r in rows for x in r[5]], dtype=np.int64).reshape(-1)
     h = _lib.C.c_void_p()
     pa, pp = _lib.i32(perm)
+    # per-tree weights dim(c) for the adjoint (weighted-inner-product) transform
+    wd = np.array([t[0].coupled.dim() for r in rows for t in r[7].trees], dtype=np.float64)
+    ws = np.array([t[0].coupled.dim() for r in rows for t in r[6].trees], dtype=np.float64)
     keep = [_lib.i64(rws), _lib.i64(cls), _lib.f64(U), _lib.i64(szd), _lib.i64(szs), _lib.i64(sod),
-            _lib.i64(sos)]
+            _lib.i64(sos), _lib.f64(wd), _lib.f64(ws)]
     _lib.check(lib.tkmi_generic_plan_create(_lib.C.byref(h), n, pp, len(rows), keep[0][1], keep[1][1],
                                             keep[2][1], keep[3][1], keep[4][1], keep[5][1], keep[6][1],
-                                            int(dst_len), int(src_len)), "tkmi_generic_plan_create")
+                                            keep[7][1], keep[8][1], int(dst_len), int(src_len)),
+               "tkmi_generic_plan_create")
     return TransformPlan(h, "generic", len(rows), None, rows)
 
 

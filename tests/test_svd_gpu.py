@@ -74,7 +74,8 @@ def _svd_checks(T, t, howmany, atol):
         else:
             R = A
         d2 += c.dim() * np.sum(R * R)
-    assert abs(np.sqrt(d2) - eps) <= 1e-9 * max(eps, 1e-300) + 1e-12
+    tnorm = np.sqrt(sum(c.dim() * np.sum(A * A) for c, A in Ab.items()))
+    assert abs(np.sqrt(d2) - eps) <= 1e-9 * eps + 1e-13 * tnorm
 
 
 def test_svd_trunc_u1(T):
