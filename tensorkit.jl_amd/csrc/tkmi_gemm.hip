@@ -34,7 +34,13 @@ struct GGroup {
     int64_t a_off, lda, b_off, ldb, c_off, ldc;
     int32_t ta, tb;
     int64_t tiles_m, tiles_n, tile_start;
+    int32_t skinny_mode;  // skinny kernel only: 0 = long along m, 1 = long along n
+    int32_t pad_;
 };
+
+// Groups whose smaller output dimension is at most SKINNY go to the
+// bandwidth-bound skinny kernel (they would leave most of an MFMA tile idle).
+constexpr int64_t SKINNY = 32;
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -97,9 +103,12 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int AE = BM * BK / NT;  // A elements per thread per slab
     constexpr int BE = BN * BK / NT;
-    constexpr int PAD = 0;
-    __shared__ TC As[2][BK][BM + PAD];
-    __shared__ TC Bs[2][BK][BN + PAD];
+    // LDS rows hold the K slab of one output row/column contiguously; the pad makes the
+    // row pitch an odd multiple of 16 bytes, so the 16-byte reads of 16 lanes (16 rows)
+    // fall in distinct banks.
+    constexpr int KP = BK + 16 / (int)sizeof(TC);
+    __shared__ __attribute__((aligned(16))) TC As[2][BM][KP];
+    __shared__ __attribute__((aligned(16))) TC Bs[2][BN][KP];
     using acc_t = typename MfmaK4<TC>::acc_t;
 
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
@@ -108,8 +117,6 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
     const int64_t tm = lt / G.tiles_n, tn = lt % G.tiles_n;
     const int64_t m = G.m, n = G.n, K = G.k;
     const int64_t row0 = tm * BM, col0 = tn * BN;
-    const double *Ag = A + G.a_off;
-    const double *Bg = Bm + G.b_off;
     const int64_t lda = G.lda, ldb = G.ldb;
     const int ta = G.ta, tb = G.tb;
 
@@ -123,47 +130,50 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
+    // Per-thread element coordinates of the A and B slabs (fixed over K): global order is
+    // coalesced (contiguous index fastest), the LDS position is [row][k].
+    int ai[AE], al[AE], bj[BE], bl[BE];
+    const double *ap[AE];
+    const double *bp[BE];
+    bool aok[AE], bok[BE];
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+        const int idx = e * NT + tid;
+        if (!ta) { ai[e] = idx % BM; al[e] = idx / BM; }
+        else     { al[e] = idx % BK; ai[e] = idx / BK; }
+        const int64_t gi = row0 + ai[e];
+        aok[e] = gi < m;
+        ap[e] = A + G.a_off + (ta ? (al[e] + gi * lda) : (gi + al[e] * lda));
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+        const int idx = e * NT + tid;
+        if (!tb) { bl[e] = idx % BK; bj[e] = idx / BK; }
+        else     { bj[e] = idx % BN; bl[e] = idx / BN; }
+        const int64_t gj = col0 + bj[e];
+        bok[e] = gj < n;
+        bp[e] = Bm + G.b_off + (tb ? (gj + bl[e] * ldb) : (bl[e] + gj * ldb));
+    }
+    const int64_t astep = ta ? BK : BK * lda;  // pointer advance per K slab
+    const int64_t bstep = tb ? BK * ldb : BK;
+
     double ra[AE], rb[BE];
-    // element e of this thread in the A slab: (i, l) with coalesced global order
-    auto a_coord = [&](int e, int &i, int &l) {
-        const int idx = e * NT + tid;
-        if (!ta) { i = idx % BM; l = idx / BM; }   // column-major: i contiguous
-        else     { l = idx % BK; i = idx / BK; }   // transposed: l contiguous
-    };
-    auto b_coord = [&](int e, int &l, int &j) {
-        const int idx = e * NT + tid;
-        if (!tb) { l = idx % BK; j = idx / BK; }   // column-major k×n: l contiguous
-        else     { j = idx % BN; l = idx / BN; }   // transposed: j contiguous
-    };
     auto load_slab = [&](int64_t k0) {
+        const bool full = (k0 + BK <= K);
 #pragma unroll
-        for (int e = 0; e < AE; ++e) {
-            int i, l;
-            a_coord(e, i, l);
-            const int64_t gi = row0 + i, gl = k0 + l;
-            ra[e] = (gi < m && gl < K) ? (ta ? Ag[gl + gi * lda] : Ag[gi + gl * lda]) : 0.0;
-        }
+        for (int e = 0; e < AE; ++e) ra[e] = (aok[e] && (full || k0 + al[e] < K)) ? *ap[e] : 0.0;
 #pragma unroll
-        for (int e = 0; e < BE; ++e) {
-            int l, j;
-            b_coord(e, l, j);
-            const int64_t gj = col0 + j, gl = k0 + l;
-            rb[e] = (gj < n && gl < K) ? (tb ? Bg[gj + gl * ldb] : Bg[gl + gj * ldb]) : 0.0;
-        }
+        for (int e = 0; e < BE; ++e) rb[e] = (bok[e] && (full || k0 + bl[e] < K)) ? *bp[e] : 0.0;
+#pragma unroll
+        for (int e = 0; e < AE; ++e) ap[e] += astep;
+#pragma unroll
+        for (int e = 0; e < BE; ++e) bp[e] += bstep;
     };
     auto store_slab = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < AE; ++e) {
-            int i, l;
-            a_coord(e, i, l);
-            As[buf][l][i] = (TC)ra[e];
-        }
+        for (int e = 0; e < AE; ++e) As[buf][ai[e]][al[e]] = (TC)ra[e];
 #pragma unroll
-        for (int e = 0; e < BE; ++e) {
-            int l, j;
-            b_coord(e, l, j);
-            Bs[buf][l][j] = (TC)rb[e];
-        }
+        for (int e = 0; e < BE; ++e) Bs[buf][bj[e]][bl[e]] = (TC)rb[e];
     };
 
     const int64_t nk = (K + BK - 1) / BK;
@@ -172,20 +182,33 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
         store_slab(0);
     }
     __syncthreads();
+    // MFMA s of a K slab uses k = 4*(lane/16) + s for both operands: each lane reads
+    // its 4 k values of a row as one contiguous 4-element vector.
+    const int kq = 4 * (lane >> 4);
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int buf = (int)(kt & 1);
         if (kt + 1 < nk) load_slab((kt + 1) * BK);
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            TC af[FM], bfr[FN];
+        for (int kk = 0; kk < BK; kk += 16) {
+            TC af[FM][4], bfr[FN][4];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+            for (int i = 0; i < FM; ++i) {
+                const TC *src = &As[buf][wm * WM + i * 16 + (lane & 15)][kk + kq];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) bfr[j] = Bs[buf][kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+                for (int s = 0; s < 4; ++s) af[i][s] = src[s];
+            }
 #pragma unroll
-            for (int i = 0; i < FM; ++i)
+            for (int j = 0; j < FN; ++j) {
+                const TC *src = &Bs[buf][wn * WN + j * 16 + (lane & 15)][kk + kq];
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i], bfr[j], acc[i][j]);
+                for (int s = 0; s < 4; ++s) bfr[j][s] = src[s];
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i][s], bfr[j][s], acc[i][j]);
         }
         if (kt + 1 < nk) store_slab(buf ^ 1);
         __syncthreads();
@@ -212,6 +235,7 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
             }
         }
 }
+
 
 // --------------------------------------------------------------------------------
 // 3×BF16 split-product kernel: x = x0 + x1 + x2 (bf16 each), products
@@ -374,6 +398,80 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
         }
 }
 
+
+// --------------------------------------------------------------------------------
+// Skinny groups (min(m, n) ≤ 32): one thread owns one index of the long output
+// dimension and accumulates its whole short row/column in FP64 registers; the small
+// operand is staged through LDS in 32-wide K chunks.  These products move
+// O(long·(k + short)) bytes for O(long·k·short) flops, i.e. they are HBM-bound, so
+// FMA in FP64 is used for every precision form.
+// --------------------------------------------------------------------------------
+template <int SK>
+__global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                         const double *__restrict__ A,
+                                                         const double *__restrict__ Bm, double *__restrict__ C,
+                                                         double alpha, double beta) {
+    constexpr int KC = 32;
+    __shared__ double Sm[KC][SK + 1];
+    const int64_t tile = (int64_t)blockIdx.x;
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t m = G.m, n = G.n, K = G.k;
+    const double *Ag = A + G.a_off;
+    const double *Bg = Bm + G.b_off;
+    const int64_t lda = G.lda, ldb = G.ldb, ldc = G.ldc;
+    const int ta = G.ta, tb = G.tb;
+    const bool longm = (G.skinny_mode == 0);
+    const int64_t L = longm ? m : n;          // long dimension
+    const int S = (int)(longm ? n : m);       // short dimension (≤ SK)
+    const int64_t x = lt * NT + threadIdx.x;  // this thread's long index
+    double acc[SK];
+#pragma unroll
+    for (int q = 0; q < SK; ++q) acc[q] = 0.0;
+
+    for (int64_t k0 = 0; k0 < K; k0 += KC) {
+        const int kc = (int)min((int64_t)KC, K - k0);
+        // stage the short operand: longm -> op(B)[k0+l][q] ; else op(A)[q][k0+l]
+        for (int e = threadIdx.x; e < KC * SK; e += NT) {
+            const int l = e / SK, q = e % SK;
+            double v = 0.0;
+            if (l < kc && q < S) {
+                const int64_t gl = k0 + l;
+                if (longm)
+                    v = tb ? Bg[q + gl * ldb] : Bg[gl + (int64_t)q * ldb];
+                else
+                    v = ta ? Ag[gl + (int64_t)q * lda] : Ag[q + gl * lda];
+            }
+            Sm[l][q] = v;
+        }
+        __syncthreads();
+        if (x < L) {
+            for (int l = 0; l < kc; ++l) {
+                const int64_t gl = k0 + l;
+                double v;
+                if (longm)
+                    v = ta ? Ag[gl + x * lda] : Ag[x + gl * lda];
+                else
+                    v = tb ? Bg[x + gl * ldb] : Bg[gl + x * ldb];
+#pragma unroll
+                for (int q = 0; q < SK; ++q) acc[q] = fma(v, Sm[l][q], acc[q]);
+            }
+        }
+        __syncthreads();
+    }
+    if (x >= L) return;
+    const bool has_beta = (beta != 0.0);
+    double *Cg = C + G.c_off;
+#pragma unroll
+    for (int q = 0; q < SK; ++q) {
+        if (q < S) {
+            double *pc = longm ? (Cg + x + (int64_t)q * ldc) : (Cg + q + x * ldc);
+            const double r = alpha * acc[q];
+            *pc = has_beta ? r + beta * *pc : r;
+        }
+    }
+}
+
 struct TileSet {
     int tile = 0;  // 64 or 128
     std::vector<GGroup> groups;
@@ -383,8 +481,16 @@ struct TileSet {
 
 }  // namespace
 
+struct SkinnySet {
+    std::vector<GGroup> groups;
+    int64_t ntiles = 0;
+    int sk = 0;  // register width (8, 16 or 32)
+    GGroup *d_groups = nullptr;
+};
+
 struct tkmi_gemm_plan_s {
-    std::vector<GGroup> base;  // groups without tiling
+    std::vector<GGroup> base;  // MFMA groups without tiling
+    SkinnySet skinny;
     double flops = 0;
     TileSet t64, t128;
     int choice[3] = {0, 0, 0};  // per precision: 0 auto
@@ -433,8 +539,24 @@ int launch(const TileSet &ts, int prec, const double *A, const double *B, double
     return TKMI_OK;
 }
 
+int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double *C, double alpha, double beta,
+                  hipStream_t s) {
+    if (ss.ntiles == 0) return TKMI_OK;
+    dim3 grid((unsigned)ss.ntiles), block(NT);
+    const int64_t ng = (int64_t)ss.groups.size();
+    if (ss.sk <= 8)
+        hipLaunchKernelGGL(gemm_skinny_kernel<8>, grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
+    else if (ss.sk <= 16)
+        hipLaunchKernelGGL(gemm_skinny_kernel<16>, grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
+    else
+        hipLaunchKernelGGL(gemm_skinny_kernel<32>, grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
+    TKMI_HIP(hipGetLastError());
+    return TKMI_OK;
+}
+
 void free_gemm(tkmi_gemm_plan_s *p) {
     if (!p) return;
+    (void)hipFree(p->skinny.d_groups);
     (void)hipFree(p->t64.d_groups);
     (void)hipFree(p->t128.d_groups);
     delete p;
@@ -478,7 +600,27 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
             }
         }
         p->flops += 2.0 * (double)G.m * (double)G.n * (double)G.k;
-        p->base.push_back(G);
+        if (G.m == 0 || G.n == 0) continue;
+        if (std::min(G.m, G.n) <= SKINNY) {
+            G.skinny_mode = (G.n <= G.m) ? 0 : 1;
+            const int64_t L = G.skinny_mode == 0 ? G.m : G.n;
+            G.tiles_m = (L + NT - 1) / NT;
+            G.tiles_n = 1;
+            G.tile_start = p->skinny.ntiles;
+            p->skinny.ntiles += G.tiles_m;
+            p->skinny.sk = (int)std::max<int64_t>(p->skinny.sk, std::min(G.m, G.n));
+            p->skinny.groups.push_back(G);
+        } else {
+            p->base.push_back(G);
+        }
+    }
+    if (!p->skinny.groups.empty()) {
+        int rc0 = tkmi::upload((void **)&p->skinny.d_groups, p->skinny.groups.data(),
+                               p->skinny.groups.size() * sizeof(GGroup));
+        if (rc0) {
+            free_gemm(p);
+            return rc0;
+        }
     }
     // largest groups first: their tiles start earliest
     std::stable_sort(p->base.begin(), p->base.end(), [](const GGroup &a, const GGroup &b) {
@@ -508,6 +650,8 @@ int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double 
     if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
     int tile = plan->choice[precision] ? plan->choice[precision] : auto_tile(plan);
     hipStream_t s = tkmi::as_stream(stream);
+    int rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
+    if (rc) return rc;
     if (tile == 128) return launch<128>(plan->t128, precision, A, B, C, alpha, beta, s);
     return launch<64>(plan->t64, precision, A, B, C, alpha, beta, s);
 }
