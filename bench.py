@@ -36,9 +36,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gemm_kernel_name(prec, tile):
-    return {0: f"gemm_k4_kernel<double, {tile}, {tile}>", 1: f"gemm_k4_kernel<float, {tile}, {tile}>",
-            2: f"gemm_bf16x3_kernel<{tile}, {tile}>"}[prec]
+def gemm_kernel_name(prec, cfg, L):
+    import ctypes as C
+    bm, bn, bk = C.c_int32(), C.c_int32(), C.c_int32()
+    L.tkmi_gemm_config_info(cfg, C.byref(bm), C.byref(bn), C.byref(bk))
+    bm, bn, bk = bm.value, bn.value, bk.value
+    return {0: f"gemm_k4_kernel<double, {bm}, {bn}, {bk}>", 1: f"gemm_k4_kernel<float, {bm}, {bn}, {bk}>",
+            2: f"gemm_bf16x3_kernel<{bm}, {bn}>"}[prec]
 
 
 def time_launches(fn, reps, torch):
@@ -55,7 +59,9 @@ def time_launches(fn, reps, torch):
 
 
 def kernel_breakdown(trace, T, lib, torch, reps=20):
-    """Per-launch timing of every kernel in one chain (eager re-issue)."""
+    """Per-launch timing of every kernel in one chain (eager re-issue on the
+    current stream); a grouped GEMM is timed as its MFMA launch and its skinny
+    (bandwidth-bound) launch separately."""
     import ctypes as C
     L = lib.load()
     s = torch.cuda.current_stream().cuda_stream
@@ -66,13 +72,20 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
             pa, pb, pc = ta_.data_ptr(), tb_.data_ptr(), tc_.data_ptr()
             tile, nt = C.c_int32(), C.c_int64()
             L.tkmi_gemm_plan_query(plan.handle, prec, C.byref(tile), C.byref(nt))
-            fn = lambda plan=plan, prec=prec, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec(
-                plan.handle, prec, a, b, pa, pb, pc, s)
-            ms = time_launches(fn, reps, torch)
-            gemms.append({"flops": plan.flops, "ms": ms, "kernel": gemm_kernel_name(prec, tile.value),
-                          "groups": plan.ngroups, "tiles": nt.value})
+            fm, fs, ns = C.c_double(), C.c_double(), C.c_int64()
+            L.tkmi_gemm_plan_split(plan.handle, C.byref(fm), C.byref(fs), C.byref(ns))
+            for part, flops, name in ((1, fm.value, gemm_kernel_name(prec, tile.value, L)),
+                                      (2, fs.value, "gemm_skinny_kernel")):
+                if flops == 0 and not (part == 2 and ns.value > 0):
+                    continue
+                if part == 1 and nt.value == 0:
+                    continue
+                fn = lambda plan=plan, prec=prec, part=part, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
+                    plan.handle, prec, part, a, b, pa, pb, pc, s)
+                ms = time_launches(fn, reps, torch)
+                gemms.append({"flops": flops, "ms": ms, "kernel": name, "tiles": nt.value if part == 1 else None})
         else:
-            _, plan, perm, (ts_, ns), (td_, nd), a, b = op
+            _, plan, perm, (ts_, ns_), (td_, nd), a, b = op
             ps, pd = ts_.data_ptr(), td_.data_ptr()
             fn = lambda plan=plan, ps=ps, pd=pd, a=a, b=b: L.tkmi_transform_exec(plan.handle, 0, ps, pd, a, b, s)
             ms = time_launches(fn, reps, torch)
@@ -158,6 +171,11 @@ def main():
     log(f"chain: {sum(1 for o in trace if o[0]=='gemm')} grouped GEMM launches, {ngemm_groups} groups, "
         f"{flops/1e9:.3f} GFLOP, {sum(1 for o in trace if o[0]=='transform')} permutes")
 
+    # tile-shape autotuning of every grouped GEMM (per precision form), then recompute
+    T.autotune_trace(trace, precisions=(_lib.PREC_F64, _lib.PREC_F32, _lib.PREC_BF16X3))
+    y_eager = H.apply()
+    torch.cuda.synchronize()
+
     # FP64 graph
     y64 = H.capture(precision=_lib.PREC_F64)
     for _ in range(args.warmup):
@@ -208,7 +226,8 @@ def main():
         d["flops"] += g["flops"]
         d["ms"] += g["ms"]
         d["calls"] += 1
-    dom_name, dom = max(by_kernel.items(), key=lambda kv: kv[1]["ms"])
+    mfma_kernels = {k: v for k, v in by_kernel.items() if k != "gemm_skinny_kernel"}
+    dom_name, dom = max(mfma_kernels.items(), key=lambda kv: kv[1]["ms"])
     achieved_tf = dom["flops"] / dom["ms"] / 1e9
     perm_bytes = sum(p["bytes"] for p in perms)
     perm_ms = sum(p["ms"] for p in perms)

@@ -125,13 +125,29 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
                           const int64_t *lda, const int32_t *transa, const int64_t *b_off,
                           const int64_t *ldb, const int32_t *transb, const int64_t *c_off,
                           const int64_t *ldc);
-/* tile: 0 = automatic (plan's autotuned choice), else 64 or 128 */
 int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta,
                    const double *A, const double *B, double *C, void *stream);
+/* Tile shape for `precision`: 0 = automatic, 64 / 128 = square tiles with a
+ * 16-deep K slab, 1000 + i = configuration i (see tkmi_gemm_config_info).      */
 int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile);
+int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk);
+/* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
+ * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64), bit 1 = the
+ * bandwidth-bound skinny groups; parts = 3 is tkmi_gemm_exec.  Used to time the
+ * two kernels separately.                                                     */
+int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alpha,
+                         double beta, const double *A, const double *B, double *C, void *stream);
+/* Algorithmic flops routed to the MFMA kernel and to the skinny kernel.       */
+int tkmi_gemm_plan_split(tkmi_gemm_plan plan, double *flops_mfma, double *flops_skinny,
+                         int64_t *skinny_groups);
 int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops);
-/* Tile edge (64 or 128) and tile count the next exec with `precision` uses.   */
-int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *tile, int64_t *ntiles);
+/* Autotuner: times every tile shape of the plan on the given operands (α=1,
+ * β=0: C is overwritten) and keeps the fastest for `precision`.  Synchronises
+ * `stream`; meant for warm-up, not for capture.                               */
+int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double *A, const double *B,
+                            double *C, int32_t reps, void *stream);
+/* Configuration index and tile count the next exec with `precision` uses.      */
+int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *config, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
 
 /* ---- svd_trunc (FP64 block factorisation + truncation on the device) -------

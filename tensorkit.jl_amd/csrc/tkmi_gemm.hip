@@ -38,9 +38,10 @@ struct GGroup {
     int32_t pad_;
 };
 
-// Groups whose smaller output dimension is at most SKINNY go to the
-// bandwidth-bound skinny kernel (they would leave most of an MFMA tile idle).
+// Groups whose smaller output dimension is at most SKINNY (and K at most SKINNY_K) go
+// to the bandwidth-bound skinny kernel (they would leave most of an MFMA tile idle).
 constexpr int64_t SKINNY = 32;
+constexpr int64_t SKINNY_K = 64;  // and a short K: otherwise one thread's serial K loop dominates
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -93,12 +94,11 @@ struct MfmaK4<float> {
     __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <typename TC, int BM, int BN>
+template <typename TC, int BM, int BN, int BK>
 __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
                                                      int64_t ntiles, const double *__restrict__ A,
                                                      const double *__restrict__ Bm, double *__restrict__ C,
                                                      double alpha, double beta) {
-    constexpr int BK = 16;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int AE = BM * BK / NT;  // A elements per thread per slab
@@ -473,13 +473,10 @@ __global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restric
 }
 
 struct TileSet {
-    int tile = 0;  // 64 or 128
     std::vector<GGroup> groups;
     int64_t ntiles = 0;
     GGroup *d_groups = nullptr;
 };
-
-}  // namespace
 
 struct SkinnySet {
     std::vector<GGroup> groups;
@@ -488,24 +485,35 @@ struct SkinnySet {
     GGroup *d_groups = nullptr;
 };
 
+// Tile shapes the autotuner chooses from (BM × BN output tile, BK slab depth; the
+// 3×BF16 kernel always uses BK = 32).
+struct Cfg {
+    int bm, bn, bk;
+};
+constexpr Cfg CFGS[] = {{64, 64, 16}, {128, 128, 16}, {128, 64, 16}, {64, 128, 16}, {64, 64, 32}};
+constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
+
+}  // namespace
+
 struct tkmi_gemm_plan_s {
     std::vector<GGroup> base;  // MFMA groups without tiling
     SkinnySet skinny;
+    double flops_mfma = 0, flops_skinny = 0;
     double flops = 0;
-    TileSet t64, t128;
-    int choice[3] = {0, 0, 0};  // per precision: 0 auto
+    TileSet ts[NCFG];
+    int choice[3] = {-1, -1, -1};  // per precision: config index, -1 = automatic
 };
 
 namespace {
 
-int build_tiles(tkmi_gemm_plan_s *p, TileSet &ts, int tile) {
-    ts.tile = tile;
+int build_tiles(tkmi_gemm_plan_s *p, int c) {
+    TileSet &ts = p->ts[c];
+    const int bm = CFGS[c].bm, bn = CFGS[c].bn;
     ts.groups.clear();
     int64_t start = 0;
     for (auto g : p->base) {
-        g.tiles_m = (g.m + tile - 1) / tile;
-        g.tiles_n = (g.n + tile - 1) / tile;
-        if (g.m == 0 || g.n == 0) continue;
+        g.tiles_m = (g.m + bm - 1) / bm;
+        g.tiles_n = (g.n + bn - 1) / bn;
         g.tile_start = start;
         start += g.tiles_m * g.tiles_n;
         ts.groups.push_back(g);
@@ -515,26 +523,40 @@ int build_tiles(tkmi_gemm_plan_s *p, TileSet &ts, int tile) {
     return tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
 }
 
-int auto_tile(const tkmi_gemm_plan_s *p) {
-    // prefer 128×128 tiles when that still gives at least two tiles per CU
-    return (p->t128.ntiles >= 512) ? 128 : 64;
+int auto_cfg(const tkmi_gemm_plan_s *p) {
+    // 128×128 tiles when that still gives at least two tiles per CU, else 64×64
+    return (p->ts[1].ntiles >= 512) ? 1 : 0;
 }
 
-template <int BM>
-int launch(const TileSet &ts, int prec, const double *A, const double *B, double *C, double alpha, double beta,
-           hipStream_t s) {
-    if (ts.ntiles == 0) return TKMI_OK;
+int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
+
+template <int BM, int BN, int BK>
+void launch_k(const TileSet &ts, int prec, const double *A, const double *B, double *C, double alpha, double beta,
+              hipStream_t s) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
     if (prec == TKMI_PREC_F64)
-        hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BM>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B,
-                           C, alpha, beta);
+        hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, alpha, beta);
     else if (prec == TKMI_PREC_F32)
-        hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BM>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B,
-                           C, alpha, beta);
+        hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, alpha, beta);
     else
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BM>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B, C,
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B, C,
                            alpha, beta);
+}
+
+int launch_cfg(const tkmi_gemm_plan_s *p, int c, int prec, const double *A, const double *B, double *C,
+               double alpha, double beta, hipStream_t s) {
+    const TileSet &ts = p->ts[c];
+    if (ts.ntiles == 0) return TKMI_OK;
+    switch (c) {
+        case 0: launch_k<64, 64, 16>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 1: launch_k<128, 128, 16>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 2: launch_k<128, 64, 16>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 3: launch_k<64, 128, 16>(ts, prec, A, B, C, alpha, beta, s); break;
+        default: launch_k<64, 64, 32>(ts, prec, A, B, C, alpha, beta, s); break;
+    }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
@@ -557,9 +579,13 @@ int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double 
 void free_gemm(tkmi_gemm_plan_s *p) {
     if (!p) return;
     (void)hipFree(p->skinny.d_groups);
-    (void)hipFree(p->t64.d_groups);
-    (void)hipFree(p->t128.d_groups);
+    for (auto &t : p->ts) (void)hipFree(t.d_groups);
     delete p;
+}
+
+int check_prec(int32_t precision) {
+    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
+    return TKMI_OK;
 }
 
 }  // namespace
@@ -599,9 +625,11 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
                 TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_gemm_plan_create: leading dimension too small");
             }
         }
-        p->flops += 2.0 * (double)G.m * (double)G.n * (double)G.k;
+        const double f = 2.0 * (double)G.m * (double)G.n * (double)G.k;
+        p->flops += f;
         if (G.m == 0 || G.n == 0) continue;
-        if (std::min(G.m, G.n) <= SKINNY) {
+        if (std::min(G.m, G.n) <= SKINNY && G.k <= SKINNY_K) {
+            p->flops_skinny += f;
             G.skinny_mode = (G.n <= G.m) ? 0 : 1;
             const int64_t L = G.skinny_mode == 0 ? G.m : G.n;
             G.tiles_m = (L + NT - 1) / NT;
@@ -611,23 +639,19 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
             p->skinny.sk = (int)std::max<int64_t>(p->skinny.sk, std::min(G.m, G.n));
             p->skinny.groups.push_back(G);
         } else {
+            p->flops_mfma += f;
             p->base.push_back(G);
         }
     }
-    if (!p->skinny.groups.empty()) {
-        int rc0 = tkmi::upload((void **)&p->skinny.d_groups, p->skinny.groups.data(),
-                               p->skinny.groups.size() * sizeof(GGroup));
-        if (rc0) {
-            free_gemm(p);
-            return rc0;
-        }
-    }
+    int rc = TKMI_OK;
+    if (!p->skinny.groups.empty())
+        rc = tkmi::upload((void **)&p->skinny.d_groups, p->skinny.groups.data(),
+                          p->skinny.groups.size() * sizeof(GGroup));
     // largest groups first: their tiles start earliest
     std::stable_sort(p->base.begin(), p->base.end(), [](const GGroup &a, const GGroup &b) {
         return a.m * a.n * std::max<int64_t>(a.k, 1) > b.m * b.n * std::max<int64_t>(b.k, 1);
     });
-    int rc = build_tiles(p, p->t64, 64);
-    if (!rc) rc = build_tiles(p, p->t128, 128);
+    for (int c = 0; c < NCFG && !rc; ++c) rc = build_tiles(p, c);
     if (rc) {
         free_gemm(p);
         return rc;
@@ -638,37 +662,94 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
 
 int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
-    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
-    if (tile != 0 && tile != 64 && tile != 128) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: tile must be 0, 64 or 128");
-    plan->choice[precision] = tile;
+    if (int rc = check_prec(precision)) return rc;
+    int c;
+    if (tile == 0) c = -1;
+    else if (tile == 64) c = 0;
+    else if (tile == 128) c = 1;
+    else if (tile >= 1000 && tile < 1000 + NCFG) c = tile - 1000;
+    else TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: tile must be 0, 64, 128 or 1000 + configuration index");
+    plan->choice[precision] = c;
     return TKMI_OK;
+}
+
+int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk) {
+    if (config < 0 || config >= NCFG) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown configuration");
+    if (bm) *bm = CFGS[config].bm;
+    if (bn) *bn = CFGS[config].bn;
+    if (bk) *bk = CFGS[config].bk;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alpha, double beta,
+                         const double *A, const double *B, double *C, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (int rc = check_prec(precision)) return rc;
+    hipStream_t s = tkmi::as_stream(stream);
+    int rc = TKMI_OK;
+    if (parts & 1) rc = launch_cfg(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s);
+    if (rc) return rc;
+    if (parts & 2) rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
+    return rc;
 }
 
 int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta, const double *A,
                    const double *B, double *C, void *stream) {
+    return tkmi_gemm_exec_parts(plan, precision, 3, alpha, beta, A, B, C, stream);
+}
+
+int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double *A, const double *B, double *C,
+                            int32_t reps, void *stream) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
-    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
-    int tile = plan->choice[precision] ? plan->choice[precision] : auto_tile(plan);
+    if (int rc = check_prec(precision)) return rc;
+    if (reps < 1) reps = 1;
     hipStream_t s = tkmi::as_stream(stream);
-    int rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
+    hipEvent_t e0, e1;
+    TKMI_HIP(hipEventCreate(&e0));
+    TKMI_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_c = auto_cfg(plan);
+    int rc = TKMI_OK;
+    for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
+        if (plan->ts[c].ntiles == 0) continue;
+        if (precision == TKMI_PREC_BF16X3 && c == 4) continue;  // same shape as config 0 for that kernel
+        if ((rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
+        for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s);
+        if (rc) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; best_c = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
-    if (tile == 128) return launch<128>(plan->t128, precision, A, B, C, alpha, beta, s);
-    return launch<64>(plan->t64, precision, A, B, C, alpha, beta, s);
+    plan->choice[precision] = best_c;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *config, int64_t *ntiles) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (int rc = check_prec(precision)) return rc;
+    const int c = cfg_of(plan, precision);
+    if (config) *config = c;
+    if (ntiles) *ntiles = plan->ts[c].ntiles;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_split(tkmi_gemm_plan plan, double *flops_mfma, double *flops_skinny, int64_t *skinny_groups) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (flops_mfma) *flops_mfma = plan->flops_mfma;
+    if (flops_skinny) *flops_skinny = plan->flops_skinny;
+    if (skinny_groups) *skinny_groups = (int64_t)plan->skinny.groups.size();
+    return TKMI_OK;
 }
 
 int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
-    if (ntiles) *ntiles = plan->t64.ntiles;
+    if (ntiles) *ntiles = plan->ts[0].ntiles;
     if (flops) *flops = plan->flops;
-    return TKMI_OK;
-}
-
-int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *tile, int64_t *ntiles) {
-    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
-    if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
-    const int t = plan->choice[precision] ? plan->choice[precision] : auto_tile(plan);
-    if (tile) *tile = t;
-    if (ntiles) *ntiles = (t == 128) ? plan->t128.ntiles : plan->t64.ntiles;
     return TKMI_OK;
 }
 

@@ -625,3 +625,20 @@ def contract_labels(A, la, B, lb, lc, nout, conjA=False, conjB=False, out=None, 
         out = TensorMap.empty(VC, A.dtype)
         beta = 0.0
     return tensorcontract_(out, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, precision)
+
+
+def autotune_trace(trace, precisions=(DEFAULT_PRECISION,), reps=5):
+    """Run the GEMM autotuner on every grouped GEMM of a recorded trace, with the
+    recorded operands (outputs are overwritten; re-run the computation after)."""
+    lib = _lib.load()
+    done = set()
+    for op in trace:
+        if op[0] != "gemm":
+            continue
+        _, plan, prec, (tA, _), (tB, _), (tC, _), a, b = op
+        for p in precisions:
+            if (id(plan), p) in done:
+                continue
+            done.add((id(plan), p))
+            _lib.check(lib.tkmi_gemm_plan_autotune(plan.handle, int(p), tA.data_ptr(), tB.data_ptr(),
+                                                   tC.data_ptr(), int(reps), _stream()), "autotune")
