@@ -131,43 +131,48 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     // Per-thread element coordinates of the A and B slabs (fixed over K): global order is
-    // coalesced (contiguous index fastest), the LDS position is [row][k].
+    // coalesced (contiguous index fastest), the LDS position is [row][k].  Addresses are a
+    // wave-uniform slab base pointer (advanced on the scalar side) plus a 32-bit offset per
+    // element; interior tiles and slabs skip every bounds test.
     int ai[AE], al[AE], bj[BE], bl[BE];
-    const double *ap[AE];
-    const double *bp[BE];
-    bool aok[AE], bok[BE];
+    uint32_t aoff[AE], boff[BE];
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
         const int idx = e * NT + tid;
         if (!ta) { ai[e] = idx % BM; al[e] = idx / BM; }
         else     { al[e] = idx % BK; ai[e] = idx / BK; }
-        const int64_t gi = row0 + ai[e];
-        aok[e] = gi < m;
-        ap[e] = A + G.a_off + (ta ? (al[e] + gi * lda) : (gi + al[e] * lda));
+        aoff[e] = ta ? (uint32_t)(al[e] + ai[e] * lda) : (uint32_t)(ai[e] + al[e] * lda);
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
         const int idx = e * NT + tid;
         if (!tb) { bl[e] = idx % BK; bj[e] = idx / BK; }
         else     { bj[e] = idx % BN; bl[e] = idx / BN; }
-        const int64_t gj = col0 + bj[e];
-        bok[e] = gj < n;
-        bp[e] = Bm + G.b_off + (tb ? (gj + bl[e] * ldb) : (bl[e] + gj * ldb));
+        boff[e] = tb ? (uint32_t)(bj[e] + bl[e] * ldb) : (uint32_t)(bl[e] + bj[e] * ldb);
     }
+    const double *__restrict__ Aslab = A + G.a_off + (ta ? row0 * lda : row0);
+    const double *__restrict__ Bslab = Bm + G.b_off + (tb ? col0 : col0 * ldb);
     const int64_t astep = ta ? BK : BK * lda;  // pointer advance per K slab
     const int64_t bstep = tb ? BK * ldb : BK;
+    const bool tile_full = (row0 + BM <= m) && (col0 + BN <= n);
 
     double ra[AE], rb[BE];
     auto load_slab = [&](int64_t k0) {
-        const bool full = (k0 + BK <= K);
+        if (tile_full && k0 + BK <= K) {
 #pragma unroll
-        for (int e = 0; e < AE; ++e) ra[e] = (aok[e] && (full || k0 + al[e] < K)) ? *ap[e] : 0.0;
+            for (int e = 0; e < AE; ++e) ra[e] = Aslab[aoff[e]];
 #pragma unroll
-        for (int e = 0; e < BE; ++e) rb[e] = (bok[e] && (full || k0 + bl[e] < K)) ? *bp[e] : 0.0;
+            for (int e = 0; e < BE; ++e) rb[e] = Bslab[boff[e]];
+        } else {
 #pragma unroll
-        for (int e = 0; e < AE; ++e) ap[e] += astep;
+            for (int e = 0; e < AE; ++e)
+                ra[e] = (row0 + ai[e] < m && k0 + al[e] < K) ? Aslab[aoff[e]] : 0.0;
 #pragma unroll
-        for (int e = 0; e < BE; ++e) bp[e] += bstep;
+            for (int e = 0; e < BE; ++e)
+                rb[e] = (col0 + bj[e] < n && k0 + bl[e] < K) ? Bslab[boff[e]] : 0.0;
+        }
+        Aslab += astep;
+        Bslab += bstep;
     };
     auto store_slab = [&](int buf) {
 #pragma unroll
@@ -190,19 +195,14 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
         if (kt + 1 < nk) load_slab((kt + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
-            TC af[FM][4], bfr[FN][4];
+            typedef TC v4_t __attribute__((ext_vector_type(4)));
+            v4_t af[FM], bfr[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                const TC *src = &As[buf][wm * WM + i * 16 + (lane & 15)][kk + kq];
+            for (int i = 0; i < FM; ++i)
+                af[i] = *reinterpret_cast<const v4_t *>(&As[buf][wm * WM + i * 16 + (lane & 15)][kk + kq]);
 #pragma unroll
-                for (int s = 0; s < 4; ++s) af[i][s] = src[s];
-            }
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const TC *src = &Bs[buf][wn * WN + j * 16 + (lane & 15)][kk + kq];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) bfr[j][s] = src[s];
-            }
+            for (int j = 0; j < FN; ++j)
+                bfr[j] = *reinterpret_cast<const v4_t *>(&Bs[buf][wn * WN + j * 16 + (lane & 15)][kk + kq]);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -618,6 +618,12 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
             free_gemm(p);
             TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_gemm_plan_create: inconsistent group dimensions");
         }
+        const bool to_skinny = std::min(G.m, G.n) <= SKINNY && G.k <= SKINNY_K;
+        if (G.k > 0 && G.m > 0 && G.n > 0 && !to_skinny &&
+            (G.lda * (G.ta ? 128 : 32) >= ((int64_t)1 << 31) || G.ldb * (G.tb ? 32 : 128) >= ((int64_t)1 << 31))) {
+            free_gemm(p);
+            TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "tkmi_gemm_plan_create: leading dimension too large for 32-bit tile offsets");
+        }
         if (G.k > 0) {
             const int64_t need_a = G.ta ? G.k : G.m, need_b = G.tb ? G.n : G.k;
             if (G.lda < std::max<int64_t>(1, need_a) || G.ldb < std::max<int64_t>(1, need_b)) {
@@ -628,7 +634,7 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         const double f = 2.0 * (double)G.m * (double)G.n * (double)G.k;
         p->flops += f;
         if (G.m == 0 || G.n == 0) continue;
-        if (std::min(G.m, G.n) <= SKINNY && G.k <= SKINNY_K) {
+        if (to_skinny) {
             p->flops_skinny += f;
             G.skinny_mode = (G.n <= G.m) ? 0 : 1;
             const int64_t L = G.skinny_mode == 0 ? G.m : G.n;

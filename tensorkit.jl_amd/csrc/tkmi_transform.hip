@@ -141,26 +141,30 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     const int tid = threadIdx.x;
 
     if (B.mode == 0) {
-        // copy mode: lanes along dW for both sides
-        const int tx = tid % tW, ty = tid / tW, rstep = NTHREADS / tW;
-        if (tx < nw) {
-            const int64_t w = w0 + tx;
-            for (int r = ty; r < nr; r += rstep) {
-                const int64_t rr = r0 + r;
-                T v[L];
+        // copy mode: the tile is `nr` runs of `nw` elements along dW (contiguous on both
+        // sides); its elements are numbered run after run and dealt to the threads in
+        // order, so consecutive lanes touch consecutive addresses even for short runs.
+        const int ne = nw * nr;
+        const float inv = 1.0f / (float)nw;
+        for (int e = tid; e < ne; e += NTHREADS) {
+            int r = (int)((float)e * inv);
+            int w = e - r * nw;
+            if (w < 0) { --r; w += nw; }
+            else if (w >= nw) { ++r; w -= nw; }
+            const int64_t ww = w0 + w, rr = r0 + r;
+            T v[L];
 #pragma unroll
-                for (int i = 0; i < L; ++i)
-                    v[i] = (i < nread && has_alpha) ? src[rbase[i] + w * rsW[i] + rr * rsR[i]] : (T)0;
+            for (int i = 0; i < L; ++i)
+                v[i] = (i < nread && has_alpha) ? src[rbase[i] + ww * rsW[i] + rr * rsR[i]] : (T)0;
 #pragma unroll
-                for (int j = 0; j < L; ++j) {
-                    if (j < nwrite) {
-                        T o = 0;
+            for (int j = 0; j < L; ++j) {
+                if (j < nwrite) {
+                    T o = 0;
 #pragma unroll
-                        for (int i = 0; i < L; ++i)
-                            if (i < nread) o += u[j][i] * v[i];
-                        const int64_t a = wbase[j] + w * wsW[j] + rr * wsR[j];
-                        dst[a] = has_beta ? o + tbeta * dst[a] : o;
-                    }
+                    for (int i = 0; i < L; ++i)
+                        if (i < nread) o += u[j][i] * v[i];
+                    const int64_t a = wbase[j] + ww * wsW[j] + rr * wsR[j];
+                    dst[a] = has_beta ? o + tbeta * dst[a] : o;
                 }
             }
         }
@@ -264,8 +268,15 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
         }
         if (second < 0) second = (dW == 0) ? 1 : 0;
         dR = second;
-        b.tW = std::min(NTHREADS, pow2ceil(b.size[dW]));
-        b.tR = std::max(1, std::min(pow2ceil(b.size[dR]), 4096 / b.tW));
+        // tiles of whole runs: up to COPY_TILE elements; runs longer than that are split
+        constexpr int64_t COPY_TILE = 4096;
+        if (b.size[dW] >= COPY_TILE / 2) {
+            b.tW = (int)COPY_TILE;
+            b.tR = 1;
+        } else {
+            b.tW = (int)b.size[dW];
+            b.tR = (int)std::max<int64_t>(1, std::min<int64_t>(b.size[dR], COPY_TILE / b.size[dW]));
+        }
     } else {
         b.mode = 1;
         const int cap = (L == 1) ? 64 : (L <= 4 ? 32 : 16);
