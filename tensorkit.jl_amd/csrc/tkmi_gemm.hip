@@ -36,6 +36,14 @@ struct GGroup {
     int64_t tiles_m, tiles_n, tile_start;
     int32_t skinny_mode;  // skinny kernel only: 0 = long along m, 1 = long along n
     int32_t pad_;
+    int64_t splits;  // split-K factor (MFMA kernels): tiles_m·tiles_n·splits workgroups
+    int64_t kchunk;  // K elements per split (multiple of 32)
+    int64_t ws_off;  // split > 1: partial products at ws[ws_off + z·m·n], z < splits
+};
+
+// Split-K reduction record: C = α·Σ_z ws[ws_off + z·m·n + ·] + β·C, fixed z order.
+struct RGroup {
+    int64_t m, n, c_off, ldc, ws_off, splits, elem_start;
 };
 
 // Groups whose smaller output dimension is at most SKINNY (and K at most SKINNY_K) go
@@ -98,7 +106,7 @@ template <typename TC, int BM, int BN, int BK>
 __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
                                                      int64_t ntiles, const double *__restrict__ A,
                                                      const double *__restrict__ Bm, double *__restrict__ C,
-                                                     double alpha, double beta) {
+                                                     double *__restrict__ ws, double alpha, double beta) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int AE = BM * BK / NT;  // A elements per thread per slab
@@ -114,8 +122,12 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
     const GGroup &G = groups[find_group(groups, ngroups, tile)];
     const int64_t lt = tile - G.tile_start;
-    const int64_t tm = lt / G.tiles_n, tn = lt % G.tiles_n;
-    const int64_t m = G.m, n = G.n, K = G.k;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t tm = lmn / G.tiles_n, tn = lmn % G.tiles_n;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);  // end of this split's K range
     const int64_t row0 = tm * BM, col0 = tn * BN;
     const int64_t lda = G.lda, ldb = G.ldb;
     const int ta = G.ta, tb = G.tb;
@@ -150,10 +162,10 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
         else     { bj[e] = idx % BN; bl[e] = idx / BN; }
         boff[e] = tb ? (uint32_t)(bj[e] + bl[e] * ldb) : (uint32_t)(bl[e] + bj[e] * ldb);
     }
-    const double *__restrict__ Aslab = A + G.a_off + (ta ? row0 * lda : row0);
-    const double *__restrict__ Bslab = Bm + G.b_off + (tb ? col0 : col0 * ldb);
     const int64_t astep = ta ? BK : BK * lda;  // pointer advance per K slab
     const int64_t bstep = tb ? BK * ldb : BK;
+    const double *__restrict__ Aslab = A + G.a_off + (ta ? row0 * lda + kbeg : row0 + kbeg * lda);
+    const double *__restrict__ Bslab = Bm + G.b_off + (tb ? col0 + kbeg * ldb : col0 * ldb + kbeg);
     const bool tile_full = (row0 + BM <= m) && (col0 + BN <= n);
 
     double ra[AE], rb[BE];
@@ -181,9 +193,9 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
         for (int e = 0; e < BE; ++e) Bs[buf][bj[e]][bl[e]] = (TC)rb[e];
     };
 
-    const int64_t nk = (K + BK - 1) / BK;
+    const int64_t nk = (K - kbeg + BK - 1) / BK;
     if (nk > 0) {
-        load_slab(0);
+        load_slab(kbeg);
         store_slab(0);
     }
     __syncthreads();
@@ -192,7 +204,7 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
     const int kq = 4 * (lane >> 4);
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int buf = (int)(kt & 1);
-        if (kt + 1 < nk) load_slab((kt + 1) * BK);
+        if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             typedef TC v4_t __attribute__((ext_vector_type(4)));
@@ -214,10 +226,12 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
         __syncthreads();
     }
 
-    // epilogue
-    double *Cg = C + G.c_off;
-    const int64_t ldc = G.ldc;
-    const bool has_beta = (beta != 0.0);
+    // epilogue: C = α·acc + β·C, or this split's raw partial product into the workspace
+    const bool partial = G.splits > 1;
+    double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
+    const int64_t ldc = partial ? m : G.ldc;
+    const bool has_beta = !partial && (beta != 0.0);
+    const double a_ = partial ? 1.0 : alpha;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -228,12 +242,35 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
             for (int v = 0; v < 4; ++v) {
                 const int64_t row = row0 + wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
                 if (row < m) {
-                    double r = alpha * (double)acc[i][j][v];
+                    double r = a_ * (double)acc[i][j][v];
                     double *p = Cg + row + col * ldc;
                     *p = has_beta ? r + beta * *p : r;
                 }
             }
         }
+}
+
+// Split-K reduction: one thread per output element of the split groups.
+__global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ rg, int64_t ng, int64_t total,
+                                                    const double *__restrict__ ws, double *__restrict__ C,
+                                                    double alpha, double beta) {
+    const bool has_beta = (beta != 0.0);
+    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < total; e += (int64_t)gridDim.x * NT) {
+        int64_t lo = 0, hi = ng - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (rg[mid].elem_start <= e) lo = mid; else hi = mid - 1;
+        }
+        const RGroup &R = rg[lo];
+        const int64_t le = e - R.elem_start;
+        const int64_t mn = R.m * R.n;
+        double acc = 0.0;
+        for (int64_t zz = 0; zz < R.splits; ++zz) acc += ws[R.ws_off + zz * mn + le];
+        const int64_t row = le % R.m, col = le / R.m;
+        double *p = C + R.c_off + row + col * R.ldc;
+        const double r = alpha * acc;
+        *p = has_beta ? r + beta * *p : r;
+    }
 }
 
 
@@ -248,7 +285,7 @@ template <int BM, int BN>
 __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
                                                          int64_t ntiles, const double *__restrict__ A,
                                                          const double *__restrict__ Bm, double *__restrict__ C,
-                                                         double alpha, double beta) {
+                                                         double *__restrict__ ws, double alpha, double beta) {
     constexpr int BK = 32;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -262,8 +299,12 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
     const GGroup &G = groups[find_group(groups, ngroups, tile)];
     const int64_t lt = tile - G.tile_start;
-    const int64_t tm = lt / G.tiles_n, tn = lt % G.tiles_n;
-    const int64_t m = G.m, n = G.n, K = G.k;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t tm = lmn / G.tiles_n, tn = lmn % G.tiles_n;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);  // end of this split's K range
     const int64_t row0 = tm * BM, col0 = tn * BN;
     const double *Ag = A + G.a_off;
     const double *Bg = Bm + G.b_off;
@@ -341,13 +382,13 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
         }
     };
 
-    const int64_t nk = (K + BK - 1) / BK;
-    if (nk > 0) load_slab(0);
+    const int64_t nk = (K - kbeg + BK - 1) / BK;
+    if (nk > 0) load_slab(kbeg);
     for (int64_t kt = 0; kt < nk; ++kt) {
         __syncthreads();  // previous slab fully consumed
         store_slab();
         __syncthreads();
-        if (kt + 1 < nk) load_slab((kt + 1) * BK);
+        if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
         const int kq = 8 * (lane >> 4);
         bf8_t a[3][FM], b[3][FN];
 #pragma unroll
@@ -377,9 +418,11 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
             }
     }
 
-    double *Cg = C + G.c_off;
-    const int64_t ldc = G.ldc;
-    const bool has_beta = (beta != 0.0);
+    const bool partial = G.splits > 1;
+    double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
+    const int64_t ldc = partial ? m : G.ldc;
+    const bool has_beta = !partial && (beta != 0.0);
+    const double a_ = partial ? 1.0 : alpha;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -390,7 +433,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
             for (int v = 0; v < 4; ++v) {
                 const int64_t row = row0 + wm * WM + i * 16 + 4 * (lane >> 4) + v;
                 if (row < m) {
-                    double r = alpha * (acc64[i][j][v] + (double)lo[i][j][v]);
+                    double r = a_ * (acc64[i][j][v] + (double)lo[i][j][v]);
                     double *p = Cg + row + col * ldc;
                     *p = has_beta ? r + beta * *p : r;
                 }
@@ -476,6 +519,10 @@ struct TileSet {
     std::vector<GGroup> groups;
     int64_t ntiles = 0;
     GGroup *d_groups = nullptr;
+    std::vector<RGroup> rgroups;  // split-K groups to reduce
+    int64_t relems = 0;
+    RGroup *d_rgroups = nullptr;
+    double *ws = nullptr;  // split-K partial products (plan-owned)
 };
 
 struct SkinnySet {
@@ -509,18 +556,47 @@ namespace {
 int build_tiles(tkmi_gemm_plan_s *p, int c) {
     TileSet &ts = p->ts[c];
     const int bm = CFGS[c].bm, bn = CFGS[c].bn;
+    const int bk = CFGS[c].bk;
     ts.groups.clear();
-    int64_t start = 0;
+    // split-K when the output tiles alone cannot fill the chip: aim for ≥ SPLIT_TARGET
+    // workgroups, at most MAX_SPLIT splits, at least MIN_SLABS K slabs per split
+    constexpr int64_t SPLIT_TARGET = 1024, MAX_SPLIT = 8, MIN_SLABS = 4;
+    int64_t tiles_mn_total = 0;
+    for (auto &g : p->base) tiles_mn_total += ((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn);
+    const int64_t S = tiles_mn_total >= SPLIT_TARGET
+                          ? 1
+                          : std::min<int64_t>(MAX_SPLIT, (SPLIT_TARGET + tiles_mn_total - 1) / std::max<int64_t>(1, tiles_mn_total));
+    int64_t start = 0, ws_len = 0;
+    ts.rgroups.clear();
+    ts.relems = 0;
     for (auto g : p->base) {
         g.tiles_m = (g.m + bm - 1) / bm;
         g.tiles_n = (g.n + bn - 1) / bn;
+        const int64_t slabs = (g.k + bk - 1) / bk;
+        int64_t sg = std::max<int64_t>(1, std::min<int64_t>(S, slabs / MIN_SLABS));
+        // chunk: whole slabs of this config and a multiple of 32 (the 3×BF16 slab)
+        g.kchunk = std::max<int64_t>(32, (((slabs + sg - 1) / sg) * bk + 31) / 32 * 32);
+        sg = g.k > 0 ? (g.k + g.kchunk - 1) / g.kchunk : 1;
+        g.splits = sg;
+        g.ws_off = 0;
+        if (sg > 1) {
+            g.ws_off = ws_len;
+            ws_len += sg * g.m * g.n;
+            RGroup r{g.m, g.n, g.c_off, g.ldc, g.ws_off, sg, ts.relems};
+            ts.rgroups.push_back(r);
+            ts.relems += g.m * g.n;
+        }
         g.tile_start = start;
-        start += g.tiles_m * g.tiles_n;
+        start += g.tiles_m * g.tiles_n * g.splits;
         ts.groups.push_back(g);
     }
     ts.ntiles = start;
     if (ts.groups.empty()) return TKMI_OK;
-    return tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
+    int rc = tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
+    if (!rc && !ts.rgroups.empty())
+        rc = tkmi::upload((void **)&ts.d_rgroups, ts.rgroups.data(), ts.rgroups.size() * sizeof(RGroup));
+    if (!rc && ws_len > 0) rc = tkmi::upload((void **)&ts.ws, nullptr, ws_len * sizeof(double));
+    return rc;
 }
 
 int auto_cfg(const tkmi_gemm_plan_s *p) {
@@ -537,13 +613,18 @@ void launch_k(const TileSet &ts, int prec, const double *A, const double *B, dou
     const int64_t ng = (int64_t)ts.groups.size();
     if (prec == TKMI_PREC_F64)
         hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, alpha, beta);
+                           B, C, ts.ws, alpha, beta);
     else if (prec == TKMI_PREC_F32)
         hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, alpha, beta);
+                           B, C, ts.ws, alpha, beta);
     else
         hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B, C,
-                           alpha, beta);
+                           ts.ws, alpha, beta);
+    if (ts.relems > 0) {
+        const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
+        hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
+                           (int64_t)ts.rgroups.size(), ts.relems, ts.ws, C, alpha, beta);
+    }
 }
 
 int launch_cfg(const tkmi_gemm_plan_s *p, int c, int prec, const double *A, const double *B, double *C,
@@ -579,7 +660,11 @@ int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double 
 void free_gemm(tkmi_gemm_plan_s *p) {
     if (!p) return;
     (void)hipFree(p->skinny.d_groups);
-    for (auto &t : p->ts) (void)hipFree(t.d_groups);
+    for (auto &t : p->ts) {
+        (void)hipFree(t.d_groups);
+        (void)hipFree(t.d_rgroups);
+        (void)hipFree(t.ws);
+    }
     delete p;
 }
 

@@ -115,3 +115,23 @@ def test_reduced_precision_forms(lib, prec, tol):
         out = _run(lib, tab, A, B, Cm, prec, 1.0, 0.0, tile)
         err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
         assert err <= tol, (prec, tile, err)
+
+
+@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004])
+def test_split_k_exact(lib, tile):
+    """Few output tiles and long K engage split-K (deterministic workspace
+    reduction); integer data keeps the check exact."""
+    rng = np.random.default_rng(21)
+    shapes = [(100, 90, 3000), (64, 64, 1000), (130, 33, 4097), (40, 700, 513)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0, 1], [1, 0, 0, 1])
+    A = rng.integers(-4, 5, na).astype(np.float64)
+    B = rng.integers(-4, 5, nb).astype(np.float64)
+    Cm = rng.integers(-4, 5, nc).astype(np.float64)
+    for alpha, beta in ((1.0, 0.0), (0.5, 2.0)):
+        for prec in (0, 2):
+            out = _run(lib, tab, A, B, Cm, prec, alpha, beta, tile)
+            ref = _ref(tab, A, B, Cm, alpha, beta)
+            if prec == 0:
+                np.testing.assert_array_equal(out, ref)
+            else:
+                assert np.linalg.norm(out - ref) <= 1e-6 * np.linalg.norm(ref)
