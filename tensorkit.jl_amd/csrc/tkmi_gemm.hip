@@ -558,9 +558,10 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     const int bm = CFGS[c].bm, bn = CFGS[c].bn;
     const int bk = CFGS[c].bk;
     ts.groups.clear();
-    // split-K when the output tiles alone cannot fill the chip: aim for ≥ SPLIT_TARGET
-    // workgroups, at most MAX_SPLIT splits, at least MIN_SLABS K slabs per split
-    constexpr int64_t SPLIT_TARGET = 1024, MAX_SPLIT = 8, MIN_SLABS = 4;
+    // split-K for long-K groups (≥ LONG_K slabs) when the output tiles alone cannot
+    // fill the chip: aim for ≥ SPLIT_TARGET workgroups, at most MAX_SPLIT splits, at
+    // least MIN_SLABS K slabs per split
+    constexpr int64_t SPLIT_TARGET = 1024, MAX_SPLIT = 8, MIN_SLABS = 16, LONG_K = 32;
     int64_t tiles_mn_total = 0;
     for (auto &g : p->base) tiles_mn_total += ((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn);
     const int64_t S = tiles_mn_total >= SPLIT_TARGET
@@ -573,7 +574,7 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
         g.tiles_m = (g.m + bm - 1) / bm;
         g.tiles_n = (g.n + bn - 1) / bn;
         const int64_t slabs = (g.k + bk - 1) / bk;
-        int64_t sg = std::max<int64_t>(1, std::min<int64_t>(S, slabs / MIN_SLABS));
+        int64_t sg = slabs >= LONG_K ? std::max<int64_t>(1, std::min<int64_t>(S, slabs / MIN_SLABS)) : 1;
         // chunk: whole slabs of this config and a multiple of 32 (the 3×BF16 slab)
         g.kchunk = std::max<int64_t>(32, (((slabs + sg - 1) / sg) * bk + 31) / 32 * 32);
         sg = g.k > 0 ? (g.k + g.kchunk - 1) / g.kchunk : 1;
