@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-svd", action="store_true")
+    ap.add_argument("--no-su2", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -218,6 +220,49 @@ def main():
                              "speedup_vs_fp64": round(ms64 / ms, 3), "rel_frobenius_err_vs_fp64": err}
         log(f"{name}: {ms:.4f} ms, x{ms64/ms:.2f} vs FP64, rel err {err:.2e}")
 
+    # svd_trunc of the two-site result (truncrank 1024 ∩ trunctol 1e-10), FP64 on the device,
+    # for the FP64 result and for the 3×BF16 result: discarded-weight difference
+    svd_report = None
+    if not args.no_svd:
+        from tkmi.factorizations import svd_trunc
+        y64t = T.permute(H.graph_y, (0, 1), (2, 3), copy=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Ut, Sk, Vtt, eps64 = svd_trunc(y64t, 1024, 1e-10)
+        torch.cuda.synchronize()
+        svd_ms = (time.perf_counter() - t0) * 1e3
+        kept = sum(int(v.numel()) for v in Sk.values())
+        ybf = T.permute(H2.graph_y, (0, 1), (2, 3), copy=True)
+        _, Skb, _, epsbf = svd_trunc(ybf, 1024, 1e-10)
+        svd_report = {"ms": round(svd_ms, 2), "kept": kept, "blocks": len(Sk), "eps_fp64": eps64,
+                      "eps_bf16x3": epsbf, "abs_eps_diff": abs(epsbf - eps64),
+                      "rel_eps_diff": abs(epsbf - eps64) / eps64 if eps64 else None}
+        log(f"svd_trunc: {svd_ms:.1f} ms, kept {kept}, eps64 {eps64:.6e}, eps_bf16 {epsbf:.6e}")
+
+    # non-abelian permute at the full SU(2) rank-4 size (82.8 GB per tensor): HBM-resident
+    su2_report = None
+    if not args.no_su2:
+        from tkmi.spaces import HomSpace as _HS, ProductSpace as _PS, SU2Space
+        Vs = SU2Space((0, 64), (0.5, 96), (1, 96), (1.5, 64), (2, 32))
+        Ws = _HS(_PS(Vs, Vs), _PS(Vs, Vs))
+        nS = Ws.dim()
+        if torch.cuda.mem_get_info()[0] > 2 * nS * 8 + (8 << 30):
+            g = torch.Generator(device="cuda")
+            g.manual_seed(2)
+            As = T.TensorMap(torch.randn(nS, dtype=torch.float64, device="cuda", generator=g), Ws)
+            Bs = T.TensorMap.empty(Ws.permute((3, 1), (2, 0)))
+            T.permute_(Bs, As, (3, 1), (2, 0))  # builds and uploads the table once
+            torch.cuda.synchronize()
+            ms_su2 = time_launches(lambda: T.permute_(Bs, As, (3, 1), (2, 0)), 3, torch)
+            gbs = 16 * nS / ms_su2 / 1e6
+            plan = T.treebraider(Bs.space, Ws, (3, 1), (2, 0))
+            su2_report = {"elements": nS, "ms": round(ms_su2, 3), "GB/s": round(gbs, 1),
+                          "roofline_frac": round(gbs / PEAK_HBM_GBS, 4), "fusion_blocks": plan.nentries,
+                          "kernel": "transform_kernel<double, 8>"}
+            log(f"SU2 permute 82.8 GB: {ms_su2:.2f} ms, {gbs:.0f} GB/s")
+            del As, Bs
+            torch.cuda.empty_cache()
+
     # per-kernel breakdown and roofline of the dominant kernel
     gemms, perms = kernel_breakdown(trace, T, _lib, torch)
     by_kernel = {}
@@ -267,6 +312,8 @@ def main():
                      "traffic": None, "calls_per_step": dom["calls"],
                      "avg_launch_us": round(dom["ms"] / dom["calls"] * 1e3, 2)},
         "graph_vs_eager_maxabs": eager_err,
+        "svd_trunc": svd_report,
+        "permute_su2_full": su2_report,
     }
 
     if not args.no_cpu:

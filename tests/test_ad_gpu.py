@@ -61,7 +61,7 @@ def cost(T, A, M, Id, with_grad):
 
 def test_fermionic_cost_gradient_fd(T):
     from tkmi import chainrules as CR
-    V = fspace(160)
+    V = fspace(1024)  # D ≈ 1024 over (p, q), p ∈ {0,1}, q ∈ -4:4
     P = physical()
     rng = np.random.default_rng(4)
     A = T.TensorMap.randn(HomSpace(ProductSpace(V, P), ProductSpace(V)), rng)
