@@ -258,7 +258,7 @@ def main():
             plan = T.treebraider(Bs.space, Ws, (3, 1), (2, 0))
             su2_report = {"elements": nS, "ms": round(ms_su2, 3), "GB/s": round(gbs, 1),
                           "roofline_frac": round(gbs / PEAK_HBM_GBS, 4), "fusion_blocks": plan.nentries,
-                          "kernel": "transform_kernel<double, 8>"}
+                          "kernel": "transform_kernel<double, L>, one launch per tree class L = 1, 2, 4, 8"}
             log(f"SU2 permute 82.8 GB: {ms_su2:.2f} ms, {gbs:.0f} GB/s")
             del As, Bs
             torch.cuda.empty_cache()

@@ -632,6 +632,8 @@ int launch_cfg(const tkmi_gemm_plan_s *p, int c, int prec, const double *A, cons
                double alpha, double beta, hipStream_t s) {
     const TileSet &ts = p->ts[c];
     if (ts.ntiles == 0) return TKMI_OK;
+    if (ts.ntiles * NT >= ((int64_t)1 << 31))
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more output tiles than one launch can hold (2^31 work-items)");
     switch (c) {
         case 0: launch_k<64, 64, 16>(ts, prec, A, B, C, alpha, beta, s); break;
         case 1: launch_k<128, 128, 16>(ts, prec, A, B, C, alpha, beta, s); break;
@@ -646,6 +648,8 @@ int launch_cfg(const tkmi_gemm_plan_s *p, int c, int prec, const double *A, cons
 int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double *C, double alpha, double beta,
                   hipStream_t s) {
     if (ss.ntiles == 0) return TKMI_OK;
+    if (ss.ntiles * NT >= ((int64_t)1 << 31))
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more skinny tiles than one launch can hold (2^31 work-items)");
     dim3 grid((unsigned)ss.ntiles), block(NT);
     const int64_t ng = (int64_t)ss.groups.size();
     if (ss.sk <= 8)

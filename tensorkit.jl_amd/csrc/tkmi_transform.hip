@@ -1,21 +1,29 @@
 // Grouped permute / braid of block-sparse tensors (K1 abelian, K2 generic recoupling).
 //
-// One launch covers every sub-block (abelian) or every fusion block (generic) of a
-// transform.  Host-side plan creation canonicalises each work item:
+// Host-side plan creation canonicalises each work item:
 //   * dims of size 1 are dropped, dims that stay contiguous on both sides are merged;
 //   * dW = the dim with the smallest stride on the written side,
 //     dR = the dim with the smallest stride on the read side;
-//   * dW == dR  -> "copy" mode: lanes run along dW for both the read and the write;
-//     dW != dR  -> "transpose" mode: a (tW × tR) tile is read with lanes along dR,
-//                  staged in LDS, and written with lanes along dW.
-// Tiles of all work items are numbered consecutively (tile_start prefix sums); each
-// workgroup owns one tile and finds its work item by binary search, so the whole
-// transform is a single launch with a device-side block schedule.
+//   * dW == dR  -> "copy" mode: a tile is a run of whole rows along dW, numbered
+//                  element after element, lanes consecutive on both sides;
+//     dW != dR  -> "transpose" mode: a (tW × tR) tile (powers of two) is read with
+//                  lanes along dR, staged in LDS, and written with lanes along dW.
+// Work items are split into classes by the number of trees they touch (L = 1, 2, 4,
+// 8); each class is one launch of transform_kernel<T, L>.  Inside a launch the tiles
+// of all items are numbered consecutively (tile_start prefix sums) and each
+// workgroup finds its item by binary search: a device-side block schedule.
 //
-// Generic (non-abelian) blocks carry L_read source trees and L_write destination
-// trees of identical shape plus a dense L_write × L_read recoupling matrix U; each
-// written element is  β·dst_j + Σ_i (α·U[j,i])·src_i, i.e. the reference's
-// pack → buffer_src·Uᵀ → unpack (indexmanipulations.jl:692-721) fused in registers.
+// Every thread owns E = 16/L elements of each of the L trees of a tile, so each
+// thread has 16 independent loads in flight before its first LDS store, whatever
+// the class; the tile area is 256·E elements per tree.
+//
+// Generic (non-abelian) blocks carry R destination and C source trees of identical
+// shape plus a dense R × C recoupling matrix U; each written element is
+//   β·dst_j + Σ_i (α·U[j,i])·src_i,
+// i.e. the reference's pack → buffer_src·Uᵀ → unpack (indexmanipulations.jl:692-721)
+// fused in registers.  Destination trees are dealt out in chunks of at most 8 (one
+// work item each); source trees are consumed in chunks of L inside the kernel with
+// the sums kept in registers, so a fusion block of any size runs in one pass.
 #include "tkmi_internal.h"
 
 #include <algorithm>
@@ -26,14 +34,18 @@ namespace {
 
 constexpr int MAXD = 12;
 constexpr int NTHREADS = 256;
+constexpr int NCLASS = 4;  // L = 1, 2, 4, 8
+constexpr int MAXW = 8;    // destination trees per work item
 
 struct TBlock {
     int32_t ndim;
-    int32_t mode;  // 0 copy, 1 transpose
-    int32_t nread, nwrite;
+    int32_t mode;           // 0 copy, 1 transpose
+    int32_t nread, nwrite;  // nwrite ≤ L; nread may exceed L (chunked in the kernel)
     int32_t dW, dR;
-    int32_t tW, tR;
+    int32_t tW, tR;  // tile extents (transpose mode: powers of two)
+    int32_t lgW, lgR;
     int32_t nother;
+    int32_t u_ld;  // leading dimension of the (column-major) coefficient matrix
     int32_t odims[MAXD];
     int64_t size[MAXD];
     int64_t ntW, ntR, nbatch;
@@ -47,35 +59,168 @@ struct TTree {
     int64_t str[MAXD];
 };
 
-template <int L>
-struct TileCfg;
-template <>
-struct TileCfg<1> {
-    static constexpr int CAP = 64;
-};
-template <>
-struct TileCfg<2> {
-    static constexpr int CAP = 32;
-};
-template <>
-struct TileCfg<4> {
-    static constexpr int CAP = 32;
-};
-template <>
-struct TileCfg<8> {
-    static constexpr int CAP = 16;
-};
+// per-class tile geometry, shared by the kernel and the host tiler
+constexpr int class_elems(int L) { return 16 / L; }                // E: elements per thread per tree
+constexpr int class_area(int L) { return NTHREADS * (16 / L); }     // tile area per tree
+constexpr int class_twmax(int L) { return L <= 2 ? 256 : (L == 4 ? 128 : 64); }
+constexpr int class_L(int c) { return 1 << c; }
+
+// One tile of one work item.  COPY: the tile is nr runs of nw elements along dW
+// (contiguous on both sides), numbered run after run; otherwise a tW × tR tile
+// read with lanes along dR and written with lanes along dW through LDS.
+template <typename T, int L, bool COPY>
+__device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__restrict__ trees,
+                                               const double *__restrict__ Umat, const T *__restrict__ src,
+                                               T *__restrict__ dst, double alpha, double beta, int64_t t,
+                                               T *__restrict__ lds) {
+    constexpr int E = class_elems(L);
+    const int nread = B.nread, nwrite = B.nwrite;
+    const int dW = B.dW, dR = B.dR;
+    const int tW = B.tW, tR = B.tR;
+
+    const int64_t iW = t % B.ntW;
+    t /= B.ntW;
+    const int64_t iR = t % B.ntR;
+    const int64_t bt = t / B.ntR;
+    const int64_t w0 = iW * tW, r0 = iR * tR;
+    const int nw = (int)min((int64_t)tW, B.size[dW] - w0);
+    const int nr = (int)min((int64_t)tR, B.size[dR] - r0);
+    const int tid = threadIdx.x;
+
+    // offset of the tile corner in one tree (the batch index bt is decomposed
+    // again per tree: cheaper than holding the multi-index in scalar registers)
+    auto corner = [&](const TTree &tr) {
+        int64_t o = tr.off + w0 * tr.str[dW] + r0 * tr.str[dR];
+        int64_t b = bt;
+        for (int k = 0; k < B.nother; ++k) {
+            const int d = B.odims[k];
+            const int64_t sz = B.size[d];
+            int64_t q;
+            if ((uint64_t)b < 0x80000000ull)
+                q = (int64_t)((uint32_t)b / (uint32_t)sz);  // sz ≤ b here or q = 0
+            else
+                q = b / sz;
+            o += (b - q * sz) * tr.str[d];
+            b = q;
+        }
+        return o;
+    };
+
+    // Element k of this thread sits at (w, r) inside the tile.  Transpose tiles
+    // have tW, tR ≤ NTHREADS, so on the read side r is fixed per thread and w steps
+    // by NTHREADS/tR with k (on the write side w is fixed and r steps): addresses
+    // advance by a uniform stride and only the loaded values stay live.
+    const float inv = 1.0f / (float)nw;
+    auto pos_read = [&](int k, int &w, int &r) {
+        if (COPY) {
+            const int e = tid + k * NTHREADS;
+            r = (int)((float)e * inv);
+            w = e - r * nw;
+            if (w < 0) { --r; w += nw; }
+            else if (w >= nw) { ++r; w -= nw; }
+            return e < nw * nr;
+        }
+        r = tid & (tR - 1);
+        w = (tid >> B.lgR) + k * (NTHREADS >> B.lgR);
+        return w < nw && r < nr;
+    };
+    auto pos_write = [&](int k, int &w, int &r) {
+        if (COPY) return pos_read(k, w, r);
+        w = tid & (tW - 1);
+        r = (tid >> B.lgW) + k * (NTHREADS >> B.lgW);
+        return w < nw && r < nr;
+    };
+
+    const bool has_alpha = (alpha != 0.0);  // α = 0: the source is not read (BLAS convention)
+    T o[L][E];
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+#pragma unroll
+        for (int k = 0; k < E; ++k) o[j][k] = (T)0;
+
+    for (int ic = 0; ic < nread && has_alpha; ic += L) {
+        const int nc = min(L, nread - ic);
+        T x[L][E];
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            if (i < nc) {
+                const TTree &tr = trees[B.read_tree + ic + i];
+                const T *sp = src + corner(tr);
+                const int64_t sW = tr.str[dW], sR = tr.str[dR];
+#pragma unroll
+                for (int k = 0; k < E; ++k) {
+                    int w, r;
+                    x[i][k] = pos_read(k, w, r) ? sp[(int64_t)w * sW + (int64_t)r * sR] : (T)0;
+                }
+            }
+        }
+        if (!COPY) {
+            // stage through LDS: read layout (lanes along dR) -> write layout (lanes along dW)
+            const int pitch = tR + 1;
+            if (ic > 0) __syncthreads();
+#pragma unroll
+            for (int i = 0; i < L; ++i)
+                if (i < nc)
+#pragma unroll
+                    for (int k = 0; k < E; ++k) {
+                        int w, r;
+                        if (pos_read(k, w, r)) lds[(i * tW + w) * pitch + r] = x[i][k];
+                    }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < L; ++i)
+                if (i < nc)
+#pragma unroll
+                    for (int k = 0; k < E; ++k) {
+                        int w, r;
+                        x[i][k] = pos_write(k, w, r) ? lds[(i * tW + w) * pitch + r] : (T)0;
+                    }
+        }
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            if (j < nwrite) {
+#pragma unroll
+                for (int i = 0; i < L; ++i) {
+                    if (i < nc) {
+                        const T u = (T)(alpha * Umat[B.u_start + j + (int64_t)(ic + i) * B.u_ld]);
+#pragma unroll
+                        for (int k = 0; k < E; ++k) o[j][k] += u * x[i][k];
+                    }
+                }
+            }
+        }
+    }
+
+    const T tbeta = (T)beta;
+    const bool has_beta = (beta != 0.0);
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if (j < nwrite) {
+            const TTree &tw = trees[B.write_tree + j];
+            T *dp = dst + corner(tw);
+            const int64_t sW = tw.str[dW], sR = tw.str[dR];
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                int w, r;
+                if (pos_write(k, w, r)) {
+                    T *a = dp + (int64_t)w * sW + (int64_t)r * sR;
+                    *a = has_beta ? o[j][k] + tbeta * *a : o[j][k];
+                }
+            }
+        }
+    }
+}
 
 template <typename T, int L>
 __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     const TBlock *__restrict__ blocks, int64_t nblocks, const TTree *__restrict__ trees,
     const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, double alpha,
     double beta, int64_t tile0) {
-    constexpr int CAP = TileCfg<L>::CAP;
-    __shared__ T lds[L * CAP * (CAP + 1)];
+    constexpr int LDSN = L * (class_area(L) + class_twmax(L));  // tW·(tR+1) ≤ area + tW per tree
+    __shared__ T lds[LDSN];
 
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
-    // binary search: last block with tile_start <= gt
+    // binary search: last item with tile_start <= gt
     int64_t lo = 0, hi = nblocks - 1;
     while (lo < hi) {
         int64_t mid = (lo + hi + 1) >> 1;
@@ -85,127 +230,10 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
             hi = mid - 1;
     }
     const TBlock &B = blocks[lo];
-    const int nread = B.nread, nwrite = B.nwrite;
-    const int dW = B.dW, dR = B.dR;
-    const int tW = B.tW, tR = B.tR;
-
-    int64_t t = gt - B.tile_start;
-    const int64_t iW = t % B.ntW;
-    t /= B.ntW;
-    const int64_t iR = t % B.ntR;
-    int64_t bt = t / B.ntR;
-
-    int64_t rbase[L], wbase[L];
-    int64_t rsW[L], rsR[L], wsW[L], wsR[L];
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-        if (i < nread) {
-            const TTree &tr = trees[B.read_tree + i];
-            rbase[i] = tr.off;
-            rsW[i] = tr.str[dW];
-            rsR[i] = tr.str[dR];
-        }
-        if (i < nwrite) {
-            const TTree &tw = trees[B.write_tree + i];
-            wbase[i] = tw.off;
-            wsW[i] = tw.str[dW];
-            wsR[i] = tw.str[dR];
-        }
-    }
-    for (int k = 0; k < B.nother; ++k) {
-        const int d = B.odims[k];
-        const int64_t sz = B.size[d];
-        const int64_t idx = bt % sz;
-        bt /= sz;
-#pragma unroll
-        for (int i = 0; i < L; ++i) {
-            if (i < nread) rbase[i] += idx * trees[B.read_tree + i].str[d];
-            if (i < nwrite) wbase[i] += idx * trees[B.write_tree + i].str[d];
-        }
-    }
-
-    T u[L][L];
-#pragma unroll
-    for (int j = 0; j < L; ++j)
-#pragma unroll
-        for (int i = 0; i < L; ++i)
-            u[j][i] = (i < nread && j < nwrite) ? (T)(alpha * Umat[B.u_start + j + (int64_t)i * nwrite])
-                                                : (T)0;
-    const T tbeta = (T)beta;
-    const bool has_beta = (beta != 0.0);
-    const bool has_alpha = (alpha != 0.0);  // α = 0: the source is not read (BLAS convention)
-
-    const int64_t w0 = iW * tW, r0 = iR * tR;
-    const int nw = (int)min((int64_t)tW, B.size[dW] - w0);
-    const int nr = (int)min((int64_t)tR, B.size[dR] - r0);
-    const int tid = threadIdx.x;
-
-    if (B.mode == 0) {
-        // copy mode: the tile is `nr` runs of `nw` elements along dW (contiguous on both
-        // sides); its elements are numbered run after run and dealt to the threads in
-        // order, so consecutive lanes touch consecutive addresses even for short runs.
-        const int ne = nw * nr;
-        const float inv = 1.0f / (float)nw;
-        for (int e = tid; e < ne; e += NTHREADS) {
-            int r = (int)((float)e * inv);
-            int w = e - r * nw;
-            if (w < 0) { --r; w += nw; }
-            else if (w >= nw) { ++r; w -= nw; }
-            const int64_t ww = w0 + w, rr = r0 + r;
-            T v[L];
-#pragma unroll
-            for (int i = 0; i < L; ++i)
-                v[i] = (i < nread && has_alpha) ? src[rbase[i] + ww * rsW[i] + rr * rsR[i]] : (T)0;
-#pragma unroll
-            for (int j = 0; j < L; ++j) {
-                if (j < nwrite) {
-                    T o = 0;
-#pragma unroll
-                    for (int i = 0; i < L; ++i)
-                        if (i < nread) o += u[j][i] * v[i];
-                    const int64_t a = wbase[j] + ww * wsW[j] + rr * wsR[j];
-                    dst[a] = has_beta ? o + tbeta * dst[a] : o;
-                }
-            }
-        }
-    } else {
-        // transpose mode: read with lanes along dR into LDS, write with lanes along dW
-        {
-            const int tx = tid % tR, ty = tid / tR, wstep = NTHREADS / tR;
-            if (tx < nr) {
-                const int64_t rr = r0 + tx;
-                for (int w = ty; w < nw; w += wstep) {
-                    const int64_t ww = w0 + w;
-#pragma unroll
-                    for (int i = 0; i < L; ++i)
-                        if (i < nread)
-                            lds[(i * CAP + w) * (CAP + 1) + tx] =
-                                has_alpha ? src[rbase[i] + ww * rsW[i] + rr * rsR[i]] : (T)0;
-                }
-            }
-        }
-        __syncthreads();
-        {
-            const int tx = tid % tW, ty = tid / tW, rstep = NTHREADS / tW;
-            if (tx < nw) {
-                const int64_t ww = w0 + tx;
-                for (int r = ty; r < nr; r += rstep) {
-                    const int64_t rr = r0 + r;
-#pragma unroll
-                    for (int j = 0; j < L; ++j) {
-                        if (j < nwrite) {
-                            T o = 0;
-#pragma unroll
-                            for (int i = 0; i < L; ++i)
-                                if (i < nread) o += u[j][i] * lds[(i * CAP + tx) * (CAP + 1) + r];
-                            const int64_t a = wbase[j] + ww * wsW[j] + rr * wsR[j];
-                            dst[a] = has_beta ? o + tbeta * dst[a] : o;
-                        }
-                    }
-                }
-            }
-        }
-    }
+    if (B.mode == 0)
+        transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
+    else
+        transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
 }
 
 // -------------------------------------------------------------------------------------
@@ -217,10 +245,16 @@ struct HostItem {               // one work item before tiling
     int64_t read_tree, write_tree, u_start;
 };
 
-int pow2ceil(int64_t x) {
-    int p = 1;
-    while (p < x && p < (1 << 20)) p <<= 1;
+int64_t pow2ceil(int64_t x) {
+    int64_t p = 1;
+    while (p < x) p <<= 1;
     return p;
+}
+
+int log2i(int64_t x) {
+    int l = 0;
+    while (((int64_t)1 << l) < x) ++l;
+    return l;
 }
 
 struct PlanSide {
@@ -233,21 +267,48 @@ struct PlanSide {
 
 struct tkmi_transform_plan_s {
     int32_t N = 0;
-    int L = 1;  // template width (max trees per item, rounded to 1/2/4/8)
     int64_t dst_len = 0, src_len = 0, elements = 0, nitems = 0;
     std::vector<TTree> trees;
     std::vector<double> U, Ut;
-    PlanSide fwd, adj;
+    PlanSide fwd[NCLASS], adj[NCLASS];
     TTree *d_trees = nullptr;
     double *d_U = nullptr, *d_Ut = nullptr;
 };
 
 namespace {
 
+// Transpose-mode tile extents: powers of two with tW·tR ≤ area, tW ≤ twmax ≤ NTHREADS
+// and tR ≤ NTHREADS,
+// runs of at least 16 elements where the dims allow it (whole 128-byte lines for
+// doubles), chosen to minimise tiles × (fixed cost + tile slots).
+void choose_tile(int64_t sW, int64_t sR, int area, int twmax, int &tW, int &tR) {
+    const int64_t pW = pow2ceil(sW), pR = pow2ceil(sR);
+    double best = 1e300;
+    tW = tR = 0;
+    for (int64_t r = 1; r <= NTHREADS && r <= pR; r <<= 1) {
+        if (r < std::min<int64_t>(16, pR)) continue;
+        for (int64_t w = 1; w <= twmax && w <= pW && w * r <= area; w <<= 1) {
+            if (w < std::min<int64_t>(16, pW)) continue;
+            const double tiles = (double)((sW + w - 1) / w) * (double)((sR + r - 1) / r);
+            const double cost = tiles * (256.0 + 0.25 * (double)(w * r));
+            const bool better = cost < best ||
+                                (cost == best && std::min(w, r) > std::min<int64_t>(tW, tR)) ||
+                                (cost == best && std::min(w, r) == std::min<int64_t>(tW, tR) && r > tR);
+            if (better) {
+                best = cost;
+                tW = (int)w;
+                tR = (int)r;
+            }
+        }
+    }
+}
+
 // Fill tiling fields of a block given which trees are read/written.
 void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
     const TTree &tw = trees[b.write_tree];
     const TTree &tr = trees[b.read_tree];
+    const int area = (L == 1) ? class_area(1) : (L == 2 ? class_area(2) : (L == 4 ? class_area(4) : class_area(8)));
+    const int twmax = (L == 1) ? class_twmax(1) : (L == 2 ? class_twmax(2) : (L == 4 ? class_twmax(4) : class_twmax(8)));
     int dW = 0, dR = 0;
     int64_t bestW = INT64_MAX, bestR = INT64_MAX;
     for (int d = 0; d < b.ndim; ++d) {
@@ -258,7 +319,7 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
     }
     if (dW == dR) {
         b.mode = 0;
-        // second tiled dim: next smallest write stride among the others (or a padded dim)
+        // second tiled dim: next smallest write stride among the others (or a size-1 dim)
         int second = -1;
         int64_t best2 = INT64_MAX;
         for (int d = 0; d < b.ndim; ++d) {
@@ -268,20 +329,20 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
         }
         if (second < 0) second = (dW == 0) ? 1 : 0;
         dR = second;
-        // tiles of whole runs: up to COPY_TILE elements; runs longer than that are split
-        constexpr int64_t COPY_TILE = 4096;
-        if (b.size[dW] >= COPY_TILE / 2) {
-            b.tW = (int)COPY_TILE;
+        // tiles of whole runs: up to `area` elements; runs longer than half of it are split
+        if (b.size[dW] >= area / 2) {
+            b.tW = area;
             b.tR = 1;
         } else {
             b.tW = (int)b.size[dW];
-            b.tR = (int)std::max<int64_t>(1, std::min<int64_t>(b.size[dR], COPY_TILE / b.size[dW]));
+            b.tR = (int)std::max<int64_t>(1, std::min<int64_t>(b.size[dR], area / b.size[dW]));
         }
+        b.lgW = b.lgR = 0;
     } else {
         b.mode = 1;
-        const int cap = (L == 1) ? 64 : (L <= 4 ? 32 : 16);
-        b.tW = std::min(cap, pow2ceil(b.size[dW]));
-        b.tR = std::min(cap, pow2ceil(b.size[dR]));
+        choose_tile(b.size[dW], b.size[dR], area, twmax, b.tW, b.tR);
+        b.lgW = log2i(b.tW);
+        b.lgR = log2i(b.tR);
     }
     b.dW = dW;
     b.dR = dR;
@@ -296,7 +357,7 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
     }
 }
 
-// Canonicalise one work item: `sizes` are the dst-order sizes; `wstr[t][k]` the
+// Canonicalise one work item: `sizes` are the dst-order sizes; `strides[t][k]` the
 // strides (in dst order) of every tree t that takes part.  Returns the kept dims.
 std::vector<std::pair<int64_t, std::vector<int>>> canon_dims(
     const std::vector<int64_t> &sizes, const std::vector<const int64_t *> &strides) {
@@ -304,19 +365,16 @@ std::vector<std::pair<int64_t, std::vector<int>>> canon_dims(
     std::vector<int> keep;
     for (size_t k = 0; k < sizes.size(); ++k)
         if (sizes[k] > 1) keep.push_back((int)k);
-    // merge runs: dim b follows a if for every tree str[b] == str[a]*size[a]
+    // merge runs: dim k follows the group if for every tree str[k] == str[first]*size(group)
     std::vector<std::pair<int64_t, std::vector<int>>> groups;  // (size, member dims)
     for (int k : keep) {
         if (!groups.empty()) {
             auto &g = groups.back();
-            int a = g.second.back();
             bool ok = true;
             int first = g.second.front();
             for (auto *s : strides) {
-                // merged group's stride is str[first]; the size so far is g.first
                 if (s[k] != s[first] * g.first) { ok = false; break; }
             }
-            (void)a;
             if (ok) {
                 g.first *= sizes[k];
                 g.second.push_back(k);
@@ -328,35 +386,43 @@ std::vector<std::pair<int64_t, std::vector<int>>> canon_dims(
     return groups;
 }
 
+int class_of(int nread, int nwrite) {
+    const int m = std::max(std::min(nread, MAXW), nwrite);
+    return m <= 1 ? 0 : (m <= 2 ? 1 : (m <= 4 ? 2 : 3));
+}
+
 int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
-    // build forward and adjoint tile lists
+    // forward and adjoint tile lists, one per class; destination trees are split
+    // into chunks of at most MAXW per work item
     std::vector<TTree> &trees = p->trees;
-    auto build = [&](bool adjoint, PlanSide &side) {
-        side.blocks.clear();
-        int64_t tiles = 0;
-        for (auto &it : items) {
-            TBlock b;
-            std::memset(&b, 0, sizeof(b));
-            b.ndim = (int)it.size.size();
-            for (int d = 0; d < b.ndim; ++d) b.size[d] = it.size[d];
-            if (!adjoint) {
-                b.nread = it.nread;
-                b.nwrite = it.nwrite;
-                b.read_tree = it.read_tree;
-                b.write_tree = it.write_tree;
-            } else {
-                b.nread = it.nwrite;
-                b.nwrite = it.nread;
-                b.read_tree = it.write_tree;
-                b.write_tree = it.read_tree;
-            }
-            b.u_start = it.u_start;
-            tile_block(b, trees, p->L);
-            b.tile_start = tiles;
-            tiles += b.ntW * b.ntR * b.nbatch;
-            side.blocks.push_back(b);
+    auto build = [&](bool adjoint, PlanSide *sides) {
+        for (int c = 0; c < NCLASS; ++c) {
+            sides[c].blocks.clear();
+            sides[c].ntiles = 0;
         }
-        side.ntiles = tiles;
+        for (auto &it : items) {
+            const int R = adjoint ? it.nread : it.nwrite;   // written trees on this side
+            const int Cn = adjoint ? it.nwrite : it.nread;  // read trees on this side
+            const int64_t rt = adjoint ? it.write_tree : it.read_tree;
+            const int64_t wt = adjoint ? it.read_tree : it.write_tree;
+            for (int jc = 0; jc < R; jc += MAXW) {
+                TBlock b;
+                std::memset(&b, 0, sizeof(b));
+                b.ndim = (int)it.size.size();
+                for (int d = 0; d < b.ndim; ++d) b.size[d] = it.size[d];
+                b.nread = Cn;
+                b.nwrite = std::min(MAXW, R - jc);
+                b.read_tree = rt;
+                b.write_tree = wt + jc;
+                b.u_start = it.u_start + jc;
+                b.u_ld = R;
+                const int c = class_of(b.nread, b.nwrite);
+                tile_block(b, trees, class_L(c));
+                b.tile_start = sides[c].ntiles;
+                sides[c].ntiles += b.ntW * b.ntR * b.nbatch;
+                sides[c].blocks.push_back(b);
+            }
+        }
     };
     build(false, p->fwd);
     build(true, p->adj);
@@ -366,12 +432,14 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
     if ((rc = tkmi::upload((void **)&p->d_trees, trees.data(), trees.size() * sizeof(TTree)))) return rc;
     if ((rc = tkmi::upload((void **)&p->d_U, p->U.data(), p->U.size() * sizeof(double)))) return rc;
     if ((rc = tkmi::upload((void **)&p->d_Ut, p->Ut.data(), p->Ut.size() * sizeof(double)))) return rc;
-    if ((rc = tkmi::upload((void **)&p->fwd.d_blocks, p->fwd.blocks.data(),
-                           p->fwd.blocks.size() * sizeof(TBlock))))
-        return rc;
-    if ((rc = tkmi::upload((void **)&p->adj.d_blocks, p->adj.blocks.data(),
-                           p->adj.blocks.size() * sizeof(TBlock))))
-        return rc;
+    for (int c = 0; c < NCLASS; ++c) {
+        for (PlanSide *side : {&p->fwd[c], &p->adj[c]}) {
+            if (side->blocks.empty()) continue;
+            if ((rc = tkmi::upload((void **)&side->d_blocks, side->blocks.data(),
+                                   side->blocks.size() * sizeof(TBlock))))
+                return rc;
+        }
+    }
     return TKMI_OK;
 }
 
@@ -380,8 +448,10 @@ void free_plan(tkmi_transform_plan_s *p) {
     (void)hipFree(p->d_trees);
     (void)hipFree(p->d_U);
     (void)hipFree(p->d_Ut);
-    (void)hipFree(p->fwd.d_blocks);
-    (void)hipFree(p->adj.d_blocks);
+    for (int c = 0; c < NCLASS; ++c) {
+        (void)hipFree(p->fwd[c].d_blocks);
+        (void)hipFree(p->adj[c].d_blocks);
+    }
     delete p;
 }
 
@@ -397,7 +467,7 @@ int check_perm(int32_t N, const int32_t *perm) {
 }
 
 // Add one work item given dst-order sizes and per-tree (dst-order) strides.
-// read trees come first in `trs` (nread of them), then write trees.
+// read trees come first in `strs` (nread of them), then write trees.
 int add_item(tkmi_transform_plan_s *p, std::vector<HostItem> &items, const std::vector<int64_t> &sizes,
              const std::vector<std::vector<int64_t>> &strs, const std::vector<int64_t> &offs, int nread,
              int nwrite, int64_t u_start) {
@@ -426,47 +496,47 @@ int add_item(tkmi_transform_plan_s *p, std::vector<HostItem> &items, const std::
     return TKMI_OK;
 }
 
-template <typename T>
-int launch_side(const tkmi_transform_plan_s *p, const PlanSide &side, const double *dU, const T *src, T *dst,
-                double alpha, double beta, hipStream_t stream) {
-    if (side.ntiles == 0) return TKMI_OK;
-    const int64_t chunk = (int64_t)1 << 30;
+template <typename T, int L>
+void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const double *dU, const T *src, T *dst,
+                  double alpha, double beta, hipStream_t stream) {
+    // the dispatch packet holds the grid size in work-items as 32 bits: keep
+    // blocks × NTHREADS below 2^31 per launch and continue at tile0 = t0
+    const int64_t chunk = ((int64_t)1 << 31) / NTHREADS;
     for (int64_t t0 = 0; t0 < side.ntiles; t0 += chunk) {
-        unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
-        switch (p->L) {
-            case 1:
-                hipLaunchKernelGGL((transform_kernel<T, 1>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                                   (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
-                break;
-            case 2:
-                hipLaunchKernelGGL((transform_kernel<T, 2>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                                   (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
-                break;
-            case 4:
-                hipLaunchKernelGGL((transform_kernel<T, 4>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                                   (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
-                break;
-            default:
-                hipLaunchKernelGGL((transform_kernel<T, 8>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                                   (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
-                break;
-        }
+        const unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
+        hipLaunchKernelGGL((transform_kernel<T, L>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
+                           (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
     }
+}
+
+template <typename T>
+int launch_side(const tkmi_transform_plan_s *p, const PlanSide *sides, const double *dU, const T *src, T *dst,
+                double alpha, double beta, hipStream_t stream) {
+    if (sides[0].ntiles) launch_class<T, 1>(p, sides[0], dU, src, dst, alpha, beta, stream);
+    if (sides[1].ntiles) launch_class<T, 2>(p, sides[1], dU, src, dst, alpha, beta, stream);
+    if (sides[2].ntiles) launch_class<T, 4>(p, sides[2], dU, src, dst, alpha, beta, stream);
+    if (sides[3].ntiles) launch_class<T, 8>(p, sides[3], dU, src, dst, alpha, beta, stream);
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
+}
+
+int64_t total_tiles(const PlanSide *sides) {
+    int64_t n = 0;
+    for (int c = 0; c < NCLASS; ++c) n += sides[c].ntiles;
+    return n;
 }
 
 int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst, double alpha, double beta,
                 void *stream, bool adjoint) {
     if (!p) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
-    const PlanSide &side = adjoint ? p->adj : p->fwd;
+    const PlanSide *sides = adjoint ? p->adj : p->fwd;
     const double *dU = adjoint ? p->d_Ut : p->d_U;
-    if (side.ntiles > 0 && (!src || !dst)) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
+    if (total_tiles(sides) > 0 && (!src || !dst)) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
     hipStream_t s = tkmi::as_stream(stream);
     if (dtype == TKMI_F64)
-        return launch_side<double>(p, side, dU, (const double *)src, (double *)dst, alpha, beta, s);
+        return launch_side<double>(p, sides, dU, (const double *)src, (double *)dst, alpha, beta, s);
     if (dtype == TKMI_F32)
-        return launch_side<float>(p, side, dU, (const float *)src, (float *)dst, alpha, beta, s);
+        return launch_side<float>(p, sides, dU, (const float *)src, (float *)dst, alpha, beta, s);
     TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
 }
 
@@ -497,7 +567,6 @@ int tkmi_abelian_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
     const int N = numind;
     auto *p = new tkmi_transform_plan_s();
     p->N = N;
-    p->L = 1;
     p->dst_len = dst_len;
     p->src_len = src_len;
     std::vector<HostItem> items;
@@ -541,12 +610,12 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
     int rc = check_perm(numind, perm);
     if (rc) return rc;
     const int N = numind;
-    int64_t maxL = 1;
-    for (int64_t b = 0; b < nblocks; ++b) maxL = std::max(maxL, std::max(rows[b], cols[b]));
-    if (maxL > 8) TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "tkmi_generic_plan_create: more than 8 trees in a fusion block");
+    if (nblocks < 0) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_generic_plan_create: negative block count");
+    for (int64_t b = 0; b < nblocks; ++b)
+        if (rows[b] < 1 || cols[b] < 1 || rows[b] > INT32_MAX || cols[b] > INT32_MAX)
+            TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_generic_plan_create: bad recoupling matrix size");
     auto *p = new tkmi_transform_plan_s();
     p->N = N;
-    p->L = maxL <= 1 ? 1 : (maxL <= 2 ? 2 : (maxL <= 4 ? 4 : 8));
     p->dst_len = dst_len;
     p->src_len = src_len;
     std::vector<HostItem> items;
@@ -620,7 +689,7 @@ int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const v
 
 int tkmi_transform_plan_info(tkmi_transform_plan plan, int64_t *ntiles, int64_t *nentries, int64_t *elements) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
-    if (ntiles) *ntiles = plan->fwd.ntiles;
+    if (ntiles) *ntiles = total_tiles(plan->fwd);
     if (nentries) *nentries = plan->nitems;
     if (elements) *elements = plan->elements;
     return TKMI_OK;
