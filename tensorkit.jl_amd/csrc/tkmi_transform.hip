@@ -13,9 +13,10 @@
 // of all items are numbered consecutively (tile_start prefix sums) and each
 // workgroup finds its item by binary search: a device-side block schedule.
 //
-// Every thread owns E = 16/L elements of each of the L trees of a tile, so each
-// thread has 16 independent loads in flight before its first LDS store, whatever
-// the class; the tile area is 256·E elements per tree.
+// Every thread owns E elements of each of the L trees of a tile (E = 8 for L = 1,
+// else 16/L), all loaded before the first LDS store; the tile area is 256·E
+// elements per tree.  Loads and stores are non-temporal: every element is touched
+// once, so the transform streams through the caches.
 //
 // Generic (non-abelian) blocks carry R destination and C source trees of identical
 // shape plus a dense R × C recoupling matrix U; each written element is
@@ -59,9 +60,12 @@ struct TTree {
     int64_t str[MAXD];
 };
 
-// per-class tile geometry, shared by the kernel and the host tiler
-constexpr int class_elems(int L) { return 16 / L; }                // E: elements per thread per tree
-constexpr int class_area(int L) { return NTHREADS * (16 / L); }     // tile area per tree
+// per-class tile geometry, shared by the kernel and the host tiler: E elements per
+// thread per tree (8 for single-tree items, else 16/L), tile area 256·E per tree
+// (measured on 82.8 GB SU(2) permutes: larger areas lose more to occupancy than
+// they gain in loads in flight)
+constexpr int class_elems(int L) { return L == 1 ? 8 : 16 / L; }
+constexpr int class_area(int L) { return NTHREADS * class_elems(L); }
 constexpr int class_twmax(int L) { return L <= 2 ? 256 : (L == 4 ? 128 : 64); }
 constexpr int class_L(int c) { return 1 << c; }
 
@@ -150,7 +154,8 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 #pragma unroll
                 for (int k = 0; k < E; ++k) {
                     int w, r;
-                    x[i][k] = pos_read(k, w, r) ? sp[(int64_t)w * sW + (int64_t)r * sR] : (T)0;
+                    x[i][k] = pos_read(k, w, r) ? __builtin_nontemporal_load(&sp[(int64_t)w * sW + (int64_t)r * sR])
+                                                 : (T)0;
                 }
             }
         }
@@ -204,7 +209,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 int w, r;
                 if (pos_write(k, w, r)) {
                     T *a = dp + (int64_t)w * sW + (int64_t)r * sR;
-                    *a = has_beta ? o[j][k] + tbeta * *a : o[j][k];
+                    __builtin_nontemporal_store(has_beta ? o[j][k] + tbeta * *a : o[j][k], a);
                 }
             }
         }
