@@ -30,6 +30,13 @@ for _p in (ROOT, os.path.join(ROOT, "tensorkit.jl_amd")):
 METRIC = "U1 D=2048 contract GFLOP/s + permute GB/s; split-BF16 vs FP64 at stated err"
 PEAK_FP64_TF = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (two passes,
+# FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
+# Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
+TRAFFIC_PMC = {
+    ("gemm_k4_kernel<double, 64, 64, 16>", 544256): (34909 * 2 + 66218) * 1024,  # L·x
+    ("gemm_k4_kernel<double, 64, 64, 32>", 219136): (53315 * 2 + 23434) * 1024,  # ·R, split-K partials
+}
 
 
 def log(*a):
@@ -74,6 +81,11 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
             L.tkmi_gemm_plan_query(plan.handle, prec, C.byref(tile), C.byref(nt))
             fm, fs, ns = C.c_double(), C.c_double(), C.c_int64()
             L.tkmi_gemm_plan_split(plan.handle, C.byref(fm), C.byref(fs), C.byref(ns))
+            # algorithmic HBM bytes of the MFMA launch: A, B read once, C written once (and
+            # read once with β≠0), over the groups the plan gives to the MFMA kernel
+            # (smaller output side > 32 or K > 64)
+            mfma_rows = [r for r in plan.table if r[0] and r[1] and (min(r[0], r[1]) > 32 or r[2] > 64)]
+            alg_bytes = 8 * sum(r[0] * r[2] + r[2] * r[1] + r[0] * r[1] * (2 if b != 0 else 1) for r in mfma_rows)
             for part, flops, name in ((1, fm.value, gemm_kernel_name(prec, tile.value, L)),
                                       (2, fs.value, "gemm_skinny_kernel")):
                 if flops == 0 and not (part == 2 and ns.value > 0):
@@ -83,7 +95,9 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
                 fn = lambda plan=plan, prec=prec, part=part, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
                     plan.handle, prec, part, a, b, pa, pb, pc, s)
                 ms = time_launches(fn, reps, torch)
-                gemms.append({"flops": flops, "ms": ms, "kernel": name, "tiles": nt.value if part == 1 else None})
+                gemms.append({"flops": flops, "ms": ms, "kernel": name, "tiles": nt.value if part == 1 else None,
+                              "grid_x": nt.value * 256 if part == 1 else None,
+                              "alg_bytes": alg_bytes if part == 1 else None})
         else:
             _, plan, perm, (ts_, ns_), (td_, nd), a, b = op
             ps, pd = ts_.data_ptr(), td_.data_ptr()
@@ -145,6 +159,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-su2", action="store_true")
+    # profiling modes for rocprofv3 --pmc passes (no JSON line): a plain device copy of
+    # known size to calibrate FETCH_SIZE / WRITE_SIZE, and N eager applications of the
+    # autotuned FP64 chain (the kernels and tile configurations the graph replays)
+    ap.add_argument("--counter-calibration", action="store_true")
+    ap.add_argument("--counter-workload", type=int, default=0, metavar="N")
+    ap.add_argument("--gemm-configs", default="", metavar="C0,C1,…",
+                    help="counter workload: FP64 tile configuration per grouped GEMM (as the "
+                         "timed run logs them) instead of autotuning under the profiler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -158,6 +180,16 @@ def main():
     from tkmi.heff import TwoSiteHeff, two_site_spaces
 
     T._device()
+    if args.counter_calibration:
+        n = 64 << 20  # doubles: 512 MiB read and 512 MiB written per dispatch
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 3 dispatches of {8 * n} bytes read + {8 * n} bytes written")
+        return
     V, P, W = two_site_spaces(D=args.D, qmax=8, mode="round")
     log(f"V dim {V.dim()}, x dim {TwoSiteHeff.__name__}")
     H = TwoSiteHeff(V, P, W, np.random.default_rng(1))
@@ -174,9 +206,28 @@ def main():
         f"{flops/1e9:.3f} GFLOP, {sum(1 for o in trace if o[0]=='transform')} permutes")
 
     # tile-shape autotuning of every grouped GEMM (per precision form), then recompute
-    T.autotune_trace(trace, precisions=(_lib.PREC_F64, _lib.PREC_F32, _lib.PREC_BF16X3))
+    import ctypes as C
+    lib = _lib.load()
+    gemm_ops = [op for op in trace if op[0] == "gemm"]
+    if args.counter_workload and args.gemm_configs:
+        for op, c in zip(gemm_ops, [int(x) for x in args.gemm_configs.split(",")]):
+            _lib.check(lib.tkmi_gemm_plan_set_tile(op[1].handle, _lib.PREC_F64, 1000 + c), "set tile")
+    else:
+        T.autotune_trace(trace, precisions=(_lib.PREC_F64, _lib.PREC_F32, _lib.PREC_BF16X3))
+    cfgs = []
+    for op in gemm_ops:
+        c = C.c_int32()
+        lib.tkmi_gemm_plan_query(op[1].handle, _lib.PREC_F64, C.byref(c), None)
+        cfgs.append(c.value)
+    log("FP64 tile configurations per grouped GEMM: " + ",".join(map(str, cfgs)))
     y_eager = H.apply()
     torch.cuda.synchronize()
+    if args.counter_workload:
+        for _ in range(args.counter_workload):
+            H.apply()
+        torch.cuda.synchronize()
+        log(f"counter workload: {args.counter_workload} eager FP64 applications")
+        return
 
     # FP64 graph
     y64 = H.capture(precision=_lib.PREC_F64)
@@ -224,8 +275,9 @@ def main():
     # for the FP64 result and for the 3×BF16 result: discarded-weight difference
     svd_report = None
     if not args.no_svd:
-        from tkmi.factorizations import svd_trunc
+        from tkmi.factorizations import svd_trunc, DEFAULT_SVD_ALGORITHM
         y64t = T.permute(H.graph_y, (0, 1), (2, 3), copy=True)
+        svd_trunc(y64t, 1024, 1e-10)  # plan, lanes and rocSOLVER workspaces: first call only
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         Ut, Sk, Vtt, eps64 = svd_trunc(y64t, 1024, 1e-10)
@@ -234,9 +286,19 @@ def main():
         kept = sum(int(v.numel()) for v in Sk.values())
         ybf = T.permute(H2.graph_y, (0, 1), (2, 3), copy=True)
         _, Skb, _, epsbf = svd_trunc(ybf, 1024, 1e-10)
-        svd_report = {"ms": round(svd_ms, 2), "kept": kept, "blocks": len(Sk), "eps_fp64": eps64,
+        svd_report = {"ms": round(svd_ms, 2), "algorithm": DEFAULT_SVD_ALGORITHM,
+                      "block_shapes": sorted({(d1, d2) for _, (d1, d2), _ in y64t.space.degeneracystructure()[1]},
+                                             reverse=True),
+                      "kept": kept, "blocks_kept": len(Sk), "eps_fp64": eps64,
                       "eps_bf16x3": epsbf, "abs_eps_diff": abs(epsbf - eps64),
                       "rel_eps_diff": abs(epsbf - eps64) / eps64 if eps64 else None}
+        if not args.no_cpu:
+            # host LAPACK (numpy gesdd) over the same blocks, for scale
+            blocks_h = [b for b in y64t.blocks_host().values() if b.size]
+            t0 = time.perf_counter()
+            for b in blocks_h:
+                np.linalg.svd(b, full_matrices=False)
+            svd_report["host_lapack_gesdd_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
         log(f"svd_trunc: {svd_ms:.1f} ms, kept {kept}, eps64 {eps64:.6e}, eps_bf16 {epsbf:.6e}")
 
     # non-abelian permute at the full SU(2) rank-4 size (82.8 GB per tensor): HBM-resident
@@ -267,13 +329,19 @@ def main():
     gemms, perms = kernel_breakdown(trace, T, _lib, torch)
     by_kernel = {}
     for g in gemms:
-        d = by_kernel.setdefault(g["kernel"], {"flops": 0.0, "ms": 0.0, "calls": 0})
+        d = by_kernel.setdefault(g["kernel"], {"flops": 0.0, "ms": 0.0, "calls": 0, "launches": []})
         d["flops"] += g["flops"]
         d["ms"] += g["ms"]
         d["calls"] += 1
+        d["launches"].append(g)
     mfma_kernels = {k: v for k, v in by_kernel.items() if k != "gemm_skinny_kernel"}
     dom_name, dom = max(mfma_kernels.items(), key=lambda kv: kv[1]["ms"])
     achieved_tf = dom["flops"] / dom["ms"] / 1e9
+    # measured HBM traffic per launch (FETCH_SIZE·2 + WRITE_SIZE, copy-calibrated), known
+    # only for the launches profiled in that configuration; null otherwise
+    measured = [TRAFFIC_PMC.get((dom_name, g["grid_x"])) for g in dom["launches"]]
+    traffic = (sum(measured) / len(measured)) if measured and all(x is not None for x in measured) else None
+    alg_bytes = sum(g["alg_bytes"] for g in dom["launches"]) / len(dom["launches"])
     perm_bytes = sum(p["bytes"] for p in perms)
     perm_ms = sum(p["ms"] for p in perms)
     big = max(perms, key=lambda p: p["bytes"]) if perms else None
@@ -309,7 +377,9 @@ def main():
         "precision": prec_report,
         "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": round(achieved_tf, 3),
                      "peak": PEAK_FP64_TF, "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_FP64_TF, 4),
-                     "traffic": None, "calls_per_step": dom["calls"],
+                     "traffic": traffic, "traffic_unit": "B/launch",
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "launch_grids": [g["grid_x"] for g in dom["launches"]], "calls_per_step": dom["calls"],
                      "avg_launch_us": round(dom["ms"] / dom["calls"] * 1e3, 2)},
         "graph_vs_eager_maxabs": eager_err,
         "svd_trunc": svd_report,

@@ -155,11 +155,20 @@ int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
  * dimension lda[c]): A_c = U_c·diag(S_c)·Vt_c, r_c = min(m_c, n_c).  Outputs are
  * packed block after block: U_c (m_c×r_c, ld m_c), S_c (r_c), Vt_c (r_c×n_c,
  * ld r_c) — the layouts of U: codomain←V_r, Vᴴ: V_r←domain as TensorMaps.
+ * The blocks run concurrently on streams owned by the plan, ordered after the
+ * work already on `stream`; `stream` waits for all of them before continuing.
  * Replaces MAK.svd_compact! per block (src/factorizations/matrixalgebrakit.jl:34-51). */
 int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m, const int64_t *n,
                          const int64_t *a_off, const int64_t *lda);
 int tkmi_svd_plan_sizes(tkmi_svd_plan plan, int64_t *u_len, int64_t *s_len, int64_t *vt_len);
 int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, double *Vt, void *stream);
+/* Block SVD driver, the `alg` of MAK.svd_compact!: MatrixAlgebraKit's
+ * ROCSOLVER_QRIteration / ROCSOLVER_DivideAndConquer / ROCSOLVER_Jacobi
+ * (TensorKit forwards `alg` per block, src/factorizations/matrixalgebrakit.jl:34-51). */
+#define TKMI_SVD_QRITERATION 0
+#define TKMI_SVD_DIVIDEANDCONQUER 1
+#define TKMI_SVD_JACOBI 2
+int tkmi_svd_plan_set_algorithm(tkmi_svd_plan plan, int32_t alg);
 int tkmi_svd_plan_destroy(tkmi_svd_plan plan);
 
 /* Keep-mask for truncrank(howmany) ∩ trunctol(atol) over all singular values

@@ -803,21 +803,32 @@ int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double
     hipEvent_t e0, e1;
     TKMI_HIP(hipEventCreate(&e0));
     TKMI_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    int best_c = auto_cfg(plan);
+    // Interleaved rounds, best time per configuration (drift and clock ramps hit every
+    // configuration alike); another configuration replaces the default only when it
+    // is faster by more than 3%, so near-ties resolve the same way on every run.
+    constexpr int ROUNDS = 3;
+    float tbest[NCFG];
+    for (int c = 0; c < NCFG; ++c) tbest[c] = 1e30f;
     int rc = TKMI_OK;
-    for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
-        if (plan->ts[c].ntiles == 0) continue;
-        if (precision == TKMI_PREC_BF16X3 && c == 4) continue;  // same shape as config 0 for that kernel
-        if ((rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
-        if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
-        for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s);
-        if (rc) break;
-        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best) { best = ms; best_c = c; }
+    for (int round = 0; round < ROUNDS && rc == TKMI_OK; ++round) {
+        for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
+            if (plan->ts[c].ntiles == 0) continue;
+            if (precision == TKMI_PREC_BF16X3 && c == 4) continue;  // same shape as config 0 for that kernel
+            if ((rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
+            for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s);
+            if (rc) break;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tbest[c] = std::min(tbest[c], ms);
+        }
     }
+    const int d = auto_cfg(plan);
+    int best_c = d;
+    for (int c = 0; c < NCFG; ++c)
+        if (tbest[c] < tbest[best_c]) best_c = c;
+    if (tbest[d] <= 1.03f * tbest[best_c]) best_c = d;  // within 3% of the fastest: keep the default
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;

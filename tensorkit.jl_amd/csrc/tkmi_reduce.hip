@@ -11,10 +11,19 @@ template <typename T>
 __global__ __launch_bounds__(RT) void dot_partial(const T *__restrict__ a, const T *__restrict__ b, int64_t n,
                                                   double *__restrict__ parts) {
     __shared__ double sh[RT];
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
-        s += (double)a[i] * (double)b[i];
-    sh[threadIdx.x] = s;
+    // four independent strided partial sums per thread keep four load pairs in flight;
+    // the order of every addition is fixed by (n, grid), so results are reproducible
+    const int64_t stride = (int64_t)gridDim.x * RT;
+    int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        s0 += (double)a[i] * (double)b[i];
+        s1 += (double)a[i + stride] * (double)b[i + stride];
+        s2 += (double)a[i + 2 * stride] * (double)b[i + 2 * stride];
+        s3 += (double)a[i + 3 * stride] * (double)b[i + 3 * stride];
+    }
+    for (; i < n; i += stride) s0 += (double)a[i] * (double)b[i];
+    sh[threadIdx.x] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     for (int w = RT / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];

@@ -1,8 +1,12 @@
 // Block SVD and truncation selection for svd_trunc, all FP64 on the device.
 //
 // tkmi_svd_exec   per coupled block A_c (d1×d2, column-major): A_c = U_c·diag(S_c)·Vt_c
-//                 (thin: r_c = min(d1, d2)) with rocSOLVER dgesvd on the caller's stream
+//                 (thin: r_c = min(d1, d2)) with a rocSOLVER driver on the caller's stream:
+//                 QR iteration (dgesvd), divide and conquer (dgesdd) or one-sided Jacobi
+//                 (dgesvdj), MatrixAlgebraKit's ROCSOLVER_* algorithm choices
 //                 (MAK.svd_compact! per block, src/factorizations/matrixalgebrakit.jl:34-51).
+//                 Blocks are independent: they run on up to 8 concurrent lanes (stream +
+//                 rocBLAS handle each) forked from and joined back to the caller's stream.
 // tkmi_truncate   keep-mask over all singular values of all blocks for
 //                 truncrank(howmany) ∩ trunctol(atol) with quantum-dimension weights, and
 //                 the truncation error ε = sqrt(Σ_discarded w·σ²)
@@ -17,6 +21,8 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -54,23 +60,86 @@ __global__ __launch_bounds__(ST) void sum_sqrt(const double *__restrict__ x, int
 
 }  // namespace
 
+// One lane = one stream + rocBLAS handle + scratch: the blocks of a plan are dealt
+// out over up to MAX_LANES lanes (largest first, to the least loaded lane) and the
+// lanes run concurrently, each driven by its own host thread so that a driver that
+// waits on the host for its stream holds up only its own lane.
+constexpr int MAX_LANES = 8;
+
+struct SvdLane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    rocblas_handle handle = nullptr;
+    double *work = nullptr;  // packed copy of one block (the rocSOLVER drivers overwrite their input)
+    double *E = nullptr;
+    double *residual = nullptr;
+    rocblas_int *info = nullptr, *sweeps = nullptr;
+    std::vector<size_t> blocks;
+    int64_t maxmn = 0, maxr = 0;
+};
+
 struct tkmi_svd_plan_s {
     std::vector<int64_t> m, n, a_off, lda, r, u_off, s_off, v_off;
-    int64_t u_len = 0, s_len = 0, v_len = 0, maxmn = 0, maxr = 0;
-    rocblas_handle handle = nullptr;
-    double *work = nullptr;  // copy of one block (dgesvd overwrites its input)
-    double *E = nullptr;
-    rocblas_int *info = nullptr;
+    int64_t u_len = 0, s_len = 0, v_len = 0;
+    int32_t alg = TKMI_SVD_QRITERATION;
+    int device = 0;
+    hipEvent_t start = nullptr;
+    std::vector<SvdLane> lanes;
 };
 
 namespace {
 void free_svd(tkmi_svd_plan_s *p) {
     if (!p) return;
-    if (p->handle) rocblas_destroy_handle(p->handle);
-    (void)hipFree(p->work);
-    (void)hipFree(p->E);
-    (void)hipFree(p->info);
+    for (auto &l : p->lanes) {
+        if (l.handle) rocblas_destroy_handle(l.handle);
+        if (l.done) (void)hipEventDestroy(l.done);
+        if (l.stream) (void)hipStreamDestroy(l.stream);
+        (void)hipFree(l.work);
+        (void)hipFree(l.E);
+        (void)hipFree(l.residual);
+        (void)hipFree(l.info);
+        (void)hipFree(l.sweeps);
+    }
+    if (p->start) (void)hipEventDestroy(p->start);
     delete p;
+}
+
+// Factorise the blocks of one lane on its stream; returns a TKMI status (errors are
+// reported by the calling thread).
+int run_lane(const tkmi_svd_plan_s *p, SvdLane &l, const double *A, double *U, double *S, double *Vt,
+             std::string &err) {
+    if (hipSetDevice(p->device) != hipSuccess) { err = "svd: hipSetDevice failed"; return TKMI_ERR_HIP; }
+    if (rocblas_set_stream(l.handle, l.stream) != rocblas_status_success) {
+        err = "svd: rocblas_set_stream failed";
+        return TKMI_ERR_HIP;
+    }
+    for (size_t b : l.blocks) {
+        const int64_t m = p->m[b], n = p->n[b], r = p->r[b];
+        if (hipMemcpy2DAsync(l.work, m * sizeof(double), A + p->a_off[b], p->lda[b] * sizeof(double),
+                             m * sizeof(double), n, hipMemcpyDeviceToDevice, l.stream) != hipSuccess) {
+            err = "svd: block copy failed";
+            return TKMI_ERR_HIP;
+        }
+        const rocblas_int im = (rocblas_int)m, in = (rocblas_int)n, ir = (rocblas_int)r;
+        double *Sb = S + p->s_off[b], *Ub = U + p->u_off[b], *Vb = Vt + p->v_off[b];
+        rocblas_status st;
+        if (p->alg == TKMI_SVD_DIVIDEANDCONQUER)
+            st = rocsolver_dgesdd(l.handle, rocblas_svect_singular, rocblas_svect_singular, im, in, l.work, im, Sb,
+                                  Ub, im, Vb, ir, l.info);
+        else if (p->alg == TKMI_SVD_JACOBI)
+            // abstol ≤ 0: converge to machine precision
+            st = rocsolver_dgesvdj(l.handle, rocblas_svect_singular, rocblas_svect_singular, im, in, l.work, im, 0.0,
+                                   l.residual, 100, l.sweeps, Sb, Ub, im, Vb, ir, l.info);
+        else
+            st = rocsolver_dgesvd(l.handle, rocblas_svect_singular, rocblas_svect_singular, im, in, l.work, im, Sb,
+                                  Ub, im, Vb, ir, l.E, rocblas_outofplace, l.info);
+        if (st != rocblas_status_success) {
+            err = "svd: rocSOLVER driver failed";
+            return TKMI_ERR_HIP;
+        }
+    }
+    if (hipEventRecord(l.done, l.stream) != hipSuccess) { err = "svd: hipEventRecord failed"; return TKMI_ERR_HIP; }
+    return TKMI_OK;
 }
 }  // namespace
 
@@ -81,6 +150,7 @@ int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m,
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_svd_plan_create: null plan pointer");
     *plan = nullptr;
     auto *p = new tkmi_svd_plan_s();
+    std::vector<size_t> order;
     for (int64_t b = 0; b < nblocks; ++b) {
         if (m[b] < 0 || n[b] < 0 || lda[b] < std::max<int64_t>(1, m[b]) || m[b] > INT32_MAX || n[b] > INT32_MAX) {
             free_svd(p);
@@ -98,19 +168,42 @@ int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m,
         p->u_len += m[b] * r;
         p->s_len += r;
         p->v_len += r * n[b];
-        p->maxmn = std::max(p->maxmn, m[b] * n[b]);
-        p->maxr = std::max(p->maxr, r);
+        if (r > 0) order.push_back((size_t)b);
     }
-    if (rocblas_create_handle(&p->handle) != rocblas_status_success) {
-        free_svd(p);
-        TKMI_FAIL(TKMI_ERR_HIP, "tkmi_svd_plan_create: rocblas_create_handle failed");
+    // cost model m·n·r; largest block first onto the least loaded lane
+    auto cost = [&](size_t b) { return (double)p->m[b] * (double)p->n[b] * (double)p->r[b]; };
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+    const int nl = (int)std::min<size_t>(MAX_LANES, order.size());
+    p->lanes.resize(nl);
+    std::vector<double> load(nl, 0.0);
+    for (size_t b : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[k] += cost(b);
+        SvdLane &l = p->lanes[k];
+        l.blocks.push_back(b);
+        l.maxmn = std::max(l.maxmn, p->m[b] * p->n[b]);
+        l.maxr = std::max(l.maxr, p->r[b]);
     }
-    int rc;
-    if ((rc = tkmi::upload((void **)&p->work, nullptr, std::max<int64_t>(1, p->maxmn) * sizeof(double))) ||
-        (rc = tkmi::upload((void **)&p->E, nullptr, std::max<int64_t>(1, p->maxr) * sizeof(double))) ||
-        (rc = tkmi::upload((void **)&p->info, nullptr, sizeof(rocblas_int)))) {
+    if (hipGetDevice(&p->device) != hipSuccess || hipEventCreateWithFlags(&p->start, hipEventDisableTiming) != hipSuccess) {
         free_svd(p);
-        return rc;
+        TKMI_FAIL(TKMI_ERR_HIP, "tkmi_svd_plan_create: event creation failed");
+    }
+    for (auto &l : p->lanes) {
+        if (hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&l.done, hipEventDisableTiming) != hipSuccess ||
+            rocblas_create_handle(&l.handle) != rocblas_status_success) {
+            free_svd(p);
+            TKMI_FAIL(TKMI_ERR_HIP, "tkmi_svd_plan_create: stream / rocBLAS handle creation failed");
+        }
+        int rc;
+        if ((rc = tkmi::upload((void **)&l.work, nullptr, std::max<int64_t>(1, l.maxmn) * sizeof(double))) ||
+            (rc = tkmi::upload((void **)&l.E, nullptr, std::max<int64_t>(1, l.maxr) * sizeof(double))) ||
+            (rc = tkmi::upload((void **)&l.residual, nullptr, sizeof(double))) ||
+            (rc = tkmi::upload((void **)&l.sweeps, nullptr, sizeof(rocblas_int))) ||
+            (rc = tkmi::upload((void **)&l.info, nullptr, sizeof(rocblas_int)))) {
+            free_svd(p);
+            return rc;
+        }
     }
     *plan = p;
     return TKMI_OK;
@@ -126,22 +219,34 @@ int tkmi_svd_plan_sizes(tkmi_svd_plan plan, int64_t *u_len, int64_t *s_len, int6
 
 int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, double *Vt, void *stream) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: null plan");
+    if (plan->lanes.empty()) return TKMI_OK;
     hipStream_t s = tkmi::as_stream(stream);
-    if (rocblas_set_stream(plan->handle, s) != rocblas_status_success)
-        TKMI_FAIL(TKMI_ERR_HIP, "svd: rocblas_set_stream failed");
-    for (size_t b = 0; b < plan->m.size(); ++b) {
-        const int64_t m = plan->m[b], n = plan->n[b], r = plan->r[b];
-        if (r == 0) continue;
-        // dgesvd overwrites A: work on a packed copy of the block
-        TKMI_HIP(hipMemcpy2DAsync(plan->work, m * sizeof(double), A + plan->a_off[b], plan->lda[b] * sizeof(double),
-                                  m * sizeof(double), n, hipMemcpyDeviceToDevice, s));
-        rocblas_status st = rocsolver_dgesvd(plan->handle, rocblas_svect_singular, rocblas_svect_singular,
-                                             (rocblas_int)m, (rocblas_int)n, plan->work, (rocblas_int)m,
-                                             S + plan->s_off[b], U + plan->u_off[b], (rocblas_int)m,
-                                             Vt + plan->v_off[b], (rocblas_int)r, plan->E, rocblas_outofplace,
-                                             plan->info);
-        if (st != rocblas_status_success) TKMI_FAIL(TKMI_ERR_HIP, "svd: rocsolver_dgesvd failed");
+    // fork: every lane starts after the work already queued on the caller's stream
+    TKMI_HIP(hipEventRecord(plan->start, s));
+    for (auto &l : plan->lanes) TKMI_HIP(hipStreamWaitEvent(l.stream, plan->start, 0));
+    const size_t nl = plan->lanes.size();
+    std::vector<int> rc(nl, TKMI_OK);
+    std::vector<std::string> err(nl);
+    if (nl == 1) {
+        rc[0] = run_lane(plan, plan->lanes[0], A, U, S, Vt, err[0]);
+    } else {
+        std::vector<std::thread> th;
+        for (size_t k = 0; k < nl; ++k)
+            th.emplace_back([&, k] { rc[k] = run_lane(plan, plan->lanes[k], A, U, S, Vt, err[k]); });
+        for (auto &t : th) t.join();
     }
+    for (size_t k = 0; k < nl; ++k)
+        if (rc[k] != TKMI_OK) TKMI_FAIL(rc[k], err[k]);
+    // join: the caller's stream continues once every lane is done
+    for (auto &l : plan->lanes) TKMI_HIP(hipStreamWaitEvent(s, l.done, 0));
+    return TKMI_OK;
+}
+
+int tkmi_svd_plan_set_algorithm(tkmi_svd_plan plan, int32_t alg) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: null plan");
+    if (alg != TKMI_SVD_QRITERATION && alg != TKMI_SVD_DIVIDEANDCONQUER && alg != TKMI_SVD_JACOBI)
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: unknown algorithm");
+    plan->alg = alg;
     return TKMI_OK;
 }
 

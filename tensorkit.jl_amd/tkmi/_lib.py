@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libtkmi.so")
 TKMI_OK = 0
 TKMI_F64, TKMI_F32 = 0, 1
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
+SVD_QRITERATION, SVD_DIVIDEANDCONQUER, SVD_JACOBI = 0, 1, 2
 
 c_i64p = C.POINTER(C.c_int64)
 c_i32p = C.POINTER(C.c_int32)
@@ -58,6 +59,7 @@ SIGNATURES = [
     ("tkmi_svd_plan_sizes", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
     ("tkmi_svd_exec", C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_svd_plan_destroy", C.c_int, [c_vp]),
+    ("tkmi_svd_plan_set_algorithm", C.c_int, [c_vp, C.c_int32]),
     ("tkmi_truncate_select", C.c_int, [C.c_int64, c_vp, c_vp, C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_dot", C.c_int, [C.c_int32, c_vp, c_vp, C.c_int64, c_vp, c_vp]),
     ("tkmi_graph_begin", C.c_int, [c_vp]),

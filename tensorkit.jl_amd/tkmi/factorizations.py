@@ -34,16 +34,24 @@ class SVDPlan:
         _lib.check(lib.tkmi_svd_plan_create(C.byref(h), len(blocks), cols[0][1], cols[1][1], cols[2][1],
                                             cols[3][1]), "tkmi_svd_plan_create")
         self.handle = h
+        self._destroy = lib.tkmi_svd_plan_destroy
         self.r = [min(d1, d2) for _, d1, d2, _ in blocks]
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h and _lib._lib is not None:
-            _lib._lib.tkmi_svd_plan_destroy(h)
+        # the destroy function is bound at creation: module globals may already be
+        # torn down when the interpreter exits
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
             self.handle = None
 
 
 _SVD_CACHE = {}
+
+# MatrixAlgebraKit's rocSOLVER SVD algorithms (the `alg` of svd_compact / svd_trunc)
+SVD_ALGORITHMS = {"qr_iteration": _lib.SVD_QRITERATION, "divide_and_conquer": _lib.SVD_DIVIDEANDCONQUER,
+                  "jacobi": _lib.SVD_JACOBI}
+DEFAULT_SVD_ALGORITHM = "divide_and_conquer"
 
 
 def _mid_space(W, counts):
@@ -51,9 +59,13 @@ def _mid_space(W, counts):
     return GradedSpace({c: n for c, n in counts.items() if n > 0})
 
 
-def svd_compact(t):
-    """(U, S, Vᴴ) with U: codomain ← V_r, S: {c: σ_c} (descending), Vᴴ: V_r ← domain."""
+def svd_compact(t, alg=None):
+    """(U, S, Vᴴ) with U: codomain ← V_r, S: {c: σ_c} (descending), Vᴴ: V_r ← domain.
+    `alg`: one of SVD_ALGORITHMS (default DEFAULT_SVD_ALGORITHM)."""
     W = t.space
+    code = SVD_ALGORITHMS.get(alg or DEFAULT_SVD_ALGORITHM)
+    if code is None:
+        raise ValueError(f"unknown SVD algorithm {alg!r}; expected one of {sorted(SVD_ALGORITHMS)}")
     plan = _SVD_CACHE.get(W)
     if plan is None:
         plan = SVDPlan(W)
@@ -64,6 +76,7 @@ def svd_compact(t):
     Vt = T.TensorMap.empty(HomSpace(ProductSpace(Vr), W.domain))
     S = torch.empty(sum(plan.r), dtype=torch.float64, device=T._device())
     lib = _lib.load()
+    _lib.check(lib.tkmi_svd_plan_set_algorithm(plan.handle, code), "svd algorithm")
     _lib.check(lib.tkmi_svd_exec(plan.handle, t.data.data_ptr(), U.data.data_ptr(), S.data_ptr(),
                                  Vt.data.data_ptr(), T._stream()), "svd")
     offs, o = {}, 0
@@ -91,13 +104,13 @@ def _copy_leading(dst, src, sizes_src_dst):
                                        1.0, 0.0, T._stream()), "truncate copy")
 
 
-def svd_trunc(t, howmany=None, atol=0.0):
+def svd_trunc(t, howmany=None, atol=0.0, alg=None):
     """`svd_trunc(t; trunc = truncrank(howmany) & trunctol(; atol))`.
 
     Returns (Ũ, S̃, Ṽᴴ, ε) with S̃ = {c: kept σ_c (device)} and ε the
     dim(c)-weighted 2-norm of the discarded singular values (a device scalar
     read back as float)."""
-    U, (S, offs), Vt = svd_compact(t)
+    U, (S, offs), Vt = svd_compact(t, alg)
     R = S.numel()
     weights = torch.empty(R, dtype=torch.float64, device=S.device)
     w_host = np.zeros(R)

@@ -346,13 +346,16 @@ def _canonicalize(p, C):
 class GemmPlan:
     def __init__(self, handle, flops, ngroups):
         self.handle = handle
+        self._destroy = _lib.load().tkmi_gemm_plan_destroy
         self.flops = flops
         self.ngroups = ngroups
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h and _lib._lib is not None:
-            _lib._lib.tkmi_gemm_plan_destroy(h)
+        # the destroy function is bound at creation: module globals may already be
+        # torn down when the interpreter exits
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
             self.handle = None
 
 

@@ -31,15 +31,18 @@ class TransformPlan:
 
     def __init__(self, handle, kind, nentries, elements, table):
         self.handle = handle
+        self._destroy = _lib.load().tkmi_transform_plan_destroy
         self.kind = kind
         self.nentries = nentries
         self.elements = elements
         self.table = table
 
     def __del__(self):
-        h = getattr(self, "handle", None)
-        if h and _lib._lib is not None:
-            _lib._lib.tkmi_transform_plan_destroy(h)
+        # the destroy function is bound at creation: module globals may already be
+        # torn down when the interpreter exits
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
             self.handle = None
 
 

@@ -47,9 +47,12 @@ def reference_truncation(sv, howmany, atol):
     return counts, np.sqrt(err2)
 
 
-def _svd_checks(T, t, howmany, atol):
+ALGS = ["qr_iteration", "divide_and_conquer", "jacobi"]
+
+
+def _svd_checks(T, t, howmany, atol, alg=None):
     from tkmi.factorizations import svd_compact, svd_trunc
-    U, (S, offs), Vt = svd_compact(t)
+    U, (S, offs), Vt = svd_compact(t, alg)
     Sh = S.cpu().numpy()
     Ab = host_blocks(t)
     Ub, Vb = host_blocks(U), host_blocks(Vt)
@@ -60,8 +63,11 @@ def _svd_checks(T, t, howmany, atol):
         np.testing.assert_allclose(Sh[o:o + r], s_ref, rtol=1e-12, atol=1e-12 * s_ref[0])
         rec = Ub[c] @ np.diag(Sh[o:o + r]) @ Vb[c]
         assert np.linalg.norm(rec - A) <= 1e-12 * np.linalg.norm(A)
+        # isometric factors (isisometric(U), isisometric(Vᴴ'))
+        assert np.linalg.norm(Ub[c].T @ Ub[c] - np.eye(r)) <= 1e-12 * max(1, r)
+        assert np.linalg.norm(Vb[c] @ Vb[c].T - np.eye(r)) <= 1e-12 * max(1, r)
         sv[c] = (s_ref, c.dim())
-    Ut, Sk, Vtt, eps = svd_trunc(t, howmany, atol)
+    Ut, Sk, Vtt, eps = svd_trunc(t, howmany, atol, alg)
     counts, eps_ref = reference_truncation(sv, howmany, atol)
     assert {c: len(v) for c, v in Sk.items()} == {c: n for c, n in counts.items() if n}
     assert abs(eps - eps_ref) <= 1e-10 * max(eps_ref, 1e-300) + 1e-14
@@ -78,23 +84,26 @@ def _svd_checks(T, t, howmany, atol):
     assert abs(np.sqrt(d2) - eps) <= 1e-9 * eps + 1e-13 * tnorm
 
 
-def test_svd_trunc_u1(T):
+@pytest.mark.parametrize("alg", ALGS)
+def test_svd_trunc_u1(T, alg):
     rng = np.random.default_rng(0)
     V = U1Space((0, 40), (1, 30), (-1, 30), (2, 12), (-2, 12))
     P = U1Space((0.5, 1), (-0.5, 1))
     t = T.TensorMap.randn(HomSpace(ProductSpace(V, P), ProductSpace(P.dual(), V)), rng)
-    _svd_checks(T, t, howmany=60, atol=1e-10)
-    _svd_checks(T, t, howmany=10 ** 6, atol=0.0)
+    _svd_checks(T, t, howmany=60, atol=1e-10, alg=alg)
+    _svd_checks(T, t, howmany=10 ** 6, atol=0.0, alg=alg)
 
 
-def test_svd_trunc_su2_weights(T):
+@pytest.mark.parametrize("alg", ALGS)
+def test_svd_trunc_su2_weights(T, alg):
     rng = np.random.default_rng(1)
     V = SU2Space((0, 6), (0.5, 5), (1, 4), (1.5, 2))
     t = T.TensorMap.randn(HomSpace(ProductSpace(V, V), ProductSpace(V)), rng)
-    _svd_checks(T, t, howmany=40, atol=1e-10)
+    _svd_checks(T, t, howmany=40, atol=1e-10, alg=alg)
 
 
-def test_svd_rank_deficient_tol(T):
+@pytest.mark.parametrize("alg", ALGS)
+def test_svd_rank_deficient_tol(T, alg):
     """trunctol discards (near-)zero singular values of a rank-deficient block."""
     rng = np.random.default_rng(2)
     V = U1Space((0, 12), (1, 6))
@@ -105,6 +114,14 @@ def test_svd_rank_deficient_tol(T):
         x[off:off + d1 * d2] = a.reshape(-1, order="F")
     t = T.TensorMap.from_numpy(x, W)
     from tkmi.factorizations import svd_trunc
-    Ut, Sk, Vtt, eps = svd_trunc(t, 100, 1e-10)
+    Ut, Sk, Vtt, eps = svd_trunc(t, 100, 1e-10, alg)
     assert {c: len(v) for c, v in Sk.items()} == {c: 2 for c in Sk}
     assert eps < 1e-12
+
+
+def test_svd_unknown_algorithm(T):
+    from tkmi.factorizations import svd_compact
+    V = U1Space((0, 3), (1, 2))
+    t = T.TensorMap.randn(HomSpace(ProductSpace(V), ProductSpace(V)), np.random.default_rng(3))
+    with pytest.raises(ValueError):
+        svd_compact(t, "gesvd_magic")
