@@ -2,9 +2,10 @@
 //
 // One launch computes C_g = α·op(A_g)·op(B_g) + β·C_g for every coupled sector g of a
 // contraction (the merge-walk of mul!, src/tensors/linalg.jl:330-370).  Tiles of all
-// groups are numbered consecutively; workgroup → tile goes through an XCD-aware remap
-// (consecutive tiles of one group land on the same XCD, so they share A panels in that
-// XCD's L2) and a binary search over the groups' tile_start prefix sums.
+// groups are numbered heaviest first (per-tile K, descending: the long tiles start
+// early and the short ones fill the tail); workgroup → tile goes through an XCD-aware
+// remap (chunks of consecutive tiles dealt round-robin to the 8 XCDs) and a binary
+// search over the groups' tile_start prefix sums.
 //
 // Workgroup = 256 threads = 4 wave64s in a 2×2 arrangement; a BM×BN output tile is
 // computed as (BM/2)×(BN/2) per wave from 16×16 MFMA fragments.  K advances in BK
@@ -56,16 +57,16 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 __device__ inline int64_t xcd_remap(int64_t bid, int64_t ntiles) {
-    // hardware dispatches workgroup ids round-robin over 8 XCDs; give each XCD a
-    // contiguous range of tiles instead.
-    constexpr int64_t NX = 8;
-    const int64_t q = ntiles / NX, r = ntiles % NX;
+    // Hardware dispatches workgroup ids round-robin over the 8 XCDs.  Tiles are dealt
+    // to XCDs in chunks of CH consecutive tiles (chunk c → XCD c % 8): the tiles of a
+    // chunk share A panels in one XCD's L2, and every XCD sees the same mix of heavy
+    // (long-K) and light groups, so no XCD finishes last by a wide margin.  The tail
+    // that does not fill a round of 8 chunks keeps the identity mapping.
+    constexpr int64_t NX = 8, CH = 16;
+    const int64_t full = ntiles / (NX * CH) * (NX * CH);
+    if (bid >= full) return bid;
     const int64_t x = bid % NX, pos = bid / NX;
-    // XCD x owns tiles [start_x, start_x + cnt_x); cnt_x = q + (x < r)
-    const int64_t start = x * q + min(x, r);
-    const int64_t cnt = q + (x < r ? 1 : 0);
-    if (pos < cnt) return start + pos;
-    return bid;  // unreachable when gridDim.x == ntiles
+    return ((pos / CH) * NX + x) * CH + pos % CH;
 }
 
 __device__ inline int find_group(const GGroup *__restrict__ g, int64_t ng, int64_t t) {
@@ -570,6 +571,7 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     int64_t start = 0, ws_len = 0;
     ts.rgroups.clear();
     ts.relems = 0;
+    std::vector<GGroup> order;
     for (auto g : p->base) {
         g.tiles_m = (g.m + bm - 1) / bm;
         g.tiles_n = (g.n + bn - 1) / bn;
@@ -587,6 +589,13 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
             ts.rgroups.push_back(r);
             ts.relems += g.m * g.n;
         }
+        order.push_back(g);
+    }
+    // heaviest tiles first: a tile's work is proportional to its K range
+    std::stable_sort(order.begin(), order.end(), [](const GGroup &a, const GGroup &b) {
+        return std::min(a.k, a.kchunk) > std::min(b.k, b.kchunk);
+    });
+    for (auto &g : order) {
         g.tile_start = start;
         start += g.tiles_m * g.tiles_n * g.splits;
         ts.groups.push_back(g);
