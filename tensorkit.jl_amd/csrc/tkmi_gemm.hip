@@ -103,11 +103,14 @@ struct MfmaK4<float> {
     __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <typename TC, int BM, int BN, int BK>
-__global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                     int64_t ntiles, const double *__restrict__ A,
-                                                     const double *__restrict__ Bm, double *__restrict__ C,
-                                                     double *__restrict__ ws, double alpha, double beta) {
+// NBUF = 2: double-buffered LDS, one barrier per slab.  NBUF = 1: one LDS buffer (half
+// the LDS, so more workgroups per CU), the next slab waits in registers while the
+// current one is multiplied; two barriers per slab.
+template <typename TC, int BM, int BN, int BK, int NBUF>
+__device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, int64_t ngroups, int64_t ntiles,
+                                             const double *__restrict__ A, const double *__restrict__ Bm,
+                                             double *__restrict__ C, double *__restrict__ ws, double alpha,
+                                             double beta) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int AE = BM * BK / NT;  // A elements per thread per slab
@@ -116,8 +119,8 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
     // row pitch an odd multiple of 16 bytes, so the 16-byte reads of 16 lanes (16 rows)
     // fall in distinct banks.
     constexpr int KP = BK + 16 / (int)sizeof(TC);
-    __shared__ __attribute__((aligned(16))) TC As[2][BM][KP];
-    __shared__ __attribute__((aligned(16))) TC Bs[2][BN][KP];
+    __shared__ __attribute__((aligned(16))) TC As[NBUF][BM][KP];
+    __shared__ __attribute__((aligned(16))) TC Bs[NBUF][BN][KP];
     using acc_t = typename MfmaK4<TC>::acc_t;
 
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
@@ -147,21 +150,30 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
     // coalesced (contiguous index fastest), the LDS position is [row][k].  Addresses are a
     // wave-uniform slab base pointer (advanced on the scalar side) plus a 32-bit offset per
     // element; interior tiles and slabs skip every bounds test.
-    int ai[AE], al[AE], bj[BE], bl[BE];
+    // element e of this thread: (row i, k) of the A slab, (k, column j) of the B slab;
+    // cheap shifts of tid, recomputed where needed rather than held in registers
+    auto a_el = [&](int e, int &i, int &k) {
+        const int idx = e * NT + tid;
+        if (!ta) { i = idx % BM; k = idx / BM; }
+        else     { k = idx % BK; i = idx / BK; }
+    };
+    auto b_el = [&](int e, int &j, int &k) {
+        const int idx = e * NT + tid;
+        if (!tb) { k = idx % BK; j = idx / BK; }
+        else     { j = idx % BN; k = idx / BN; }
+    };
     uint32_t aoff[AE], boff[BE];
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
-        const int idx = e * NT + tid;
-        if (!ta) { ai[e] = idx % BM; al[e] = idx / BM; }
-        else     { al[e] = idx % BK; ai[e] = idx / BK; }
-        aoff[e] = ta ? (uint32_t)(al[e] + ai[e] * lda) : (uint32_t)(ai[e] + al[e] * lda);
+        int i, k;
+        a_el(e, i, k);
+        aoff[e] = ta ? (uint32_t)(k + i * lda) : (uint32_t)(i + k * lda);
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
-        const int idx = e * NT + tid;
-        if (!tb) { bl[e] = idx % BK; bj[e] = idx / BK; }
-        else     { bj[e] = idx % BN; bl[e] = idx / BN; }
-        boff[e] = tb ? (uint32_t)(bj[e] + bl[e] * ldb) : (uint32_t)(bl[e] + bj[e] * ldb);
+        int j, k;
+        b_el(e, j, k);
+        boff[e] = tb ? (uint32_t)(j + k * ldb) : (uint32_t)(k + j * ldb);
     }
     const int64_t astep = ta ? BK : BK * lda;  // pointer advance per K slab
     const int64_t bstep = tb ? BK * ldb : BK;
@@ -178,34 +190,41 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
             for (int e = 0; e < BE; ++e) rb[e] = Bslab[boff[e]];
         } else {
 #pragma unroll
-            for (int e = 0; e < AE; ++e)
-                ra[e] = (row0 + ai[e] < m && k0 + al[e] < K) ? Aslab[aoff[e]] : 0.0;
+            for (int e = 0; e < AE; ++e) {
+                int i, k;
+                a_el(e, i, k);
+                ra[e] = (row0 + i < m && k0 + k < K) ? Aslab[aoff[e]] : 0.0;
+            }
 #pragma unroll
-            for (int e = 0; e < BE; ++e)
-                rb[e] = (col0 + bj[e] < n && k0 + bl[e] < K) ? Bslab[boff[e]] : 0.0;
+            for (int e = 0; e < BE; ++e) {
+                int j, k;
+                b_el(e, j, k);
+                rb[e] = (col0 + j < n && k0 + k < K) ? Bslab[boff[e]] : 0.0;
+            }
         }
         Aslab += astep;
         Bslab += bstep;
     };
     auto store_slab = [&](int buf) {
 #pragma unroll
-        for (int e = 0; e < AE; ++e) As[buf][ai[e]][al[e]] = (TC)ra[e];
+        for (int e = 0; e < AE; ++e) {
+            int i, k;
+            a_el(e, i, k);
+            As[buf][i][k] = (TC)ra[e];
+        }
 #pragma unroll
-        for (int e = 0; e < BE; ++e) Bs[buf][bj[e]][bl[e]] = (TC)rb[e];
+        for (int e = 0; e < BE; ++e) {
+            int j, k;
+            b_el(e, j, k);
+            Bs[buf][j][k] = (TC)rb[e];
+        }
     };
 
     const int64_t nk = (K - kbeg + BK - 1) / BK;
-    if (nk > 0) {
-        load_slab(kbeg);
-        store_slab(0);
-    }
-    __syncthreads();
     // MFMA s of a K slab uses k = 4*(lane/16) + s for both operands: each lane reads
     // its 4 k values of a row as one contiguous 4-element vector.
     const int kq = 4 * (lane >> 4);
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int buf = (int)(kt & 1);
-        if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
+    auto mma_slab = [&](int buf) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             typedef TC v4_t __attribute__((ext_vector_type(4)));
@@ -223,8 +242,29 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
 #pragma unroll
                     for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i][s], bfr[j][s], acc[i][j]);
         }
-        if (kt + 1 < nk) store_slab(buf ^ 1);
+    };
+    if constexpr (NBUF == 2) {
+        if (nk > 0) {
+            load_slab(kbeg);
+            store_slab(0);
+        }
         __syncthreads();
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int buf = (int)(kt & 1);
+            if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
+            mma_slab(buf);
+            if (kt + 1 < nk) store_slab(buf ^ 1);
+            __syncthreads();
+        }
+    } else {
+        if (nk > 0) load_slab(kbeg);
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            if (kt > 0) __syncthreads();  // every wave is done reading the previous slab
+            store_slab(0);
+            __syncthreads();
+            if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);  // lands while the MFMAs run
+            mma_slab(0);
+        }
     }
 
     // epilogue: C = α·acc + β·C, or this split's raw partial product into the workspace
@@ -249,6 +289,23 @@ __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ 
                 }
             }
         }
+}
+
+template <typename TC, int BM, int BN, int BK>
+__global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                     int64_t ntiles, const double *__restrict__ A,
+                                                     const double *__restrict__ Bm, double *__restrict__ C,
+                                                     double *__restrict__ ws, double alpha, double beta) {
+    gemm_k4_body<TC, BM, BN, BK, 2>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
+}
+
+// 64×64×16 tiles fit 4 waves per SIMD (≤ 128 VGPR + AGPR) without spills when asked to
+template <typename TC, int BM, int BN, int BK>
+__global__ __launch_bounds__(NT, (BM * BN <= 64 * 64 && BK <= 16) ? 4 : 1) void gemm_k4s_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                      int64_t ntiles, const double *__restrict__ A,
+                                                      const double *__restrict__ Bm, double *__restrict__ C,
+                                                      double *__restrict__ ws, double alpha, double beta) {
+    gemm_k4_body<TC, BM, BN, BK, 1>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
 }
 
 // Split-K reduction: one thread per output element of the split groups.
@@ -536,9 +593,13 @@ struct SkinnySet {
 // Tile shapes the autotuner chooses from (BM × BN output tile, BK slab depth; the
 // 3×BF16 kernel always uses BK = 32).
 struct Cfg {
-    int bm, bn, bk;
+    int bm, bn, bk, nbuf;  // nbuf: LDS buffers of the FP64/FP32 kernel (2: gemm_k4, 1: gemm_k4s)
 };
-constexpr Cfg CFGS[] = {{64, 64, 16}, {128, 128, 16}, {128, 64, 16}, {64, 128, 16}, {64, 64, 32}};
+constexpr Cfg CFGS[] = {{64, 64, 16, 2}, {128, 128, 16, 2}, {128, 64, 16, 2}, {64, 128, 16, 2},
+                        {64, 64, 32, 2}, {64, 64, 16, 1},  {64, 64, 32, 1},  {128, 128, 16, 1}};
+// the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
+// of a configuration matter to it
+constexpr bool bf16_duplicate(int c) { return c == 4 || c == 5 || c == 6 || c == 7; }
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 }  // namespace
@@ -609,23 +670,30 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     return rc;
 }
 
-int auto_cfg(const tkmi_gemm_plan_s *p) {
-    // 128×128 tiles when that still gives at least two tiles per CU, else 64×64
-    return (p->ts[1].ntiles >= 512) ? 1 : 0;
+int auto_cfg(const tkmi_gemm_plan_s *) {
+    // 64×64×16 with one LDS buffer: the fastest on the sector GEMMs measured (K or N of
+    // a few hundred); the autotuner moves a plan elsewhere when another is > 3% faster
+    return 5;
 }
 
 int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
 
-template <int BM, int BN, int BK>
+template <int BM, int BN, int BK, int NBUF>
 void launch_k(const TileSet &ts, int prec, const double *A, const double *B, double *C, double alpha, double beta,
               hipStream_t s) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
-    if (prec == TKMI_PREC_F64)
+    if (prec == TKMI_PREC_F64 && NBUF == 2)
         hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
                            B, C, ts.ws, alpha, beta);
-    else if (prec == TKMI_PREC_F32)
+    else if (prec == TKMI_PREC_F64)
+        hipLaunchKernelGGL((gemm_k4s_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, ts.ws, alpha, beta);
+    else if (prec == TKMI_PREC_F32 && NBUF == 2)
         hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, ts.ws, alpha, beta);
+    else if (prec == TKMI_PREC_F32)
+        hipLaunchKernelGGL((gemm_k4s_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
                            B, C, ts.ws, alpha, beta);
     else
         hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B, C,
@@ -644,11 +712,14 @@ int launch_cfg(const tkmi_gemm_plan_s *p, int c, int prec, const double *A, cons
     if (ts.ntiles * NT >= ((int64_t)1 << 31))
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more output tiles than one launch can hold (2^31 work-items)");
     switch (c) {
-        case 0: launch_k<64, 64, 16>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 1: launch_k<128, 128, 16>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 2: launch_k<128, 64, 16>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 3: launch_k<64, 128, 16>(ts, prec, A, B, C, alpha, beta, s); break;
-        default: launch_k<64, 64, 32>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 0: launch_k<64, 64, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 1: launch_k<128, 128, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 2: launch_k<128, 64, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 3: launch_k<64, 128, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 4: launch_k<64, 64, 32, 2>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 5: launch_k<64, 64, 16, 1>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 6: launch_k<64, 64, 32, 1>(ts, prec, A, B, C, alpha, beta, s); break;
+        default: launch_k<128, 128, 16, 1>(ts, prec, A, B, C, alpha, beta, s); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
@@ -778,11 +849,12 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
     return TKMI_OK;
 }
 
-int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk) {
+int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf) {
     if (config < 0 || config >= NCFG) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown configuration");
     if (bm) *bm = CFGS[config].bm;
     if (bn) *bn = CFGS[config].bn;
     if (bk) *bk = CFGS[config].bk;
+    if (nbuf) *nbuf = CFGS[config].nbuf;
     return TKMI_OK;
 }
 
@@ -822,7 +894,7 @@ int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double
     for (int round = 0; round < ROUNDS && rc == TKMI_OK; ++round) {
         for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
             if (plan->ts[c].ntiles == 0) continue;
-            if (precision == TKMI_PREC_BF16X3 && c == 4) continue;  // same shape as config 0 for that kernel
+            if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
             if ((rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
             if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
             for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s);

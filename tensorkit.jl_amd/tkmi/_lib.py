@@ -47,7 +47,7 @@ SIGNATURES = [
       c_i32p, c_i64p, c_i64p]),
     ("tkmi_gemm_exec", C.c_int, [c_vp, C.c_int32, C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_gemm_plan_set_tile", C.c_int, [c_vp, C.c_int32, C.c_int32]),
-    ("tkmi_gemm_config_info", C.c_int, [C.c_int32, c_i32p, c_i32p, c_i32p]),
+    ("tkmi_gemm_config_info", C.c_int, [C.c_int32, c_i32p, c_i32p, c_i32p, c_i32p]),
     ("tkmi_gemm_exec_parts", C.c_int, [c_vp, C.c_int32, C.c_int32, C.c_double, C.c_double, c_vp, c_vp, c_vp,
                                        c_vp]),
     ("tkmi_gemm_plan_split", C.c_int, [c_vp, c_f64p, c_f64p, c_i64p]),
