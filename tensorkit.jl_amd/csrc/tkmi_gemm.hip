@@ -40,6 +40,18 @@ struct GGroup {
     int64_t splits;  // split-K factor (MFMA kernels): tiles_m·tiles_n·splits workgroups
     int64_t kchunk;  // K elements per split (multiple of 32)
     int64_t ws_off;  // split > 1: partial products at ws[ws_off + z·m·n], z < splits
+    // 3×BF16 split panels of this group: A as 3 planes [mp][kp] at pa (plane stride sa),
+    // B as 3 planes [np][kp] at pb (stride sb); rows padded to 128, K to 32, zero filled
+    int64_t pa, pb, kp, sa, sb;
+};
+
+// Work of the split/pack pre-pass for one MFMA group: nA = mp·kp/8 octets of A rows,
+// then np·kp/8 octets of B columns, numbered from start.
+struct PackGroup {
+    int64_t m, n, k, a_off, lda, b_off, ldb;
+    int32_t ta, tb;
+    int64_t mp, np, kp, pa, pb, sa, sb;
+    int64_t start, nA;
 };
 
 // Split-K reduction record: C = α·Σ_z ws[ws_off + z·m·n + ·] + β·C, fixed z order.
@@ -333,24 +345,74 @@ __global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ r
 
 
 // --------------------------------------------------------------------------------
-// 3×BF16 split-product kernel: x = x0 + x1 + x2 (bf16 each), products
+// 3×BF16 split-product GEMM: x = x0 + x1 + x2 (bf16 each), products
 // x0y0 (high), x0y1 + x1y0 + x0y2 + x1y1 + x2y0 (low) on v_mfma_f32_16x16x32_bf16,
 // FP32 accumulation per BK=32 slab, high part folded into FP64 after every slab.
+//
+// split_pack_kernel splits every operand element once per call into the three bf16
+// planes and packs each group's A rows and B columns K-contiguous, zero padded to
+// whole tiles and slabs; the GEMM then stages 16-byte vectors straight into LDS with
+// no conversion and no bounds test in its K loop.
 // --------------------------------------------------------------------------------
 __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
 
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));  // 16 bytes = 8 bf16
+
+__global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restrict__ pg, int64_t ng, int64_t total,
+                                                        const double *__restrict__ A, const double *__restrict__ Bm,
+                                                        __bf16 *__restrict__ P) {
+    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * NT) {
+        int64_t lo = 0, hi = ng - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (pg[mid].start <= q) lo = mid; else hi = mid - 1;
+        }
+        const PackGroup &G = pg[lo];
+        int64_t r = q - G.start;
+        const bool isA = r < G.nA;
+        if (!isA) r -= G.nA;
+        const int64_t rows = isA ? G.mp : G.np;
+        const int64_t i = r % rows, k8 = r / rows;  // lanes run along rows: coalesced when that side is contiguous
+        const int64_t lim = isA ? G.m : G.n;
+        __bf16 x[3][8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int64_t kk = 8 * k8 + t;
+            double v = 0.0;
+            if (i < lim && kk < G.k) {
+                if (isA) v = G.ta ? A[G.a_off + kk + i * G.lda] : A[G.a_off + i + kk * G.lda];
+                else     v = G.tb ? Bm[G.b_off + i + kk * G.ldb] : Bm[G.b_off + kk + i * G.ldb];
+            }
+            const __bf16 x0 = to_bf16((float)v);
+            const double r1 = v - (double)(float)x0;
+            const __bf16 x1 = to_bf16((float)r1);
+            const double r2 = r1 - (double)(float)x1;
+            x[0][t] = x0;
+            x[1][t] = x1;
+            x[2][t] = to_bf16((float)r2);
+        }
+        __bf16 *dst = P + (isA ? G.pa : G.pb) + i * G.kp + 8 * k8;
+        const int64_t plane = isA ? G.sa : G.sb;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            u4_t v;
+            __builtin_memcpy(&v, x[p], sizeof(v));
+            *reinterpret_cast<u4_t *>(dst + p * plane) = v;
+        }
+    }
+}
+
 template <int BM, int BN>
 __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                         int64_t ntiles, const double *__restrict__ A,
-                                                         const double *__restrict__ Bm, double *__restrict__ C,
-                                                         double *__restrict__ ws, double alpha, double beta) {
+                                                         int64_t ntiles, const __bf16 *__restrict__ P,
+                                                         double *__restrict__ C, double *__restrict__ ws,
+                                                         double alpha, double beta) {
     constexpr int BK = 32;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
-    constexpr int AE = BM * BK / NT;
-    constexpr int BE = BN * BK / NT;
-    constexpr int KP = BK + 8;  // row pitch (bf16 elements), keeps 16-byte alignment
-    // split parts: [part][row][k]
+    constexpr int AC = 3 * BM * (BK / 8) / NT;  // 16-byte chunks of the A slab per thread
+    constexpr int BC = 3 * BN * (BK / 8) / NT;
+    constexpr int KP = BK + 8;  // row pitch (bf16 elements): 80 bytes, keeps 16-byte alignment
     __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][KP];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][KP];
 
@@ -364,10 +426,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
     const int64_t kbeg = z * G.kchunk;
     const int64_t K = min(G.k, kbeg + G.kchunk);  // end of this split's K range
     const int64_t row0 = tm * BM, col0 = tn * BN;
-    const double *Ag = A + G.a_off;
-    const double *Bg = Bm + G.b_off;
-    const int64_t lda = G.lda, ldb = G.ldb;
-    const int ta = G.ta, tb = G.tb;
+    const int64_t kp = G.kp;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -383,67 +442,39 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
             lo[i][j] = f4_t{0, 0, 0, 0};
         }
 
-    double ra[AE], rb[BE];
-    auto a_coord = [&](int e, int &i, int &l) {
-        const int idx = e * NT + tid;
-        if (!ta) { i = idx % BM; l = idx / BM; }
-        else     { l = idx % BK; i = idx / BK; }
-    };
-    auto b_coord = [&](int e, int &l, int &j) {
-        const int idx = e * NT + tid;
-        if (!tb) { l = idx % BK; j = idx / BK; }
-        else     { j = idx % BN; l = idx / BN; }
-    };
+    // chunk c of a slab: plane c / (4·rows), row (c % (4·rows)) / 4, k octet c % 4
+    u4_t ra[AC], rb[BC];
+    const __bf16 *Ap = P + G.pa + row0 * kp;
+    const __bf16 *Bp = P + G.pb + col0 * kp;
     auto load_slab = [&](int64_t k0) {
 #pragma unroll
-        for (int e = 0; e < AE; ++e) {
-            int i, l;
-            a_coord(e, i, l);
-            const int64_t gi = row0 + i, gl = k0 + l;
-            ra[e] = (gi < m && gl < K) ? (ta ? Ag[gl + gi * lda] : Ag[gi + gl * lda]) : 0.0;
+        for (int e = 0; e < AC; ++e) {
+            const int c = e * NT + tid, p = c / (4 * BM), rem = c % (4 * BM);
+            ra[e] = *reinterpret_cast<const u4_t *>(Ap + p * G.sa + (rem >> 2) * kp + k0 + 8 * (rem & 3));
         }
 #pragma unroll
-        for (int e = 0; e < BE; ++e) {
-            int l, j;
-            b_coord(e, l, j);
-            const int64_t gj = col0 + j, gl = k0 + l;
-            rb[e] = (gj < n && gl < K) ? (tb ? Bg[gj + gl * ldb] : Bg[gl + gj * ldb]) : 0.0;
+        for (int e = 0; e < BC; ++e) {
+            const int c = e * NT + tid, p = c / (4 * BN), rem = c % (4 * BN);
+            rb[e] = *reinterpret_cast<const u4_t *>(Bp + p * G.sb + (rem >> 2) * kp + k0 + 8 * (rem & 3));
         }
-    };
-    auto split3 = [](double x, __bf16 &x0, __bf16 &x1, __bf16 &x2) {
-        x0 = to_bf16((float)x);
-        const double r1 = x - (double)(float)x0;
-        x1 = to_bf16((float)r1);
-        const double r2 = r1 - (double)(float)x1;
-        x2 = to_bf16((float)r2);
     };
     auto store_slab = [&]() {
 #pragma unroll
-        for (int e = 0; e < AE; ++e) {
-            int i, l;
-            a_coord(e, i, l);
-            __bf16 x0, x1, x2;
-            split3(ra[e], x0, x1, x2);
-            As[0][i][l] = x0;
-            As[1][i][l] = x1;
-            As[2][i][l] = x2;
+        for (int e = 0; e < AC; ++e) {
+            const int c = e * NT + tid, p = c / (4 * BM), rem = c % (4 * BM);
+            *reinterpret_cast<u4_t *>(&As[p][rem >> 2][8 * (rem & 3)]) = ra[e];
         }
 #pragma unroll
-        for (int e = 0; e < BE; ++e) {
-            int l, j;
-            b_coord(e, l, j);
-            __bf16 x0, x1, x2;
-            split3(rb[e], x0, x1, x2);
-            Bs[0][j][l] = x0;
-            Bs[1][j][l] = x1;
-            Bs[2][j][l] = x2;
+        for (int e = 0; e < BC; ++e) {
+            const int c = e * NT + tid, p = c / (4 * BN), rem = c % (4 * BN);
+            *reinterpret_cast<u4_t *>(&Bs[p][rem >> 2][8 * (rem & 3)]) = rb[e];
         }
     };
 
     const int64_t nk = (K - kbeg + BK - 1) / BK;
     if (nk > 0) load_slab(kbeg);
     for (int64_t kt = 0; kt < nk; ++kt) {
-        __syncthreads();  // previous slab fully consumed
+        if (kt > 0) __syncthreads();  // previous slab fully consumed
         store_slab();
         __syncthreads();
         if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
@@ -606,6 +637,11 @@ constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 struct tkmi_gemm_plan_s {
     std::vector<GGroup> base;  // MFMA groups without tiling
+    // 3×BF16 split panels (allocated by the first 3×BF16 call) and the pack work list
+    std::vector<PackGroup> pack;
+    int64_t pack_total = 0, panel_len = 0;
+    PackGroup *d_pack = nullptr;
+    __bf16 *panels = nullptr;
     SkinnySet skinny;
     double flops_mfma = 0, flops_skinny = 0;
     double flops = 0;
@@ -679,8 +715,8 @@ int auto_cfg(const tkmi_gemm_plan_s *) {
 int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
 
 template <int BM, int BN, int BK, int NBUF>
-void launch_k(const TileSet &ts, int prec, const double *A, const double *B, double *C, double alpha, double beta,
-              hipStream_t s) {
+void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const double *A, const double *B, double *C,
+              double alpha, double beta, hipStream_t s) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
     if (prec == TKMI_PREC_F64 && NBUF == 2)
@@ -695,9 +731,13 @@ void launch_k(const TileSet &ts, int prec, const double *A, const double *B, dou
     else if (prec == TKMI_PREC_F32)
         hipLaunchKernelGGL((gemm_k4s_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
                            B, C, ts.ws, alpha, beta);
-    else
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A, B, C,
-                           ts.ws, alpha, beta);
+    else {
+        const int64_t pblocks = std::min<int64_t>((p->pack_total + NT - 1) / NT, 16384);
+        hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)pblocks), block, 0, s, p->d_pack,
+                           (int64_t)p->pack.size(), p->pack_total, A, B, p->panels);
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
+                           (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
+    }
     if (ts.relems > 0) {
         const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
         hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
@@ -705,21 +745,31 @@ void launch_k(const TileSet &ts, int prec, const double *A, const double *B, dou
     }
 }
 
-int launch_cfg(const tkmi_gemm_plan_s *p, int c, int prec, const double *A, const double *B, double *C,
+int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const double *B, double *C,
                double alpha, double beta, hipStream_t s) {
     const TileSet &ts = p->ts[c];
     if (ts.ntiles == 0) return TKMI_OK;
+    if (prec == TKMI_PREC_BF16X3 && !p->panels) {
+        // first 3×BF16 call on this plan: split panels and pack list (not during graph capture)
+        int rc = tkmi::upload((void **)&p->panels, nullptr, (size_t)p->panel_len * sizeof(__bf16));
+        if (!rc) rc = tkmi::upload((void **)&p->d_pack, p->pack.data(), p->pack.size() * sizeof(PackGroup));
+        if (rc) {
+            (void)hipFree(p->panels);
+            p->panels = nullptr;
+            return rc;
+        }
+    }
     if (ts.ntiles * NT >= ((int64_t)1 << 31))
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more output tiles than one launch can hold (2^31 work-items)");
     switch (c) {
-        case 0: launch_k<64, 64, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 1: launch_k<128, 128, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 2: launch_k<128, 64, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 3: launch_k<64, 128, 16, 2>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 4: launch_k<64, 64, 32, 2>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 5: launch_k<64, 64, 16, 1>(ts, prec, A, B, C, alpha, beta, s); break;
-        case 6: launch_k<64, 64, 32, 1>(ts, prec, A, B, C, alpha, beta, s); break;
-        default: launch_k<128, 128, 16, 1>(ts, prec, A, B, C, alpha, beta, s); break;
+        case 0: launch_k<64, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 1: launch_k<128, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 2: launch_k<128, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 3: launch_k<64, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 4: launch_k<64, 64, 32, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 5: launch_k<64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 6: launch_k<64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        default: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
@@ -744,6 +794,8 @@ int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double 
 
 void free_gemm(tkmi_gemm_plan_s *p) {
     if (!p) return;
+    (void)hipFree(p->panels);
+    (void)hipFree(p->d_pack);
     (void)hipFree(p->skinny.d_groups);
     for (auto &t : p->ts) {
         (void)hipFree(t.d_groups);
@@ -827,6 +879,28 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
     std::stable_sort(p->base.begin(), p->base.end(), [](const GGroup &a, const GGroup &b) {
         return a.m * a.n * std::max<int64_t>(a.k, 1) > b.m * b.n * std::max<int64_t>(b.k, 1);
     });
+    // 3×BF16 panels: rows (A) and columns (B) padded to 128, K to 32
+    for (auto &G : p->base) {
+        G.kp = (G.k + 31) / 32 * 32;
+        const int64_t mp = (G.m + 127) / 128 * 128, np = (G.n + 127) / 128 * 128;
+        G.sa = mp * G.kp;
+        G.sb = np * G.kp;
+        G.pa = p->panel_len;
+        p->panel_len += 3 * G.sa;
+        G.pb = p->panel_len;
+        p->panel_len += 3 * G.sb;
+        PackGroup P;
+        std::memset(&P, 0, sizeof(P));
+        P.m = G.m; P.n = G.n; P.k = G.k;
+        P.a_off = G.a_off; P.lda = G.lda; P.ta = G.ta;
+        P.b_off = G.b_off; P.ldb = G.ldb; P.tb = G.tb;
+        P.mp = mp; P.np = np; P.kp = G.kp;
+        P.pa = G.pa; P.pb = G.pb; P.sa = G.sa; P.sb = G.sb;
+        P.start = p->pack_total;
+        P.nA = mp * (G.kp / 8);
+        p->pack_total += (mp + np) * (G.kp / 8);
+        p->pack.push_back(P);
+    }
     for (int c = 0; c < NCFG && !rc; ++c) rc = build_tiles(p, c);
     if (rc) {
         free_gemm(p);
