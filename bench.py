@@ -48,7 +48,7 @@ def gemm_kernel_name(prec, cfg, L):
     bm, bn, bk, nb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     L.tkmi_gemm_config_info(cfg, C.byref(bm), C.byref(bn), C.byref(bk), C.byref(nb))
     bm, bn, bk = bm.value, bn.value, bk.value
-    k4 = "gemm_k4_kernel" if nb.value == 2 else "gemm_k4s_kernel"
+    k4 = {2: "gemm_k4_kernel", 1: "gemm_k4s_kernel", 3: "gemm_k4p_kernel"}[nb.value]
     return {0: f"{k4}<double, {bm}, {bn}, {bk}>", 1: f"{k4}<float, {bm}, {bn}, {bk}>",
             2: f"gemm_bf16x3_kernel<{bm}, {bn}>"}[prec]
 

@@ -130,8 +130,9 @@ int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double 
 /* Tile shape for `precision`: 0 = automatic, 64 / 128 = square tiles with a
  * 16-deep K slab, 1000 + i = configuration i (see tkmi_gemm_config_info).      */
 int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile);
-/* Tile configuration i: BM×BN output tile, BK-deep K slab, LDS buffers of the FP64/FP32
- * kernel (2: double-buffered gemm_k4_kernel, 1: single-buffered gemm_k4s_kernel). */
+/* Tile configuration i: BM×BN output tile, BK-deep K slab and the FP64/FP32 kernel's
+ * staging (2: double-buffered LDS, gemm_k4_kernel; 1: one LDS buffer and one register
+ * stage, gemm_k4s_kernel; 3: one LDS buffer and two register stages, gemm_k4p_kernel). */
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
  * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64), bit 1 = the

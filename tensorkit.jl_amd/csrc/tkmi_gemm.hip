@@ -117,7 +117,8 @@ struct MfmaK4<float> {
 
 // NBUF = 2: double-buffered LDS, one barrier per slab.  NBUF = 1: one LDS buffer (half
 // the LDS, so more workgroups per CU), the next slab waits in registers while the
-// current one is multiplied; two barriers per slab.
+// current one is multiplied; two barriers per slab.  NBUF = 3: one LDS buffer and two
+// register stages, loads two slabs ahead.
 template <typename TC, int BM, int BN, int BK, int NBUF>
 __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, int64_t ngroups, int64_t ntiles,
                                              const double *__restrict__ A, const double *__restrict__ Bm,
@@ -131,8 +132,9 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     // row pitch an odd multiple of 16 bytes, so the 16-byte reads of 16 lanes (16 rows)
     // fall in distinct banks.
     constexpr int KP = BK + 16 / (int)sizeof(TC);
-    __shared__ __attribute__((aligned(16))) TC As[NBUF][BM][KP];
-    __shared__ __attribute__((aligned(16))) TC Bs[NBUF][BN][KP];
+    constexpr int NLDS = NBUF == 2 ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) TC As[NLDS][BM][KP];
+    __shared__ __attribute__((aligned(16))) TC Bs[NLDS][BN][KP];
     using acc_t = typename MfmaK4<TC>::acc_t;
 
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
@@ -193,44 +195,47 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     const double *__restrict__ Bslab = Bm + G.b_off + (tb ? col0 + kbeg * ldb : col0 * ldb + kbeg);
     const bool tile_full = (row0 + BM <= m) && (col0 + BN <= n);
 
+    // slabs are loaded strictly in K order: every load advances the slab pointers
     double ra[AE], rb[BE];
-    auto load_slab = [&](int64_t k0) {
+    auto load_into = [&](double (&xa)[AE], double (&xb)[BE], int64_t k0) {
         if (tile_full && k0 + BK <= K) {
 #pragma unroll
-            for (int e = 0; e < AE; ++e) ra[e] = Aslab[aoff[e]];
+            for (int e = 0; e < AE; ++e) xa[e] = Aslab[aoff[e]];
 #pragma unroll
-            for (int e = 0; e < BE; ++e) rb[e] = Bslab[boff[e]];
+            for (int e = 0; e < BE; ++e) xb[e] = Bslab[boff[e]];
         } else {
 #pragma unroll
             for (int e = 0; e < AE; ++e) {
                 int i, k;
                 a_el(e, i, k);
-                ra[e] = (row0 + i < m && k0 + k < K) ? Aslab[aoff[e]] : 0.0;
+                xa[e] = (row0 + i < m && k0 + k < K) ? Aslab[aoff[e]] : 0.0;
             }
 #pragma unroll
             for (int e = 0; e < BE; ++e) {
                 int j, k;
                 b_el(e, j, k);
-                rb[e] = (col0 + j < n && k0 + k < K) ? Bslab[boff[e]] : 0.0;
+                xb[e] = (col0 + j < n && k0 + k < K) ? Bslab[boff[e]] : 0.0;
             }
         }
         Aslab += astep;
         Bslab += bstep;
     };
-    auto store_slab = [&](int buf) {
+    auto store_from = [&](int buf, const double (&xa)[AE], const double (&xb)[BE]) {
 #pragma unroll
         for (int e = 0; e < AE; ++e) {
             int i, k;
             a_el(e, i, k);
-            As[buf][i][k] = (TC)ra[e];
+            As[buf][i][k] = (TC)xa[e];
         }
 #pragma unroll
         for (int e = 0; e < BE; ++e) {
             int j, k;
             b_el(e, j, k);
-            Bs[buf][j][k] = (TC)rb[e];
+            Bs[buf][j][k] = (TC)xb[e];
         }
     };
+    auto load_slab = [&](int64_t k0) { load_into(ra, rb, k0); };
+    auto store_slab = [&](int buf) { store_from(buf, ra, rb); };
 
     const int64_t nk = (K - kbeg + BK - 1) / BK;
     // MFMA s of a K slab uses k = 4*(lane/16) + s for both operands: each lane reads
@@ -268,7 +273,7 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
             if (kt + 1 < nk) store_slab(buf ^ 1);
             __syncthreads();
         }
-    } else {
+    } else if constexpr (NBUF == 1) {
         if (nk > 0) load_slab(kbeg);
         for (int64_t kt = 0; kt < nk; ++kt) {
             if (kt > 0) __syncthreads();  // every wave is done reading the previous slab
@@ -276,6 +281,25 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
             __syncthreads();
             if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);  // lands while the MFMAs run
             mma_slab(0);
+        }
+    } else {
+        // one LDS buffer, two register stages: loads run two slabs ahead of the MFMAs
+        double ra1[AE], rb1[BE];
+        if (nk > 0) load_into(ra, rb, kbeg);
+        if (nk > 1) load_into(ra1, rb1, kbeg + BK);
+        for (int64_t kt = 0; kt < nk; kt += 2) {
+            if (kt > 0) __syncthreads();
+            store_from(0, ra, rb);
+            __syncthreads();
+            if (kt + 2 < nk) load_into(ra, rb, kbeg + (kt + 2) * BK);
+            mma_slab(0);
+            if (kt + 1 < nk) {
+                __syncthreads();
+                store_from(0, ra1, rb1);
+                __syncthreads();
+                if (kt + 3 < nk) load_into(ra1, rb1, kbeg + (kt + 3) * BK);
+                mma_slab(0);
+            }
         }
     }
 
@@ -318,6 +342,14 @@ __global__ __launch_bounds__(NT, (BM * BN <= 64 * 64 && BK <= 16) ? 4 : 1) void 
                                                       const double *__restrict__ Bm, double *__restrict__ C,
                                                       double *__restrict__ ws, double alpha, double beta) {
     gemm_k4_body<TC, BM, BN, BK, 1>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
+}
+
+template <typename TC, int BM, int BN, int BK>
+__global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                      int64_t ntiles, const double *__restrict__ A,
+                                                      const double *__restrict__ Bm, double *__restrict__ C,
+                                                      double *__restrict__ ws, double alpha, double beta) {
+    gemm_k4_body<TC, BM, BN, BK, 3>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
 }
 
 // Split-K reduction: one thread per output element of the split groups.
@@ -624,13 +656,14 @@ struct SkinnySet {
 // Tile shapes the autotuner chooses from (BM × BN output tile, BK slab depth; the
 // 3×BF16 kernel always uses BK = 32).
 struct Cfg {
-    int bm, bn, bk, nbuf;  // nbuf: LDS buffers of the FP64/FP32 kernel (2: gemm_k4, 1: gemm_k4s)
+    int bm, bn, bk, nbuf;  // FP64/FP32 kernel: 2 gemm_k4 (two LDS buffers), 1 gemm_k4s, 3 gemm_k4p
 };
 constexpr Cfg CFGS[] = {{64, 64, 16, 2}, {128, 128, 16, 2}, {128, 64, 16, 2}, {64, 128, 16, 2},
-                        {64, 64, 32, 2}, {64, 64, 16, 1},  {64, 64, 32, 1},  {128, 128, 16, 1}};
+                        {64, 64, 32, 2}, {64, 64, 16, 1},  {64, 64, 32, 1},  {128, 128, 16, 1},
+                        {64, 64, 16, 3}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
 // of a configuration matter to it
-constexpr bool bf16_duplicate(int c) { return c == 4 || c == 5 || c == 6 || c == 7; }
+constexpr bool bf16_duplicate(int c) { return c >= 4; }
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 }  // namespace
@@ -719,18 +752,30 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
               double alpha, double beta, hipStream_t s) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
-    if (prec == TKMI_PREC_F64 && NBUF == 2)
-        hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
-    else if (prec == TKMI_PREC_F64)
-        hipLaunchKernelGGL((gemm_k4s_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
-    else if (prec == TKMI_PREC_F32 && NBUF == 2)
-        hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
-    else if (prec == TKMI_PREC_F32)
-        hipLaunchKernelGGL((gemm_k4s_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
+    if (prec != TKMI_PREC_BF16X3) {
+        if constexpr (NBUF == 3) {
+            if (prec == TKMI_PREC_F64)
+                hipLaunchKernelGGL((gemm_k4p_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
+                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
+            else
+                hipLaunchKernelGGL((gemm_k4p_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
+                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
+        } else if constexpr (NBUF == 2) {
+            if (prec == TKMI_PREC_F64)
+                hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
+                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
+            else
+                hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
+                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
+        } else {
+            if (prec == TKMI_PREC_F64)
+                hipLaunchKernelGGL((gemm_k4s_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
+                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
+            else
+                hipLaunchKernelGGL((gemm_k4s_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
+                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
+        }
+    }
     else {
         const int64_t pblocks = std::min<int64_t>((p->pack_total + NT - 1) / NT, 16384);
         hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)pblocks), block, 0, s, p->d_pack,
@@ -769,7 +814,8 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const doub
         case 4: launch_k<64, 64, 32, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
         case 5: launch_k<64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
         case 6: launch_k<64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        default: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 7: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        default: launch_k<64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
