@@ -616,8 +616,10 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
     if (rc) return rc;
     const int N = numind;
     if (nblocks < 0) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_generic_plan_create: negative block count");
+    // cols = 0 is allowed: a destination with no source contributions is only scaled by β
+    // (a partial trace whose traced sectors never match for that destination sub-block)
     for (int64_t b = 0; b < nblocks; ++b)
-        if (rows[b] < 1 || cols[b] < 1 || rows[b] > INT32_MAX || cols[b] > INT32_MAX)
+        if (rows[b] < 1 || cols[b] < 0 || rows[b] > INT32_MAX || cols[b] > INT32_MAX)
             TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_generic_plan_create: bad recoupling matrix size");
     auto *p = new tkmi_transform_plan_s();
     p->N = N;

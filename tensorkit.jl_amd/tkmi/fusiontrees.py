@@ -406,3 +406,23 @@ def permute_block(src, p1, p2, unit):
 
 def permute_pair(f1, f2, p1, p2):
     return braid_pair(f1, f2, p1, p2)
+
+
+def split_tree(f, M):
+    """`split(f, M)` (`src/fusiontrees/basic_manipulations.jl:30-55`): the first M
+    uncoupled sectors of f fused to the inner sector between legs M and M+1, and a
+    second tree whose first uncoupled sector is that inner sector followed by the
+    remaining legs, coupling to f.coupled.  0-based M, 0 ≤ M ≤ N."""
+    N = len(f.uncoupled)
+    if not 0 <= M <= N:
+        raise ValueError(f"M should be between 0 and N = {N}")
+    if N == 0:
+        return f, FusionTree((f.coupled,), f.coupled, (False,), (), ())
+    inner_ext = (f.uncoupled[0],) + tuple(f.innerlines) + (f.coupled,)
+    vert_ext = (1,) + tuple(f.vertices)
+    coupled1 = f.coupled.unit() if M == 0 else inner_ext[M - 1]
+    f1 = FusionTree(f.uncoupled[:M], coupled1, f.isdual[:M], f.innerlines[:max(0, M - 2)],
+                    f.vertices[:max(0, M - 1)])
+    f2 = FusionTree((coupled1,) + f.uncoupled[M:], f.coupled, (False,) + f.isdual[M:],
+                    inner_ext[M:M + max(0, N - M - 1)], vert_ext[M:N])
+    return f1, f2

@@ -214,9 +214,11 @@ class ProductSpace:
 class HomSpace:
     """codomain ← domain."""
 
-    __slots__ = ("codomain", "domain", "_h", "_cache")
+    __slots__ = ("codomain", "domain", "_h", "_cache", "_unit")
 
-    def __init__(self, codomain, domain):
+    def __init__(self, codomain, domain, unit=None):
+        """`unit`: the unit sector, needed only when both sides are empty (a scalar:
+        TensorKit's empty ProductSpace still carries its sector type)."""
         if isinstance(codomain, GradedSpace):
             codomain = ProductSpace(codomain)
         if isinstance(domain, GradedSpace):
@@ -225,6 +227,7 @@ class HomSpace:
         self.domain = domain
         self._h = hash((codomain, domain))
         self._cache = {}
+        self._unit = unit
 
     numout = property(lambda self: len(self.codomain))
     numin = property(lambda self: len(self.domain))
@@ -250,6 +253,8 @@ class HomSpace:
         for V in list(self.codomain) + list(self.domain):
             for k in V._keys:
                 return k.unit()
+        if self._unit is not None:
+            return self._unit
         raise ValueError("HomSpace without sectors")
 
     @property
@@ -263,7 +268,7 @@ class HomSpace:
         """`select(W, (p₁, p₂))` (`homspace.jl:240-244`)."""
         cod = ProductSpace(*(self[n] for n in p1))
         dom = ProductSpace(*(self[n].dual() for n in p2))
-        return HomSpace(cod, dom)
+        return HomSpace(cod, dom, unit=self.unit)
 
     def permute(self, p1, p2):
         """`permute(W, (p₁, p₂))` (`homspace.jl:212-217`); 0-based indices."""

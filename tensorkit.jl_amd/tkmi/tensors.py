@@ -20,8 +20,9 @@ import torch
 
 from . import _lib
 from .spaces import HomSpace, SpaceMismatch, IndexError_
-from .sectors import FERMIONIC
-from .treetransformers import treebraider, flat_plan, twist_factor
+from .sectors import BOSONIC, FERMIONIC
+from .spaces import ProductSpace, SectorMismatch
+from .treetransformers import treebraider, flat_plan, twist_factor, tracer, _make_trace_plan, _TRANSFORMER_CACHE
 
 _DT = {torch.float64: _lib.TKMI_F64, torch.float32: _lib.TKMI_F32}
 
@@ -316,6 +317,90 @@ def twist_(t, inds):
     tmp = t.copy()
     add_transform_(t, tmp, t.codomainind(), t.domainind(), 1.0, 0.0, dst_twist=tuple(inds))
     return t
+
+
+# ---------------------------------------------------------------------------
+# partial trace: trace_permute! / TO.tensortrace! / planartrace! / tr
+# ---------------------------------------------------------------------------
+def trace_permute_(tdst, tsrc, p, q, alpha=1.0, beta=0.0):
+    """`trace_permute!(tdst, tsrc, (p₁, p₂), (q₁, q₂), α, β)`
+    (`src/tensors/tensoroperations.jl:203-300`): tdst = β·tdst + α·(tsrc with the legs
+    q₁[i], q₂[i] traced and the rest permuted to (p₁, p₂)).  One device transform:
+    each destination sub-block sums the traced diagonal slices of the source sub-blocks
+    whose traced remainders match, with the reference's coefficients."""
+    (p1, p2), (q1, q2) = (tuple(p[0]), tuple(p[1])), (tuple(q[0]), tuple(q[1]))
+    if tsrc.space.unit.braiding_style not in (BOSONIC, FERMIONIC):
+        raise SectorMismatch("only tensors with symmetric braiding rules can be contracted; try `@planar` instead")
+    if len(q1) != len(q2):
+        raise IndexError_("number of trace indices does not match")
+    if sorted(p1 + p2 + q1 + q2) != list(range(tsrc.numind)):
+        raise IndexError_(f"{(p1, p2)}, {(q1, q2)} do not cover the indices of {tsrc.space}")
+    W = tsrc.space
+    if tdst.space != W.select(p1, p2):
+        raise SpaceMismatch(f"trace: tsrc = {W}, tdst = {tdst.space}, p₁ = {p1}, p₂ = {p2}")
+    if any(W[a] != W[b].dual() for a, b in zip(q1, q2)):
+        raise SpaceMismatch(f"trace: tsrc = {W}, q₁ = {q1}, q₂ = {q2}")
+    if tdst.dtype != tsrc.dtype:
+        raise TypeError("trace_permute!: dtype mismatch")
+    if isinstance(tsrc, AdjointTensorMap):
+        tsrc = permute(tsrc, tsrc.codomainind(), tsrc.domainind(), copy=True)
+    plan = tracer(tdst.space, W, p1, p2, q1, q2)
+    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(),
+                                               tdst.data.data_ptr(), float(alpha), float(beta), _stream()),
+               "trace_permute!")
+    return tdst
+
+
+def tensortrace_(C, A, p, q, conjA=False, alpha=1.0, beta=0.0):
+    """`TO.tensortrace!(C, A, p, q, conjA, α, β)` (`tensoroperations.jl:82-98`)."""
+    if conjA:
+        Ap = A.adjoint()
+        pc = _canonicalize(p, C)
+        p2 = (adjointtensorindices(A, pc[0]), adjointtensorindices(A, pc[1]))
+        q2 = (adjointtensorindices(A, q[0]), adjointtensorindices(A, q[1]))
+        return trace_permute_(C, Ap, p2, q2, alpha, beta)
+    return trace_permute_(C, A, _canonicalize(p, C), q, alpha, beta)
+
+
+def tensortrace(A, p, q, conjA=False, alpha=1.0):
+    """Allocating form: the result's space is `select(space(A), p)`."""
+    Asp = A.space if not conjA else A.space.adjoint()
+    C = TensorMap.empty(Asp.select(tuple(p[0]), tuple(p[1])), A.dtype)
+    return tensortrace_(C, A, p, q, conjA, alpha, 0.0)
+
+
+def planartrace_(C, A, p, q, alpha=1.0, beta=0.0):
+    """`planartrace!` (`src/planar/planaroperations.jl:66-111`): bosonic sectors
+    forward to `trace_permute!`."""
+    if A.space.unit.braiding_style != BOSONIC:
+        raise NotImplementedError("planartrace! for non-bosonic sectors")
+    return trace_permute_(C, A, p, q, alpha, beta)
+
+
+def tr(t):
+    """`tr(t)` = Σ_c dim(c)·tr(block_c) (`src/tensors/linalg.jl:319-327`), for
+    domain(t) == codomain(t): one device transform summing every block diagonal (with
+    weight dim(c)) into a scalar."""
+    W = t.space
+    if W.domain != W.codomain:
+        raise SpaceMismatch("Trace of a tensor only exist when domain == codomain")
+    if isinstance(t, AdjointTensorMap):
+        t = permute(t, t.codomainind(), t.domainind(), copy=True)
+    key = ("tr", W)
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is None:
+        coeffs, slices = [], []
+        for c, (d1, d2), off in W.degeneracystructure()[1]:
+            for i in range(d1):
+                slices.append([off + i * (d1 + 1)])
+                coeffs.append(float(c.dim()))
+        rows = [(np.array(coeffs, dtype=np.float64).reshape(1, -1), (), [[0]], slices)]
+        plan = _make_trace_plan(rows, 0, 1, W.dim())
+        _TRANSFORMER_CACHE.put(key, plan)
+    out = torch.empty(1, dtype=t.dtype, device=_device())
+    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[t.dtype], t.data.data_ptr(), out.data_ptr(),
+                                               1.0, 0.0, _stream()), "tr")
+    return float(out.item())
 
 
 # ---------------------------------------------------------------------------

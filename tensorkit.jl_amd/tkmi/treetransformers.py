@@ -15,12 +15,13 @@ The serialized table is handed to `tkmi_abelian_plan_create` /
 """
 from __future__ import annotations
 
+import itertools
 from collections import OrderedDict
 
 import numpy as np
 
 from . import _lib
-from .fusiontrees import braid_pair, braid_block
+from .fusiontrees import braid_pair, braid_block, permute_block, split_tree
 from .sectors import UNIQUE
 
 DEFAULT_GLOBALCACHE_SIZE = 10 ** 4  # src/auxiliary/caches.jl:19
@@ -245,4 +246,95 @@ def flat_plan(length):
            "src": np.array([[length, 1, 0]] if length else np.zeros((0, 3)))}
     plan = _make_abelian_plan(tab, (0,), length, length)
     _TRANSFORMER_CACHE.put(key, plan)
+    return plan
+
+
+# ---------------------------------------------------------------------------
+# partial trace (trace_permute!) as a transform: every destination sub-block sums
+# the traced diagonal slices of the source sub-blocks that reach it
+# ---------------------------------------------------------------------------
+def trace_table(Vdst, Vsrc, p1, p2, q1, q2):
+    """Rows of a trace transform, one per destination sub-block (in `fusiontrees(Vdst)`
+    order): (coeffs, dst sizes, [dst (strides..., offset)], [source slice (strides...,
+    offset)]).  Restates `_trace_permute!` (`src/tensors/tensoroperations.jl:245-300`):
+    each source tree pair (or block column of the recoupling matrix) is permuted to
+    ((p1, q1), (p2, q2)), split after the kept legs, and contributes when both traced
+    remainders are the same tree, with the factor dim(coupled)/dim(first uncoupled) and
+    the twists of the non-dual traced legs.  A contribution is the diagonal sum over the
+    traced degeneracy indices, listed as one strided slice per index tuple, already in
+    destination order."""
+    fts_src = Vsrc.subblockstructure()
+    fts_dst = Vdst.subblockstructure()
+    n1, n2 = len(p1), len(p2)
+    r1, r2 = tuple(p1) + tuple(q1), tuple(p2) + tuple(q2)
+    contrib = {}
+
+    def add(coeff, g1full, g2full, spair):
+        f1pp, g1 = split_tree(g1full, n1)
+        f2pp, g2 = split_tree(g2full, n2)
+        if g1 != g2:
+            return
+        c = coeff * g1.coupled.dim() / g1.uncoupled[0].dim()
+        for i in range(1, len(g1.uncoupled)):
+            if not g1.isdual[i]:
+                c *= g1.uncoupled[i].twist()
+        contrib.setdefault((f1pp, f2pp), []).append((float(c), spair))
+
+    unit = Vsrc.unit
+    if unit.fusion_style == UNIQUE:
+        for spair in Vsrc.fusiontrees():
+            (g1, g2), coeff = braid_pair(spair[0], spair[1], r1, r2)
+            add(coeff, g1, g2, spair)
+    else:
+        for blk in Vsrc.fusionblocks():
+            dst, U = permute_block(blk, r1, r2, unit)
+            for i, spair in enumerate(blk.trees):
+                for j, (g1, g2) in enumerate(dst.trees):
+                    if U[j, i] != 0:
+                        add(U[j, i], g1, g2, spair)
+    keep = tuple(p1) + tuple(p2)
+    rows = []
+    for dpair in Vdst.fusiontrees():
+        szd, std, offd = fts_dst[dpair]
+        coeffs, slices = [], []
+        for c, spair in contrib.get(dpair, []):
+            szs, sts, offs = fts_src[spair]
+            if tuple(szs[x] for x in keep) != tuple(szd) or any(szs[a] != szs[b] for a, b in zip(q1, q2)):
+                raise AssertionError("trace: sub-block sizes do not match")
+            base = [sts[x] for x in keep]
+            diag = [sts[a] + sts[b] for a, b in zip(q1, q2)]
+            for t in itertools.product(*[range(szs[a]) for a in q1]):
+                slices.append(base + [offs + sum(tk * dk for tk, dk in zip(t, diag))])
+                coeffs.append(c)
+        rows.append((np.array(coeffs, dtype=np.float64).reshape(1, len(coeffs)), tuple(szd),
+                     [list(std) + [offd]], slices))
+    return rows
+
+
+def _make_trace_plan(rows, n, dst_len, src_len):
+    lib = _lib.ensure_device()
+    nb = len(rows)
+    rws = np.ones(nb, dtype=np.int64)
+    cls = np.array([r[0].shape[1] for r in rows], dtype=np.int64)
+    U = np.concatenate([r[0].reshape(-1) for r in rows] + [np.zeros(0)])
+    szd = np.array([r[1] for r in rows], dtype=np.int64).reshape(-1)
+    sod = np.array([x for r in rows for x in r[2]], dtype=np.int64).reshape(-1)
+    sos = np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1)
+    h = _lib.C.c_void_p()
+    pa, pp = _lib.i32(list(range(n)))
+    keep = [_lib.i64(rws), _lib.i64(cls), _lib.f64(U), _lib.i64(szd), _lib.i64(szd), _lib.i64(sod), _lib.i64(sos)]
+    _lib.check(lib.tkmi_generic_plan_create(_lib.C.byref(h), n, pp, nb, keep[0][1], keep[1][1], keep[2][1],
+                                            keep[3][1], keep[4][1], keep[5][1], keep[6][1], None, None,
+                                            int(dst_len), int(src_len)), "tkmi_generic_plan_create (trace)")
+    return TransformPlan(h, "trace", nb, None, rows)
+
+
+def tracer(Vdst, Vsrc, p1, p2, q1, q2):
+    """Cached device plan for `trace_permute!` from `Vsrc` to `Vdst`."""
+    key = ("trace", Vdst, Vsrc, tuple(p1), tuple(p2), tuple(q1), tuple(q2))
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is None:
+        rows = trace_table(Vdst, Vsrc, p1, p2, q1, q2)
+        plan = _make_trace_plan(rows, len(p1) + len(p2), Vdst.dim(), Vsrc.dim())
+        _TRANSFORMER_CACHE.put(key, plan)
     return plan
