@@ -174,6 +174,19 @@ int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, dou
 int tkmi_svd_plan_set_algorithm(tkmi_svd_plan plan, int32_t alg);
 int tkmi_svd_plan_destroy(tkmi_svd_plan plan);
 
+/* ---- eigh_full / eigh_trunc (symmetric blocks, FP64 on the device) ----------
+ * Per square block A_c (n_c × n_c at a_off[c], leading dimension lda[c], lower
+ * triangle read): A_c = V_c·diag(λ_c)·V_cᵀ, λ_c ascending.  Outputs packed block after
+ * block: λ_c (n_c values) and V_c (n_c×n_c, ld n_c), the layout of
+ * V: codomain ← V_r as a TensorMap.  The plan is a factorisation plan: sizes
+ * (λ length in s_len, V length in u_len), algorithm (divide and conquer = dsyevd,
+ * Jacobi = dsyevj, QR iteration = dsyev) and destruction go through the tkmi_svd_plan_*
+ * calls.  Replaces MAK.eigh_full! per block (src/factorizations/matrixalgebrakit.jl:34-51);
+ * truncation (eigh_trunc, by |λ|) uses tkmi_truncate_select. */
+int tkmi_eigh_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *n, const int64_t *a_off,
+                          const int64_t *lda);
+int tkmi_eigh_exec(tkmi_svd_plan plan, const double *A, double *W, double *V, void *stream);
+
 /* Keep-mask for truncrank(howmany) ∩ trunctol(atol) over all singular values
  * (weights = quantum dimension of each value's sector) and the truncation
  * error ε = sqrt(Σ_discarded w·σ²) (findtruncated / truncation_error!,

@@ -7,6 +7,8 @@
 //                 (MAK.svd_compact! per block, src/factorizations/matrixalgebrakit.jl:34-51).
 //                 Blocks are independent: they run on up to 8 concurrent lanes (stream +
 //                 rocBLAS handle each) forked from and joined back to the caller's stream.
+// tkmi_eigh_exec  per square block A_c = V_c·diag(λ_c)·V_cᵀ (symmetric, λ ascending) with
+//                 dsyevd / dsyevj / dsyev on the same lanes (MAK.eigh_full! per block).
 // tkmi_truncate   keep-mask over all singular values of all blocks for
 //                 truncrank(howmany) ∩ trunctol(atol) with quantum-dimension weights, and
 //                 the truncation error ε = sqrt(Σ_discarded w·σ²)
@@ -79,6 +81,7 @@ struct SvdLane {
 };
 
 struct tkmi_svd_plan_s {
+    int32_t kind = 0;  // 0: SVD (U, S, Vt), 1: symmetric eigendecomposition (V in U's slot, λ in S's)
     std::vector<int64_t> m, n, a_off, lda, r, u_off, s_off, v_off;
     int64_t u_len = 0, s_len = 0, v_len = 0;
     int32_t alg = TKMI_SVD_QRITERATION;
@@ -115,14 +118,34 @@ int run_lane(const tkmi_svd_plan_s *p, SvdLane &l, const double *A, double *U, d
     }
     for (size_t b : l.blocks) {
         const int64_t m = p->m[b], n = p->n[b], r = p->r[b];
-        if (hipMemcpy2DAsync(l.work, m * sizeof(double), A + p->a_off[b], p->lda[b] * sizeof(double),
-                             m * sizeof(double), n, hipMemcpyDeviceToDevice, l.stream) != hipSuccess) {
+        const rocblas_int im = (rocblas_int)m, in = (rocblas_int)n, ir = (rocblas_int)r;
+        double *Sb = S + p->s_off[b], *Ub = U + p->u_off[b], *Vb = Vt ? Vt + p->v_off[b] : nullptr;
+        // the drivers overwrite their input: SVD works on a packed copy, the
+        // eigensolvers on the block copied into the eigenvector output itself
+        if (hipMemcpy2DAsync(p->kind == 1 ? Ub : l.work, m * sizeof(double), A + p->a_off[b],
+                             p->lda[b] * sizeof(double), m * sizeof(double), n, hipMemcpyDeviceToDevice,
+                             l.stream) != hipSuccess) {
             err = "svd: block copy failed";
             return TKMI_ERR_HIP;
         }
-        const rocblas_int im = (rocblas_int)m, in = (rocblas_int)n, ir = (rocblas_int)r;
-        double *Sb = S + p->s_off[b], *Ub = U + p->u_off[b], *Vb = Vt + p->v_off[b];
         rocblas_status st;
+        if (p->kind == 1) {
+            // eigenvalues ascending, orthonormal eigenvectors in place (lower triangle read)
+            if (p->alg == TKMI_SVD_DIVIDEANDCONQUER)
+                st = rocsolver_dsyevd(l.handle, rocblas_evect_original, rocblas_fill_lower, in, Ub, in, Sb, l.E,
+                                      l.info);
+            else if (p->alg == TKMI_SVD_JACOBI)
+                st = rocsolver_dsyevj(l.handle, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower,
+                                      in, Ub, in, 0.0, l.residual, 100, l.sweeps, Sb, l.info);
+            else
+                st = rocsolver_dsyev(l.handle, rocblas_evect_original, rocblas_fill_lower, in, Ub, in, Sb, l.E,
+                                     l.info);
+            if (st != rocblas_status_success) {
+                err = "eigh: rocSOLVER driver failed";
+                return TKMI_ERR_HIP;
+            }
+            continue;
+        }
         if (p->alg == TKMI_SVD_DIVIDEANDCONQUER)
             st = rocsolver_dgesdd(l.handle, rocblas_svect_singular, rocblas_svect_singular, im, in, l.work, im, Sb,
                                   Ub, im, Vb, ir, l.info);
@@ -145,16 +168,21 @@ int run_lane(const tkmi_svd_plan_s *p, SvdLane &l, const double *A, double *U, d
 
 extern "C" {
 
-int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m, const int64_t *n,
-                         const int64_t *a_off, const int64_t *lda) {
+static int factor_plan_create(tkmi_svd_plan *plan, int32_t kind, int64_t nblocks, const int64_t *m,
+                              const int64_t *n, const int64_t *a_off, const int64_t *lda) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_svd_plan_create: null plan pointer");
     *plan = nullptr;
     auto *p = new tkmi_svd_plan_s();
+    p->kind = kind;
     std::vector<size_t> order;
     for (int64_t b = 0; b < nblocks; ++b) {
         if (m[b] < 0 || n[b] < 0 || lda[b] < std::max<int64_t>(1, m[b]) || m[b] > INT32_MAX || n[b] > INT32_MAX) {
             free_svd(p);
             TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_svd_plan_create: bad block dimensions");
+        }
+        if (kind == 1 && m[b] != n[b]) {
+            free_svd(p);
+            TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_eigh_plan_create: blocks must be square");
         }
         const int64_t r = std::min(m[b], n[b]);
         p->m.push_back(m[b]);
@@ -167,7 +195,7 @@ int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m,
         p->v_off.push_back(p->v_len);
         p->u_len += m[b] * r;
         p->s_len += r;
-        p->v_len += r * n[b];
+        p->v_len += kind == 1 ? 0 : r * n[b];
         if (r > 0) order.push_back((size_t)b);
     }
     // cost model m·n·r; largest block first onto the least loaded lane
@@ -196,7 +224,7 @@ int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m,
             TKMI_FAIL(TKMI_ERR_HIP, "tkmi_svd_plan_create: stream / rocBLAS handle creation failed");
         }
         int rc;
-        if ((rc = tkmi::upload((void **)&l.work, nullptr, std::max<int64_t>(1, l.maxmn) * sizeof(double))) ||
+        if ((rc = tkmi::upload((void **)&l.work, nullptr, std::max<int64_t>(1, kind == 1 ? 1 : l.maxmn) * sizeof(double))) ||
             (rc = tkmi::upload((void **)&l.E, nullptr, std::max<int64_t>(1, l.maxr) * sizeof(double))) ||
             (rc = tkmi::upload((void **)&l.residual, nullptr, sizeof(double))) ||
             (rc = tkmi::upload((void **)&l.sweeps, nullptr, sizeof(rocblas_int))) ||
@@ -209,12 +237,28 @@ int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m,
     return TKMI_OK;
 }
 
+int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m, const int64_t *n,
+                         const int64_t *a_off, const int64_t *lda) {
+    return factor_plan_create(plan, 0, nblocks, m, n, a_off, lda);
+}
+
+int tkmi_eigh_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *n, const int64_t *a_off,
+                          const int64_t *lda) {
+    return factor_plan_create(plan, 1, nblocks, n, n, a_off, lda);
+}
+
 int tkmi_svd_plan_sizes(tkmi_svd_plan plan, int64_t *u_len, int64_t *s_len, int64_t *vt_len) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: null plan");
     if (u_len) *u_len = plan->u_len;
     if (s_len) *s_len = plan->s_len;
     if (vt_len) *vt_len = plan->v_len;
     return TKMI_OK;
+}
+
+int tkmi_eigh_exec(tkmi_svd_plan plan, const double *A, double *W, double *V, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "eigh: null plan");
+    if (plan->kind != 1) TKMI_FAIL(TKMI_ERR_ARGUMENT, "eigh: plan was not made by tkmi_eigh_plan_create");
+    return tkmi_svd_exec(plan, A, V, W, nullptr, stream);
 }
 
 int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, double *Vt, void *stream) {

@@ -60,6 +60,8 @@ SIGNATURES = [
     ("tkmi_svd_exec", C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_svd_plan_destroy", C.c_int, [c_vp]),
     ("tkmi_svd_plan_set_algorithm", C.c_int, [c_vp, C.c_int32]),
+    ("tkmi_eigh_plan_create", C.c_int, [C.POINTER(c_vp), C.c_int64, c_i64p, c_i64p, c_i64p]),
+    ("tkmi_eigh_exec", C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_truncate_select", C.c_int, [C.c_int64, c_vp, c_vp, C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_dot", C.c_int, [C.c_int32, c_vp, c_vp, C.c_int64, c_vp, c_vp]),
     ("tkmi_graph_begin", C.c_int, [c_vp]),

@@ -153,3 +153,113 @@ def svd_trunc(t, howmany=None, atol=0.0, alg=None):
     _copy_leading(Vtt, Vt, vcopy)
     Sk = {c: S[o:o + kept[c]] for c, (o, r) in offs.items() if kept[c]}
     return Ut, Sk, Vtt, float(err.item())
+
+
+# ---------------------------------------------------------------------------
+# eigh_full / eigh_trunc (symmetric blocks)
+# ---------------------------------------------------------------------------
+class EighPlan:
+    def __init__(self, W):
+        blocks = [(c, d1, d2, off) for c, (d1, d2), off in W.degeneracystructure()[1]]
+        if any(d1 != d2 for _, d1, d2, _ in blocks):
+            from .spaces import SpaceMismatch
+            raise SpaceMismatch("eigh: every block must be square (domain ≅ codomain)")
+        self.blocks = blocks
+        arr = np.array([[d1, off, max(d1, 1)] for c, d1, d2, off in blocks], dtype=np.int64).reshape(-1, 3)
+        lib = _lib.ensure_device()
+        h = C.c_void_p()
+        cols = [_lib.i64(arr[:, i]) for i in range(3)]
+        _lib.check(lib.tkmi_eigh_plan_create(C.byref(h), len(blocks), cols[0][1], cols[1][1], cols[2][1]),
+                   "tkmi_eigh_plan_create")
+        self.handle = h
+        self._destroy = lib.tkmi_svd_plan_destroy
+        self.r = [d1 for _, d1, _, _ in blocks]
+
+    def __del__(self):
+        h, destroy = getattr(self, "handle", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
+            self.handle = None
+
+
+_EIGH_CACHE = {}
+
+
+def eigh_full(t, alg=None):
+    """`eigh_full(t)` for a symmetric t (`MAK.eigh_full!` per block,
+    `src/factorizations/matrixalgebrakit.jl:34-51`): (D, V) with D = (λ, offsets),
+    λ_c ascending per block, and V: codomain ← V_r orthogonal with t = V·D·Vᵀ.
+    Only the lower triangle of each block is read."""
+    W = t.space
+    code = SVD_ALGORITHMS.get(alg or DEFAULT_SVD_ALGORITHM)
+    if code is None:
+        raise ValueError(f"unknown eigh algorithm {alg!r}; expected one of {sorted(SVD_ALGORITHMS)}")
+    plan = _EIGH_CACHE.get(W)
+    if plan is None:
+        plan = EighPlan(W)
+        _EIGH_CACHE[W] = plan
+    counts = {c: r for (c, _, _, _), r in zip(plan.blocks, plan.r)}
+    Vr = _mid_space(W, counts)
+    V = T.TensorMap.empty(HomSpace(W.codomain, ProductSpace(Vr)))
+    lam = torch.empty(max(sum(plan.r), 1), dtype=torch.float64, device=T._device())
+    lib = _lib.load()
+    _lib.check(lib.tkmi_svd_plan_set_algorithm(plan.handle, code), "eigh algorithm")
+    _lib.check(lib.tkmi_eigh_exec(plan.handle, t.data.data_ptr(), lam.data_ptr(), V.data.data_ptr(), T._stream()),
+               "eigh")
+    offs, o = {}, 0
+    for (c, _, _, _), r in zip(plan.blocks, plan.r):
+        offs[c] = (o, r)
+        o += r
+    return (lam, offs), V
+
+
+def eigh_trunc(t, howmany=None, atol=0.0, alg=None):
+    """`eigh_trunc(t; trunc = truncrank(howmany) & trunctol(; atol))`: keeps the
+    eigenvalues of largest |λ| (dim(c)-weighted count, `truncation.jl:171-203`) that are
+    not below atol in magnitude; returns (D̃ = {c: kept λ_c, ascending}, Ṽ, ε) with
+    ε = sqrt(Σ_discarded dim(c)·λ²)."""
+    (lam, offs), V = eigh_full(t, alg)
+    R = sum(r for _, r in offs.values())
+    weights = torch.empty(max(R, 1), dtype=torch.float64, device=lam.device)
+    w_host = np.zeros(max(R, 1))
+    for c, (o, r) in offs.items():
+        w_host[o:o + r] = c.dim()
+    weights.copy_(torch.from_numpy(w_host))
+    keep = torch.empty(max(R, 1), dtype=torch.uint8, device=lam.device)
+    disc = torch.empty(max(R, 1), dtype=torch.float64, device=lam.device)
+    err = torch.empty(1, dtype=torch.float64, device=lam.device)
+    hm = float(howmany) if howmany is not None else float("inf")
+    lib = _lib.load()
+    _lib.check(lib.tkmi_truncate_select(R, lam.data_ptr(), weights.data_ptr(), hm, float(atol), keep.data_ptr(),
+                                        disc.data_ptr(), err.data_ptr(), T._stream()), "truncate")
+    keep_h = keep[:R].cpu().numpy().astype(bool)
+    kept = {c: int(keep_h[o:o + r].sum()) for c, (o, r) in offs.items()}
+    W = t.space
+    Vk = _mid_space(W, kept)
+    Vt_ = T.TensorMap.empty(HomSpace(W.codomain, ProductSpace(Vk)))
+    vS = {c: (d, off) for c, d, off in V.space.degeneracystructure()[1]}
+    vD = {c: (d, off) for c, d, off in Vt_.space.degeneracystructure()[1]}
+    # the kept columns of a block are runs of the ascending spectrum (largest |λ| sit at
+    # both ends): copy each run as one strided sub-matrix
+    copies, Dk = [], {}
+    for c, (o, r) in offs.items():
+        if not kept[c]:
+            continue
+        (n, _), soff = vS[c]
+        (_, k), doff = vD[c]
+        mask = keep_h[o:o + r]
+        j, dst = 0, 0
+        while j < r:
+            if mask[j]:
+                j1 = j
+                while j1 < r and mask[j1]:
+                    j1 += 1
+                copies.append((n, j1 - j, n, soff + j * n, n, doff + dst * n))
+                dst += j1 - j
+                j = j1
+            else:
+                j += 1
+        idx = torch.from_numpy(np.nonzero(mask)[0] + o).to(lam.device)
+        Dk[c] = lam.index_select(0, idx)
+    _copy_leading(Vt_, V, copies)
+    return Dk, Vt_, float(err.item())

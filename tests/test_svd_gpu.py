@@ -125,3 +125,71 @@ def test_svd_unknown_algorithm(T):
     t = T.TensorMap.randn(HomSpace(ProductSpace(V), ProductSpace(V)), np.random.default_rng(3))
     with pytest.raises(ValueError):
         svd_compact(t, "gesvd_magic")
+
+
+# ---------------------------------------------------------------------------
+# eigh_full / eigh_trunc
+# ---------------------------------------------------------------------------
+def _sym_tensor(T, W, rng):
+    x = np.zeros(W.dim())
+    for c, (d1, d2), off in W.degeneracystructure()[1]:
+        a = rng.standard_normal((d1, d1))
+        x[off:off + d1 * d1] = ((a + a.T) / 2).reshape(-1, order="F")
+    return T.TensorMap.from_numpy(x, W)
+
+
+def reference_eigh_truncation(vals, howmany, atol):
+    """vals: {c: (λ ascending, weight)} → {c: kept mask}, ε: the weighted count over
+    values sorted by descending |λ| (ties by position), then |λ| ≥ atol
+    (truncrank(howmany; by=abs) ∩ trunctol(; atol), truncation.jl:171-203)."""
+    allv = [(abs(l), w, c, i) for c, (ls, w) in vals.items() for i, l in enumerate(ls)]
+    order = sorted(range(len(allv)), key=lambda j: (-allv[j][0], j))
+    keep = [False] * len(allv)
+    tot = 0
+    for j in order:
+        tot += allv[j][1]
+        if tot > howmany:
+            break
+        keep[j] = True
+    masks = {c: np.zeros(len(ls), dtype=bool) for c, (ls, w) in vals.items()}
+    err2 = 0.0
+    for k, (a, w, c, i) in zip(keep, allv):
+        k = k and a >= atol
+        masks[c][i] = k
+        if not k:
+            err2 += w * a * a
+    return masks, np.sqrt(err2)
+
+
+@pytest.mark.parametrize("alg", ALGS)
+def test_eigh_full_and_trunc(T, alg):
+    from tkmi.factorizations import eigh_full, eigh_trunc
+    rng = np.random.default_rng(30)
+    for V, howmany in ((U1Space((0, 9), (1, 7), (-1, 6), (2, 3)), 40), (SU2Space((0, 4), (0.5, 3), (1, 2)), 60)):
+        W = HomSpace(ProductSpace(V, V), ProductSpace(V, V))
+        t = _sym_tensor(T, W, rng)
+        (lam, offs), Vm = eigh_full(t, alg)
+        Ab, Vb, lh = t.blocks_host(), Vm.blocks_host(), lam.cpu().numpy()
+        vals = {}
+        for c, A in Ab.items():
+            o, r = offs[c]
+            ref = np.linalg.eigvalsh(A)
+            np.testing.assert_allclose(lh[o:o + r], ref, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(ref).max()))
+            assert np.linalg.norm(Vb[c] @ np.diag(lh[o:o + r]) @ Vb[c].T - A) <= 1e-12 * np.linalg.norm(A)
+            assert np.linalg.norm(Vb[c].T @ Vb[c] - np.eye(r)) <= 1e-12 * max(1, r)
+            vals[c] = (ref, c.dim())
+        Dk, Vt, eps = eigh_trunc(t, howmany, 1e-3, alg)
+        masks, eps_ref = reference_eigh_truncation(vals, howmany, 1e-3)
+        assert {c: len(v) for c, v in Dk.items()} == {c: int(m.sum()) for c, m in masks.items() if m.any()}
+        assert abs(eps - eps_ref) <= 1e-10 * max(eps_ref, 1e-300) + 1e-13
+        Vtb = Vt.blocks_host()
+        d2 = 0.0
+        for c, A in Ab.items():
+            R = A.copy()
+            if c in Dk:
+                dk = Dk[c].cpu().numpy()
+                np.testing.assert_allclose(dk, vals[c][0][masks[c]], rtol=1e-12, atol=1e-12)
+                R = A - Vtb[c] @ np.diag(dk) @ Vtb[c].T
+            d2 += c.dim() * np.sum(R * R)
+        tnorm = np.sqrt(sum(c.dim() * np.sum(A * A) for c, A in Ab.items()))
+        assert abs(np.sqrt(d2) - eps) <= 1e-9 * eps + 1e-12 * tnorm
