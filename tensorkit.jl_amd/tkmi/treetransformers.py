@@ -158,42 +158,67 @@ def generic_table(Vdst, Vsrc, p1, p2, levels1=None, levels2=None, src_structure=
     return rows
 
 
-def _make_abelian_plan(tab, perm, dst_len, src_len):
+def pack_abelian(tab):
+    """The arrays `tkmi_abelian_plan_create` takes, from an abelian table."""
+    return {"coeffs": np.ascontiguousarray(tab["coeffs"], dtype=np.float64),
+            "dst": np.ascontiguousarray(np.asarray(tab["dst"]).reshape(-1), dtype=np.int64),
+            "src": np.ascontiguousarray(np.asarray(tab["src"]).reshape(-1), dtype=np.int64)}
+
+
+def pack_generic(rows):
+    """The arrays `tkmi_generic_plan_create` takes, from generic-transformer rows; the
+    per-tree weights dim(c) enter only the adjoint transform."""
+    return {
+        "rows": np.array([r[1].shape[0] for r in rows], dtype=np.int64),
+        "cols": np.array([r[1].shape[1] for r in rows], dtype=np.int64),
+        "U": np.concatenate([np.asarray(r[1], dtype=np.float64).reshape(-1, order="F") for r in rows]
+                            + [np.zeros(0)]),
+        "sz_dst": np.array([r[2] for r in rows], dtype=np.int64).reshape(-1),
+        "sz_src": np.array([r[4] for r in rows], dtype=np.int64).reshape(-1),
+        "so_dst": np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1),
+        "so_src": np.array([x for r in rows for x in r[5]], dtype=np.int64).reshape(-1),
+        "w_dst": np.array([t[0].coupled.dim() for r in rows for t in r[7].trees], dtype=np.float64),
+        "w_src": np.array([t[0].coupled.dim() for r in rows for t in r[6].trees], dtype=np.float64),
+    }
+
+
+def plan_from_packed(kind, perm, packed, dst_len, src_len, table=None):
+    """Create the device plan of a packed table (the same call whether the table was
+    just built or read back from a table file)."""
     lib = _lib.ensure_device()
-    n = len(perm)
     h = _lib.C.c_void_p()
     pa, pp = _lib.i32(perm)
-    ca, cp = _lib.f64(tab["coeffs"])
-    da, dp = _lib.i64(tab["dst"].reshape(-1))
-    sa, sp = _lib.i64(tab["src"].reshape(-1))
-    _lib.check(lib.tkmi_abelian_plan_create(_lib.C.byref(h), n, pp, len(tab["coeffs"]), cp, dp, sp,
-                                            int(dst_len), int(src_len)), "tkmi_abelian_plan_create")
-    return TransformPlan(h, "abelian", len(tab["coeffs"]), None, tab)
+    n = len(perm)
+    if kind == "abelian":
+        nent = len(packed["coeffs"])
+        ca, cp = _lib.f64(packed["coeffs"])
+        da, dp = _lib.i64(packed["dst"])
+        sa, sp = _lib.i64(packed["src"])
+        _lib.check(lib.tkmi_abelian_plan_create(_lib.C.byref(h), n, pp, nent, cp, dp, sp, int(dst_len),
+                                                int(src_len)), "tkmi_abelian_plan_create")
+    else:
+        nent = len(packed["rows"])
+        keep = [_lib.i64(packed["rows"]), _lib.i64(packed["cols"]), _lib.f64(packed["U"]),
+                _lib.i64(packed["sz_dst"]), _lib.i64(packed["sz_src"]), _lib.i64(packed["so_dst"]),
+                _lib.i64(packed["so_src"])]
+        wd = _lib.f64(packed["w_dst"]) if "w_dst" in packed else (None, None)
+        ws = _lib.f64(packed["w_src"]) if "w_src" in packed else (None, None)
+        _lib.check(lib.tkmi_generic_plan_create(_lib.C.byref(h), n, pp, nent, keep[0][1], keep[1][1], keep[2][1],
+                                                keep[3][1], keep[4][1], keep[5][1], keep[6][1], wd[1], ws[1],
+                                                int(dst_len), int(src_len)), "tkmi_generic_plan_create")
+    plan = TransformPlan(h, kind, nent, None, table if table is not None else packed)
+    plan.packed = packed
+    plan.perm = tuple(int(x) for x in perm)
+    plan.lens = (int(dst_len), int(src_len))
+    return plan
+
+
+def _make_abelian_plan(tab, perm, dst_len, src_len):
+    return plan_from_packed("abelian", perm, pack_abelian(tab), dst_len, src_len, table=tab)
 
 
 def _make_generic_plan(rows, perm, dst_len, src_len):
-    lib = _lib.ensure_device()
-    n = len(perm)
-    rws = np.array([r[1].shape[0] for r in rows], dtype=np.int64)
-    cls = np.array([r[1].shape[1] for r in rows], dtype=np.int64)
-    U = np.concatenate([np.asarray(r[1], dtype=np.float64).reshape(-1, order="F") for r in rows]) \
-        if rows else np.zeros(0)
-    szd = np.array([r[2] for r in rows], dtype=np.int64).reshape(-1)
-    szs = np.array([r[4] for r in rows], dtype=np.int64).reshape(-1)
-    sod = np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1)
-    sos = np.array([x for r in rows for x in r[5]], dtype=np.int64).reshape(-1)
-    h = _lib.C.c_void_p()
-    pa, pp = _lib.i32(perm)
-    # per-tree weights dim(c) for the adjoint (weighted-inner-product) transform
-    wd = np.array([t[0].coupled.dim() for r in rows for t in r[7].trees], dtype=np.float64)
-    ws = np.array([t[0].coupled.dim() for r in rows for t in r[6].trees], dtype=np.float64)
-    keep = [_lib.i64(rws), _lib.i64(cls), _lib.f64(U), _lib.i64(szd), _lib.i64(szs), _lib.i64(sod),
-            _lib.i64(sos), _lib.f64(wd), _lib.f64(ws)]
-    _lib.check(lib.tkmi_generic_plan_create(_lib.C.byref(h), n, pp, len(rows), keep[0][1], keep[1][1],
-                                            keep[2][1], keep[3][1], keep[4][1], keep[5][1], keep[6][1],
-                                            keep[7][1], keep[8][1], int(dst_len), int(src_len)),
-               "tkmi_generic_plan_create")
-    return TransformPlan(h, "generic", len(rows), None, rows)
+    return plan_from_packed("generic", perm, pack_generic(rows), dst_len, src_len, table=rows)
 
 
 def treebraider(Vdst, Vsrc, p1, p2, levels=None, *, adjoint_src=False, dst_twist=()):
@@ -311,22 +336,22 @@ def trace_table(Vdst, Vsrc, p1, p2, q1, q2):
     return rows
 
 
+def pack_trace(rows):
+    """Generic-plan arrays of trace rows (identity permutation, sources already in
+    destination order, no adjoint weights)."""
+    return {
+        "rows": np.ones(len(rows), dtype=np.int64),
+        "cols": np.array([r[0].shape[1] for r in rows], dtype=np.int64),
+        "U": np.concatenate([r[0].reshape(-1) for r in rows] + [np.zeros(0)]),
+        "sz_dst": np.array([r[1] for r in rows], dtype=np.int64).reshape(-1),
+        "sz_src": np.array([r[1] for r in rows], dtype=np.int64).reshape(-1),
+        "so_dst": np.array([x for r in rows for x in r[2]], dtype=np.int64).reshape(-1),
+        "so_src": np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1),
+    }
+
+
 def _make_trace_plan(rows, n, dst_len, src_len):
-    lib = _lib.ensure_device()
-    nb = len(rows)
-    rws = np.ones(nb, dtype=np.int64)
-    cls = np.array([r[0].shape[1] for r in rows], dtype=np.int64)
-    U = np.concatenate([r[0].reshape(-1) for r in rows] + [np.zeros(0)])
-    szd = np.array([r[1] for r in rows], dtype=np.int64).reshape(-1)
-    sod = np.array([x for r in rows for x in r[2]], dtype=np.int64).reshape(-1)
-    sos = np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1)
-    h = _lib.C.c_void_p()
-    pa, pp = _lib.i32(list(range(n)))
-    keep = [_lib.i64(rws), _lib.i64(cls), _lib.f64(U), _lib.i64(szd), _lib.i64(szd), _lib.i64(sod), _lib.i64(sos)]
-    _lib.check(lib.tkmi_generic_plan_create(_lib.C.byref(h), n, pp, nb, keep[0][1], keep[1][1], keep[2][1],
-                                            keep[3][1], keep[4][1], keep[5][1], keep[6][1], None, None,
-                                            int(dst_len), int(src_len)), "tkmi_generic_plan_create (trace)")
-    return TransformPlan(h, "trace", nb, None, rows)
+    return plan_from_packed("trace", tuple(range(n)), pack_trace(rows), dst_len, src_len, table=rows)
 
 
 def tracer(Vdst, Vsrc, p1, p2, q1, q2):
