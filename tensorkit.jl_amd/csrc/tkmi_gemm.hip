@@ -434,7 +434,9 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
     }
 }
 
-template <int BM, int BN>
+// STAGES: register stages of prefetched slabs (1: the next slab loads while the current
+// one is multiplied; 2: loads run two slabs ahead, for the short 32-deep bf16 slabs)
+template <int BM, int BN, int STAGES>
 __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
                                                          int64_t ntiles, const __bf16 *__restrict__ P,
                                                          double *__restrict__ C, double *__restrict__ ws,
@@ -475,41 +477,49 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
         }
 
     // chunk c of a slab: plane c / (4·rows), row (c % (4·rows)) / 4, k octet c % 4
-    u4_t ra[AC], rb[BC];
+    u4_t ra[STAGES][AC], rb[STAGES][BC];
     const __bf16 *Ap = P + G.pa + row0 * kp;
     const __bf16 *Bp = P + G.pb + col0 * kp;
-    auto load_slab = [&](int64_t k0) {
+    auto load_slab = [&](u4_t (&xa)[AC], u4_t (&xb)[BC], int64_t k0) {
 #pragma unroll
         for (int e = 0; e < AC; ++e) {
             const int c = e * NT + tid, p = c / (4 * BM), rem = c % (4 * BM);
-            ra[e] = *reinterpret_cast<const u4_t *>(Ap + p * G.sa + (rem >> 2) * kp + k0 + 8 * (rem & 3));
+            xa[e] = *reinterpret_cast<const u4_t *>(Ap + p * G.sa + (rem >> 2) * kp + k0 + 8 * (rem & 3));
         }
 #pragma unroll
         for (int e = 0; e < BC; ++e) {
             const int c = e * NT + tid, p = c / (4 * BN), rem = c % (4 * BN);
-            rb[e] = *reinterpret_cast<const u4_t *>(Bp + p * G.sb + (rem >> 2) * kp + k0 + 8 * (rem & 3));
+            xb[e] = *reinterpret_cast<const u4_t *>(Bp + p * G.sb + (rem >> 2) * kp + k0 + 8 * (rem & 3));
         }
     };
-    auto store_slab = [&]() {
+    auto store_slab = [&](const u4_t (&xa)[AC], const u4_t (&xb)[BC]) {
 #pragma unroll
         for (int e = 0; e < AC; ++e) {
             const int c = e * NT + tid, p = c / (4 * BM), rem = c % (4 * BM);
-            *reinterpret_cast<u4_t *>(&As[p][rem >> 2][8 * (rem & 3)]) = ra[e];
+            *reinterpret_cast<u4_t *>(&As[p][rem >> 2][8 * (rem & 3)]) = xa[e];
         }
 #pragma unroll
         for (int e = 0; e < BC; ++e) {
             const int c = e * NT + tid, p = c / (4 * BN), rem = c % (4 * BN);
-            *reinterpret_cast<u4_t *>(&Bs[p][rem >> 2][8 * (rem & 3)]) = rb[e];
+            *reinterpret_cast<u4_t *>(&Bs[p][rem >> 2][8 * (rem & 3)]) = xb[e];
         }
     };
 
     const int64_t nk = (K - kbeg + BK - 1) / BK;
-    if (nk > 0) load_slab(kbeg);
-    for (int64_t kt = 0; kt < nk; ++kt) {
+#pragma unroll
+    for (int st = 0; st < STAGES; ++st)
+        if (st < nk) load_slab(ra[st], rb[st], kbeg + st * BK);
+    // the stage index of slab kt is kt % STAGES; the loop is unrolled by STAGES so
+    // that every register array is indexed by a constant
+    for (int64_t kb = 0; kb < nk; kb += STAGES) {
+#pragma unroll
+    for (int st = 0; st < STAGES; ++st) {
+        const int64_t kt = kb + st;
+        if (kt >= nk) break;
         if (kt > 0) __syncthreads();  // previous slab fully consumed
-        store_slab();
+        store_slab(ra[st], rb[st]);
         __syncthreads();
-        if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
+        if (kt + STAGES < nk) load_slab(ra[st], rb[st], kbeg + (kt + STAGES) * BK);
         const int kq = 8 * (lane >> 4);
         bf8_t a[3][FM], b[3][FN];
 #pragma unroll
@@ -537,6 +547,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)h[v];
             }
+    }
     }
 
     const bool partial = G.splits > 1;
@@ -662,8 +673,8 @@ constexpr Cfg CFGS[] = {{64, 64, 16, 2}, {128, 128, 16, 2}, {128, 64, 16, 2}, {6
                         {64, 64, 32, 2}, {64, 64, 16, 1},  {64, 64, 32, 1},  {128, 128, 16, 1},
                         {64, 64, 16, 3}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
-// of a configuration matter to it
-constexpr bool bf16_duplicate(int c) { return c >= 4; }
+// of a configuration matter to it, and the prefetch depth of configuration 8
+constexpr bool bf16_duplicate(int c) { return c >= 4 && c != 8; }  // 8: two prefetch stages
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 }  // namespace
@@ -780,8 +791,12 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
         const int64_t pblocks = std::min<int64_t>((p->pack_total + NT - 1) / NT, 16384);
         hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)pblocks), block, 0, s, p->d_pack,
                            (int64_t)p->pack.size(), p->pack_total, A, B, p->panels);
-        hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
-                           (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
+        if constexpr (NBUF == 3)
+            hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 2>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
+                               (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
+        else
+            hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 1>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
+                               (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
     }
     if (ts.relems > 0) {
         const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
