@@ -883,6 +883,13 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
     *plan = nullptr;
     if (ngroups < 0) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_gemm_plan_create: negative group count");
     auto *p = new tkmi_gemm_plan_s();
+    // When the plan launches the MFMA kernel anyway, skinny-shaped groups that fit in a
+    // few 64×64 tiles ride along in it instead of costing a launch of their own; the
+    // skinny kernel keeps the plans made of skinny groups only (the MPO steps).
+    auto skinny_shape = [](int64_t m, int64_t n, int64_t k) { return std::min(m, n) <= SKINNY && k <= SKINNY_K; };
+    bool any_mfma = false;
+    for (int64_t g = 0; g < ngroups; ++g)
+        if (m[g] > 0 && n[g] > 0 && k[g] > 0 && !skinny_shape(m[g], n[g], k[g])) any_mfma = true;
     for (int64_t g = 0; g < ngroups; ++g) {
         GGroup G;
         std::memset(&G, 0, sizeof(G));
@@ -901,7 +908,8 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
             free_gemm(p);
             TKMI_FAIL(TKMI_ERR_SPACE_MISMATCH, "tkmi_gemm_plan_create: inconsistent group dimensions");
         }
-        const bool to_skinny = std::min(G.m, G.n) <= SKINNY && G.k <= SKINNY_K;
+        const bool few_tiles = ((G.m + 63) / 64) * ((G.n + 63) / 64) <= 4;
+        const bool to_skinny = skinny_shape(G.m, G.n, G.k) && !(any_mfma && few_tiles);
         if (G.k > 0 && G.m > 0 && G.n > 0 && !to_skinny &&
             (G.lda * (G.ta ? 128 : 32) >= ((int64_t)1 << 31) || G.ldb * (G.tb ? 32 : 128) >= ((int64_t)1 << 31))) {
             free_gemm(p);
