@@ -99,6 +99,15 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
                 gemms.append({"flops": flops, "ms": ms, "kernel": name, "tiles": nt.value if part == 1 else None,
                               "grid_x": nt.value * 256 if part == 1 else None,
                               "alg_bytes": alg_bytes if part == 1 else None})
+        elif op[0] == "fused":
+            _, plan, _, (ta_, na), (tb_, nb), (tc_, nc), a, b = op
+            pa, pb, pc = ta_.data_ptr(), tb_.data_ptr(), tc_.data_ptr()
+            fn = lambda plan=plan, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_transform_exec_gather(
+                plan.handle, 0, pa, pc, pb, a, b, s)
+            ms = time_launches(fn, reps, torch)
+            # algorithmic bytes: A read once, C written once (and read with β≠0)
+            perms.append({"elements": nc, "bytes": 8 * (na + nc * (2 if b != 0 else 1)), "ms": ms,
+                          "entries": plan.nentries, "fused": True})
         else:
             _, plan, perm, (ts_, ns_), (td_, nd), a, b = op
             ps, pd = ts_.data_ptr(), td_.data_ptr()
@@ -131,6 +140,28 @@ def cpu_replay(trace, init, nthreads):
                 OK.abelian_permute(plan.table, perm, s, d, a, b, nthreads)
             else:
                 OK.generic_permute(plan.table, perm, s, d, a, b, nthreads)
+        elif op[0] == "fused":
+            # the reference's blas_contract!: both operand permutes, then one dgemm per sector
+            from tkmi import treetransformers as TT
+            _, plan, (pA, pB, pC, twA, twB, WA, WB, WC), ta, tb, tc, a, b = op
+            A, B, Cm = get(ta), get(tb), get(tc)
+            WA2, WB2 = WA.permute(*pA), WB.permute(*pB)
+            A2, B2 = np.zeros(WA2.dim()), np.zeros(WB2.dim())
+            OK.abelian_permute(TT.abelian_table(WA2, WA, pA[0], pA[1], None, twA), pA[0] + pA[1], A, A2, 1.0, 0.0,
+                               nthreads)
+            OK.abelian_permute(TT.abelian_table(WB2, WB, pB[0], pB[1], None, twB), pB[0] + pB[1], B, B2, 1.0, 0.0,
+                               nthreads)
+            LA = {c: (d, o) for c, d, o in WA2.degeneracystructure()[1]}
+            LB = {c: (d, o) for c, d, o in WB2.degeneracystructure()[1]}
+            WC2 = WA2.compose(WB2)
+            C2 = np.zeros(WC2.dim())
+            for c, (m, n), off in WC2.degeneracystructure()[1]:
+                if c in LA and c in LB:
+                    (_, k), oa = LA[c]
+                    (_, _), ob = LB[c]
+                    C2[off:off + m * n] = (A2[oa:oa + m * k].reshape(k, m).T @
+                                           B2[ob:ob + k * n].reshape(n, k).T).T.reshape(-1)
+            OK.abelian_permute(TT.abelian_table(WC, WC2, pC[0], pC[1]), pC[0] + pC[1], C2, Cm, a, b, nthreads)
         else:
             _, plan, prec, pa, pb, pc, a, b = op
             A, B, Cm = get(pa), get(pb), get(pc)
@@ -201,9 +232,12 @@ def main():
     y_eager = H.apply()
     trace = T.stop_trace()
     torch.cuda.synchronize()
-    flops = sum(op[1].flops for op in trace if op[0] == "gemm")
+    # Σ 2mnk of every per-sector GEMM of the chain, including those a fused
+    # small-operand contraction performs inside its transform
+    flops = sum(op[1].flops for op in trace if op[0] in ("gemm", "fused"))
     ngemm_groups = sum(op[1].ngroups for op in trace if op[0] == "gemm")
     log(f"chain: {sum(1 for o in trace if o[0]=='gemm')} grouped GEMM launches, {ngemm_groups} groups, "
+        f"{sum(1 for o in trace if o[0]=='fused')} fused small-operand contractions, "
         f"{flops/1e9:.3f} GFLOP, {sum(1 for o in trace if o[0]=='transform')} permutes")
 
     # tile-shape autotuning of every grouped GEMM (per precision form), then recompute

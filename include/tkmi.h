@@ -108,6 +108,16 @@ int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src
 int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const void *ddst,
                                 void *dsrc, double alpha, double beta, void *stream);
 
+/* Coefficients gathered from a small device tensor g at execution time:
+ * U[e] = coef[e]·g[idx[e]] (0 where idx[e] < 0), one entry per recoupling coefficient
+ * of a generic plan (n = Σ rows·cols).  tkmi_transform_exec_gather refreshes U from g
+ * and runs the transform: a contraction C = β·C + α·A·B with a small operand B,
+ * folded with the permutes of blas_contract! (src/tensors/tensoroperations.jl:392-462)
+ * into one pass over A and C.  U belongs to the plan, so calls on one plan must not
+ * overlap on different streams. */
+int tkmi_transform_plan_set_gather(tkmi_transform_plan plan, int64_t n, const double *coef, const int64_t *idx);
+int tkmi_transform_exec_gather(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
+                               const void *gsrc, double alpha, double beta, void *stream);
 int tkmi_transform_plan_info(tkmi_transform_plan plan, int64_t *ntiles, int64_t *nentries,
                              int64_t *elements);
 int tkmi_transform_plan_destroy(tkmi_transform_plan plan);

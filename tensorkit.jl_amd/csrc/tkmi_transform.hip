@@ -241,6 +241,18 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
         transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
 }
 
+// U[e] = coef[e]·g[idx[e]] (idx < 0: 0): the coefficients of a transform that depend on
+// a small device tensor (a contraction with a small operand folded into the transform)
+template <typename T>
+__global__ __launch_bounds__(NTHREADS) void gather_coeffs(const double *__restrict__ coef,
+                                                          const int64_t *__restrict__ idx, const T *__restrict__ g,
+                                                          int64_t n, double *__restrict__ U) {
+    for (int64_t e = (int64_t)blockIdx.x * NTHREADS + threadIdx.x; e < n; e += (int64_t)gridDim.x * NTHREADS) {
+        const int64_t i = idx[e];
+        U[e] = i < 0 ? 0.0 : coef[e] * (double)g[i];
+    }
+}
+
 // -------------------------------------------------------------------------------------
 // host side
 // -------------------------------------------------------------------------------------
@@ -272,6 +284,11 @@ struct PlanSide {
 
 struct tkmi_transform_plan_s {
     int32_t N = 0;
+    // optional gather of the forward coefficients: U[e] = gcoef[e]·g[gidx[e]] (0 for gidx < 0),
+    // from a device vector given at execution time
+    int64_t gather_len = 0;
+    double *d_gcoef = nullptr;
+    int64_t *d_gidx = nullptr;
     int64_t dst_len = 0, src_len = 0, elements = 0, nitems = 0;
     std::vector<TTree> trees;
     std::vector<double> U, Ut;
@@ -450,6 +467,8 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
 
 void free_plan(tkmi_transform_plan_s *p) {
     if (!p) return;
+    (void)hipFree(p->d_gcoef);
+    (void)hipFree(p->d_gidx);
     (void)hipFree(p->d_trees);
     (void)hipFree(p->d_U);
     (void)hipFree(p->d_Ut);
@@ -692,6 +711,38 @@ int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src
 int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const void *ddst, void *dsrc,
                                 double alpha, double beta, void *stream) {
     return exec_common(plan, dtype, ddst, dsrc, alpha, beta, stream, true);
+}
+
+int tkmi_transform_plan_set_gather(tkmi_transform_plan plan, int64_t n, const double *coef, const int64_t *idx) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
+    if (n != (int64_t)plan->U.size())
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_plan_set_gather: one entry per recoupling coefficient");
+    if (plan->d_gcoef) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_plan_set_gather: gather already set");
+    int rc;
+    if ((rc = tkmi::upload((void **)&plan->d_gcoef, coef, (size_t)n * sizeof(double)))) return rc;
+    if ((rc = tkmi::upload((void **)&plan->d_gidx, idx, (size_t)n * sizeof(int64_t)))) return rc;
+    plan->gather_len = n;
+    return TKMI_OK;
+}
+
+int tkmi_transform_exec_gather(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
+                               const void *gsrc, double alpha, double beta, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
+    if (!plan->d_gcoef) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_exec_gather: plan has no gather");
+    if (plan->gather_len > 0 && !gsrc) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_exec_gather: null gather source");
+    hipStream_t s = tkmi::as_stream(stream);
+    if (plan->gather_len > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>((plan->gather_len + NTHREADS - 1) / NTHREADS, 1024);
+        if (dtype == TKMI_F64)
+            hipLaunchKernelGGL(gather_coeffs<double>, dim3(grid), dim3(NTHREADS), 0, s, plan->d_gcoef, plan->d_gidx,
+                               (const double *)gsrc, plan->gather_len, plan->d_U);
+        else if (dtype == TKMI_F32)
+            hipLaunchKernelGGL(gather_coeffs<float>, dim3(grid), dim3(NTHREADS), 0, s, plan->d_gcoef, plan->d_gidx,
+                               (const float *)gsrc, plan->gather_len, plan->d_U);
+        else
+            TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
+    }
+    return exec_common(plan, dtype, src, dst, alpha, beta, stream, false);
 }
 
 int tkmi_transform_plan_info(tkmi_transform_plan plan, int64_t *ntiles, int64_t *nentries, int64_t *elements) {

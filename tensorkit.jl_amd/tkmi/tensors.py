@@ -23,6 +23,11 @@ from .spaces import HomSpace, SpaceMismatch, IndexError_
 from .sectors import BOSONIC, FERMIONIC
 from .spaces import ProductSpace, SectorMismatch
 from .treetransformers import treebraider, flat_plan, twist_factor, tracer, _make_trace_plan, _TRANSFORMER_CACHE
+from . import fusedcontract as _FC
+
+# contractions with a tiny operand and skinny per-sector GEMMs run as one fused
+# transform (see fusedcontract); False restores permute + mul! + permute
+FUSE_SMALL_OPERAND = True
 
 _DT = {torch.float64: _lib.TKMI_F64, torch.float32: _lib.TKMI_F32}
 
@@ -602,6 +607,24 @@ def blas_contract_(C, A, pA, B, pB, pAB, alpha, beta, precision=None):
         copyB |= twistB
     else:
         twistA = twistB = False
+
+    if FUSE_SMALL_OPERAND and _FC.eligible(C, A, pA, B, pB):
+        # the permutes of both operands, the skinny per-sector GEMMs and the result
+        # permute as one pass
+        n1A = len(pA[0])
+        nA2 = n1A + len(pA[1])
+        WA2, WB2 = A.space.permute(*pA), B.space.permute(*pB)
+        twA = tuple(i for i in range(n1A, nA2) if not WA2[i].isdual) if twistA else ()
+        twB = tuple(i for i in range(len(pB[0])) if WB2[i].isdual) if twistB else ()
+        pC = _canonicalize(pAB, C)
+        plan = _FC.fused_plan(C, A, pA, B, pB, pC, twA, twB)
+        if _TRACE is not None:
+            _TRACE.append(("fused", plan, (tuple(pA), tuple(pB), tuple(pC), twA, twB, A.space, B.space, C.space),
+                           _buf(A), _buf(B), _buf(C), float(alpha), float(beta)))
+        _lib.check(_lib.load().tkmi_transform_exec_gather(plan.handle, _DT[TC], A.data.data_ptr(), C.data.data_ptr(),
+                                                          B.data.data_ptr(), float(alpha), float(beta), _stream()),
+                   "contract (small operand)")
+        return C
 
     if copyA:
         Anew = TensorMap.empty(A.space.permute(*pA), TC)
