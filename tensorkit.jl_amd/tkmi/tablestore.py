@@ -81,11 +81,11 @@ def key_to_json(key):
         return {"kind": kind, "dst": homspace_to_json(Vd), "src": homspace_to_json(Vs), "p1": list(p1),
                 "p2": list(p2), "levels": None if levels is None else [list(levels[0]), list(levels[1])],
                 "adjoint_src": bool(adj), "dst_twist": list(twist)}
-    if kind == "trace":
+    if kind in ("trace", "trace_adj"):
         _, Vd, Vs, p1, p2, q1, q2 = key
         return {"kind": kind, "dst": homspace_to_json(Vd), "src": homspace_to_json(Vs), "p1": list(p1),
                 "p2": list(p2), "q1": list(q1), "q2": list(q2)}
-    if kind == "tr":
+    if kind in ("tr", "tr_adj"):
         return {"kind": kind, "space": homspace_to_json(key[1])}
     if kind == "flat":
         return {"kind": kind, "length": int(key[1])}
@@ -98,11 +98,11 @@ def key_from_json(d):
         lv = None if d["levels"] is None else (tuple(d["levels"][0]), tuple(d["levels"][1]))
         return ("braid", homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
                 tuple(d["p2"]), lv, bool(d["adjoint_src"]), tuple(d["dst_twist"]))
-    if kind == "trace":
-        return ("trace", homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
+    if kind in ("trace", "trace_adj"):
+        return (kind, homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
                 tuple(d["p2"]), tuple(d["q1"]), tuple(d["q2"]))
-    if kind == "tr":
-        return ("tr", homspace_from_json(d["space"]))
+    if kind in ("tr", "tr_adj"):
+        return (kind, homspace_from_json(d["space"]))
     if kind == "flat":
         return ("flat", int(d["length"]))
     raise ValueError(f"unknown cache key kind {kind!r}")

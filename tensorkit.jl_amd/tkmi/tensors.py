@@ -295,6 +295,11 @@ def transpose(t, p1=None, p2=None):
     if p1 is None:
         p1 = tuple(reversed(t.domainind()))
         p2 = tuple(reversed(t.codomainind()))
+    p1, p2 = _check_transpose(t, p1, p2)
+    return permute(t, p1, p2, copy=True)
+
+
+def _check_transpose(t, p1, p2):
     p1, p2 = tuple(p1), tuple(p2)
     n1, n2 = t.numout, t.numin
     lin = [x if x < n1 else n2 + 2 * n1 - 1 - x for x in p1] + \
@@ -303,7 +308,14 @@ def transpose(t, p1=None, p2=None):
         raise ValueError(f"{(p1, p2)} is not a cyclic permutation for {t.space}")
     if t.space.unit.braiding_style == FERMIONIC:
         raise NotImplementedError("planar transpose of fermionic tensors")
-    return permute(t, p1, p2, copy=True)
+    return p1, p2
+
+
+def transpose_(tdst, tsrc, p1, p2, alpha=1.0, beta=0.0):
+    """`transpose!(tdst, tsrc, (p₁, p₂), α, β)` (`indexmanipulations.jl:380-392`):
+    the in-place planar transpose (bosonic sectors: a cyclic `permute!`)."""
+    p1, p2 = _check_transpose(tsrc, p1, p2)
+    return permute_(tdst, tsrc, p1, p2, alpha, beta)
 
 
 def planarcontract_(C, A, pA, B, pB, pAB, alpha=1.0, beta=0.0, precision=None):

@@ -290,6 +290,35 @@ def trace_table(Vdst, Vsrc, p1, p2, q1, q2):
     destination order."""
     fts_src = Vsrc.subblockstructure()
     fts_dst = Vdst.subblockstructure()
+    contrib = _trace_contrib(Vsrc, p1, p2, q1, q2)
+    keep = tuple(p1) + tuple(p2)
+    rows = []
+    for dpair in Vdst.fusiontrees():
+        szd, std, offd = fts_dst[dpair]
+        coeffs, slices = [], []
+        for c, spair in contrib.get(dpair, []):
+            szs, sts, offs = fts_src[spair]
+            if tuple(szs[x] for x in keep) != tuple(szd) or any(szs[a] != szs[b] for a, b in zip(q1, q2)):
+                raise AssertionError("trace: sub-block sizes do not match")
+            for sl in _diag_slices(szs, sts, offs, keep, q1, q2):
+                slices.append(sl)
+                coeffs.append(c)
+        rows.append((np.array(coeffs, dtype=np.float64).reshape(1, len(coeffs)), tuple(szd),
+                     [list(std) + [offd]], slices))
+    return rows
+
+
+def _diag_slices(szs, sts, offs, keep, q1, q2):
+    """The traced-diagonal slices of one source sub-block: one strided view of the kept
+    legs (in destination order) per tuple of traced degeneracy indices."""
+    base = [sts[x] for x in keep]
+    diag = [sts[a] + sts[b] for a, b in zip(q1, q2)]
+    return [base + [offs + sum(tk * dk for tk, dk in zip(t, diag))]
+            for t in itertools.product(*[range(szs[a]) for a in q1])]
+
+
+def _trace_contrib(Vsrc, p1, p2, q1, q2):
+    """Destination tree pair → [(coefficient, source tree pair)] of a trace."""
     n1, n2 = len(p1), len(p2)
     r1, r2 = tuple(p1) + tuple(q1), tuple(p2) + tuple(q2)
     contrib = {}
@@ -317,23 +346,72 @@ def trace_table(Vdst, Vsrc, p1, p2, q1, q2):
                 for j, (g1, g2) in enumerate(dst.trees):
                     if U[j, i] != 0:
                         add(U[j, i], g1, g2, spair)
+    return contrib
+
+
+def trace_adjoint_table(Vdst, Vsrc, p1, p2, q1, q2):
+    """Rows of the pullback of a trace (ΔA += α·T‡ΔC), one per source tree pair of `Vsrc`
+    that feeds the trace: destinations are its traced-diagonal slices, sources are the
+    `Vdst` sub-blocks it contributes to, and U[r, k] = coeff_k·dim(c_dst)/dim(c_src) for
+    every r — the adjoint of the trace for the inner product Σ_c dim(c)·⟨x_c, y_c⟩ that
+    TensorKit's AD rules use (`trace_permute_pullback_ΔA!`,
+    src/pullbacks/tensoroperations.jl:50-64, computes the same map as a product with
+    an identity).  Each destination slice lies in exactly one row, so a β = 1 launch
+    accumulates without write conflicts."""
+    fts_src = Vsrc.subblockstructure()
+    fts_dst = Vdst.subblockstructure()
     keep = tuple(p1) + tuple(p2)
+    by_src = {}
+    for dpair, lst in _trace_contrib(Vsrc, p1, p2, q1, q2).items():
+        for c, spair in lst:
+            by_src.setdefault(spair, []).append((c, dpair))
     rows = []
-    for dpair in Vdst.fusiontrees():
-        szd, std, offd = fts_dst[dpair]
-        coeffs, slices = [], []
-        for c, spair in contrib.get(dpair, []):
-            szs, sts, offs = fts_src[spair]
-            if tuple(szs[x] for x in keep) != tuple(szd) or any(szs[a] != szs[b] for a, b in zip(q1, q2)):
+    for spair in Vsrc.fusiontrees():
+        lst = by_src.get(spair)
+        if not lst:
+            continue
+        szs, sts, offs = fts_src[spair]
+        szk = tuple(szs[x] for x in keep)
+        dsts = _diag_slices(szs, sts, offs, keep, q1, q2)
+        w_src = spair[0].coupled.dim()
+        srcs, ucol = [], []
+        for c, dpair in lst:
+            szd, std, offd = fts_dst[dpair]
+            if tuple(szd) != szk:
                 raise AssertionError("trace: sub-block sizes do not match")
-            base = [sts[x] for x in keep]
-            diag = [sts[a] + sts[b] for a, b in zip(q1, q2)]
-            for t in itertools.product(*[range(szs[a]) for a in q1]):
-                slices.append(base + [offs + sum(tk * dk for tk, dk in zip(t, diag))])
-                coeffs.append(c)
-        rows.append((np.array(coeffs, dtype=np.float64).reshape(1, len(coeffs)), tuple(szd),
-                     [list(std) + [offd]], slices))
+            srcs.append(list(std) + [offd])
+            ucol.append(c * dpair[0].coupled.dim() / w_src)
+        U = np.tile(np.array(ucol, dtype=np.float64), (len(dsts), 1))
+        rows.append((U, szk, dsts, srcs))
     return rows
+
+
+def pack_rows(rows):
+    """Generic-plan arrays of rows (U as an R×C matrix, slice sizes, R destination slices,
+    C source slices), identity permutation."""
+    return {
+        "rows": np.array([r[0].shape[0] for r in rows], dtype=np.int64),
+        "cols": np.array([r[0].shape[1] for r in rows], dtype=np.int64),
+        "U": np.concatenate([np.asarray(r[0], dtype=np.float64).reshape(-1, order="F") for r in rows]
+                            + [np.zeros(0)]),
+        "sz_dst": np.array([r[1] for r in rows], dtype=np.int64).reshape(-1),
+        "sz_src": np.array([r[1] for r in rows], dtype=np.int64).reshape(-1),
+        "so_dst": np.array([x for r in rows for x in r[2]], dtype=np.int64).reshape(-1),
+        "so_src": np.array([x for r in rows for x in r[3]], dtype=np.int64).reshape(-1),
+    }
+
+
+def trace_adjointer(Vdst, Vsrc, p1, p2, q1, q2):
+    """Cached device plan of `trace_adjoint_table` (source: a tensor in `Vdst`,
+    destination: a tensor in `Vsrc`)."""
+    key = ("trace_adj", Vdst, Vsrc, tuple(p1), tuple(p2), tuple(q1), tuple(q2))
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is None:
+        rows = trace_adjoint_table(Vdst, Vsrc, p1, p2, q1, q2)
+        plan = plan_from_packed("trace_adj", tuple(range(len(p1) + len(p2))), pack_rows(rows), Vsrc.dim(),
+                                Vdst.dim(), table=rows)
+        _TRANSFORMER_CACHE.put(key, plan)
+    return plan
 
 
 def pack_trace(rows):
