@@ -82,6 +82,8 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
             L.tkmi_gemm_plan_query(plan.handle, prec, C.byref(tile), C.byref(nt))
             fm, fs, ns = C.c_double(), C.c_double(), C.c_int64()
             L.tkmi_gemm_plan_split(plan.handle, C.byref(fm), C.byref(fs), C.byref(ns))
+            sk, slots = C.c_int64(1), C.c_int64(0)
+            L.tkmi_gemm_plan_splitk(plan.handle, tile.value, C.byref(sk), C.byref(slots))
             # algorithmic HBM bytes of the MFMA launch: A, B read once, C written once (and
             # read once with β≠0), over the groups the plan gives to the MFMA kernel
             # (smaller output side > 32 or K > 64)
@@ -97,6 +99,7 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
                     plan.handle, prec, part, a, b, pa, pb, pc, s)
                 ms = time_launches(fn, reps, torch)
                 gemms.append({"flops": flops, "ms": ms, "kernel": name, "tiles": nt.value if part == 1 else None,
+                              "splitk": sk.value if part == 1 else None, "slots": slots.value if part == 1 else None,
                               "grid_x": nt.value * 256 if part == 1 else None,
                               "alg_bytes": alg_bytes if part == 1 else None})
         elif op[0] == "fused":
@@ -381,7 +384,9 @@ def main():
     perm_ms = sum(p["ms"] for p in perms)
     big = max(perms, key=lambda p: p["bytes"]) if perms else None
     log("gemm launches: " + "; ".join(f"{g['kernel']} {g['flops']/1e9:.3f} GF {g['ms']*1e3:.1f} us "
-                                       f"{g['flops']/g['ms']/1e9:.2f} TF/s" for g in gemms))
+                                       f"{g['flops']/g['ms']/1e9:.2f} TF/s"
+                                       + (f" ({g['tiles']} tiles, split-K {g['splitk']}, {g['slots']} slots)"
+                                          if g.get('tiles') else "") for g in gemms))
     log("permutes: " + "; ".join(f"{p['elements']} el {p['ms']*1e3:.1f} us {p['bytes']/p['ms']/1e6:.0f} GB/s"
                                  for p in perms))
 

@@ -154,6 +154,11 @@ int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, 
 int tkmi_gemm_plan_split(tkmi_gemm_plan plan, double *flops_mfma, double *flops_skinny,
                          int64_t *skinny_groups);
 int tkmi_gemm_plan_info(tkmi_gemm_plan plan, int64_t *ntiles, double *flops);
+/* Plan-wide split-K factor chosen for tile configuration `config` (1 = no split) and
+ * the workgroup slots (occupancy × compute units) of its FP64 kernel that the choice
+ * assumed.  The factor minimises a model of the launch: the heaviest-first schedule of
+ * the tiles over the slots plus the partial products' extra HBM traffic.          */
+int tkmi_gemm_plan_splitk(tkmi_gemm_plan plan, int32_t config, int64_t *splitk, int64_t *slots);
 /* Autotuner: times every tile shape of the plan on the given operands (α=1,
  * β=0: C is overwritten) and keeps the fastest for `precision`.  Synchronises
  * `stream`; meant for warm-up, not for capture.                               */

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 namespace {
@@ -45,8 +46,8 @@ struct GGroup {
     int64_t pa, pb, kp, sa, sb;
 };
 
-// Work of the split/pack pre-pass for one MFMA group: nA = mp·kp/8 octets of A rows,
-// then np·kp/8 octets of B columns, numbered from start.
+// Work of the split/pack pre-pass for one MFMA group: nA tiles of A rows, then the
+// tiles of B columns (each PACK_T rows × PACK_T of K, K fastest), numbered from start.
 struct PackGroup {
     int64_t m, n, k, a_off, lda, b_off, ldb;
     int32_t ta, tb;
@@ -384,52 +385,80 @@ __global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ r
 // split_pack_kernel splits every operand element once per call into the three bf16
 // planes and packs each group's A rows and B columns K-contiguous, zero padded to
 // whole tiles and slabs; the GEMM then stages 16-byte vectors straight into LDS with
-// no conversion and no bounds test in its K loop.
+// no conversion and no bounds test in its K loop.  One workgroup converts one
+// PACK_T×PACK_T (rows × K) tile: the doubles are staged through LDS, read with lanes
+// along whichever side is contiguous in memory and written as whole 8-element octets,
+// PACK_T/8 lanes per row (128 contiguous bytes per plane), so both sides coalesce.
 // --------------------------------------------------------------------------------
 __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
 
 typedef uint32_t u4_t __attribute__((ext_vector_type(4)));  // 16 bytes = 8 bf16
 
-__global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restrict__ pg, int64_t ng, int64_t total,
+constexpr int PACK_T = 64;
+
+__global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restrict__ pg, int64_t ng,
                                                         const double *__restrict__ A, const double *__restrict__ Bm,
                                                         __bf16 *__restrict__ P) {
-    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < total; q += (int64_t)gridDim.x * NT) {
-        int64_t lo = 0, hi = ng - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (pg[mid].start <= q) lo = mid; else hi = mid - 1;
+    __shared__ double tile[PACK_T][PACK_T + 1];
+    const int64_t q = blockIdx.x;
+    int64_t lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (pg[mid].start <= q) lo = mid; else hi = mid - 1;
+    }
+    const PackGroup &G = pg[lo];
+    int64_t r = q - G.start;
+    const bool isA = r < G.nA;
+    if (!isA) r -= G.nA;
+    const int64_t tk = (G.kp + PACK_T - 1) / PACK_T;
+    const int64_t i0 = (r / tk) * PACK_T, k0 = (r % tk) * PACK_T;
+    const int64_t lim = isA ? G.m : G.n;
+    const double *src = isA ? A + G.a_off : Bm + G.b_off;
+    const int64_t ld = isA ? G.lda : G.ldb;
+    // element (i, k) of op(A) / of op(B)ᵀ lies at i + k·ld (rows contiguous) or k + i·ld
+    const bool rows_contiguous = isA ? (G.ta == 0) : (G.tb != 0);
+    const int t = threadIdx.x;
+    if (rows_contiguous) {
+        const int ii = t % PACK_T;
+        const int64_t gi = i0 + ii;
+        for (int kk = t / PACK_T; kk < PACK_T; kk += NT / PACK_T) {
+            const int64_t gk = k0 + kk;
+            tile[ii][kk] = (gi < lim && gk < G.k) ? src[gi + gk * ld] : 0.0;
         }
-        const PackGroup &G = pg[lo];
-        int64_t r = q - G.start;
-        const bool isA = r < G.nA;
-        if (!isA) r -= G.nA;
-        const int64_t rows = isA ? G.mp : G.np;
-        const int64_t i = r % rows, k8 = r / rows;  // lanes run along rows: coalesced when that side is contiguous
-        const int64_t lim = isA ? G.m : G.n;
+    } else {
+        const int kk = t % PACK_T;
+        const int64_t gk = k0 + kk;
+        for (int ii = t / PACK_T; ii < PACK_T; ii += NT / PACK_T) {
+            const int64_t gi = i0 + ii;
+            tile[ii][kk] = (gi < lim && gk < G.k) ? src[gk + gi * ld] : 0.0;
+        }
+    }
+    __syncthreads();
+    __bf16 *base = P + (isA ? G.pa : G.pb);
+    const int64_t plane = isA ? G.sa : G.sb;
+    // rows: i0 + ii < mp (np), a multiple of 128; octets: k0 + 8·o < kp, a multiple of 32
+    for (int w = t; w < PACK_T * (PACK_T / 8); w += NT) {
+        const int ii = w / (PACK_T / 8), o = w % (PACK_T / 8);
+        const int64_t gk = k0 + 8 * o;
+        if (gk >= G.kp) continue;
         __bf16 x[3][8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int64_t kk = 8 * k8 + t;
-            double v = 0.0;
-            if (i < lim && kk < G.k) {
-                if (isA) v = G.ta ? A[G.a_off + kk + i * G.lda] : A[G.a_off + i + kk * G.lda];
-                else     v = G.tb ? Bm[G.b_off + i + kk * G.ldb] : Bm[G.b_off + kk + i * G.ldb];
-            }
+        for (int e = 0; e < 8; ++e) {
+            const double v = tile[ii][8 * o + e];
             const __bf16 x0 = to_bf16((float)v);
             const double r1 = v - (double)(float)x0;
             const __bf16 x1 = to_bf16((float)r1);
             const double r2 = r1 - (double)(float)x1;
-            x[0][t] = x0;
-            x[1][t] = x1;
-            x[2][t] = to_bf16((float)r2);
+            x[0][e] = x0;
+            x[1][e] = x1;
+            x[2][e] = to_bf16((float)r2);
         }
-        __bf16 *dst = P + (isA ? G.pa : G.pb) + i * G.kp + 8 * k8;
-        const int64_t plane = isA ? G.sa : G.sb;
+        __bf16 *dst = base + (i0 + ii) * G.kp + gk;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) {
+        for (int pl = 0; pl < 3; ++pl) {
             u4_t v;
-            __builtin_memcpy(&v, x[p], sizeof(v));
-            *reinterpret_cast<u4_t *>(dst + p * plane) = v;
+            __builtin_memcpy(&v, x[pl], sizeof(v));
+            *reinterpret_cast<u4_t *>(dst + pl * plane) = v;
         }
     }
 }
@@ -655,6 +684,7 @@ struct TileSet {
     int64_t relems = 0;
     RGroup *d_rgroups = nullptr;
     double *ws = nullptr;  // split-K partial products (plan-owned)
+    int64_t split = 1;     // plan-wide split-K factor the schedule model chose
 };
 
 struct SkinnySet {
@@ -695,20 +725,114 @@ struct tkmi_gemm_plan_s {
 
 namespace {
 
+// Workgroups of configuration c's FP64 kernel that run at once on the current device
+// (occupancy per CU × CUs): the slots the split-K schedule model deals tiles to.
+template <int BM, int BN, int BK, int NBUF>
+int kernel_occupancy() {
+    int nb = 0;
+    hipError_t e;
+    if constexpr (NBUF == 3)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&gemm_k4p_kernel<double, BM, BN, BK>), NT, 0);
+    else if constexpr (NBUF == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&gemm_k4_kernel<double, BM, BN, BK>), NT, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&gemm_k4s_kernel<double, BM, BN, BK>), NT, 0);
+    return (e == hipSuccess && nb > 0) ? nb : 1;
+}
+
+int64_t cfg_slots(int c) {
+    constexpr int MAXDEV = 64;
+    static int cache[MAXDEV][NCFG] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAXDEV) dev = 0;
+    int &v = cache[dev][c];
+    if (v == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        int occ;
+        switch (c) {
+            case 0: occ = kernel_occupancy<64, 64, 16, 2>(); break;
+            case 1: occ = kernel_occupancy<128, 128, 16, 2>(); break;
+            case 2: occ = kernel_occupancy<128, 64, 16, 2>(); break;
+            case 3: occ = kernel_occupancy<64, 128, 16, 2>(); break;
+            case 4: occ = kernel_occupancy<64, 64, 32, 2>(); break;
+            case 5: occ = kernel_occupancy<64, 64, 16, 1>(); break;
+            case 6: occ = kernel_occupancy<64, 64, 32, 1>(); break;
+            case 7: occ = kernel_occupancy<128, 128, 16, 1>(); break;
+            default: occ = kernel_occupancy<64, 64, 16, 3>(); break;
+        }
+        v = occ * cus;
+    }
+    return v;
+}
+
+// Split-K factor of a group for a plan-wide factor S: long-K groups (≥ LONG_K slabs) split
+// into at most S chunks of at least MIN_SLABS slabs; a chunk is whole slabs of this
+// configuration and a multiple of 32 (the 3×BF16 slab).
+constexpr int64_t MAX_SPLIT = 8, MIN_SLABS = 16, LONG_K = 32;
+
+void split_group(GGroup &g, int64_t S, int bk) {
+    const int64_t slabs = (g.k + bk - 1) / bk;
+    int64_t sg = slabs >= LONG_K ? std::max<int64_t>(1, std::min<int64_t>(S, slabs / MIN_SLABS)) : 1;
+    g.kchunk = std::max<int64_t>(32, (((slabs + sg - 1) / sg) * bk + 31) / 32 * 32);
+    g.splits = g.k > 0 ? (g.k + g.kchunk - 1) / g.kchunk : 1;
+}
+
+// Modelled launch time of the plan's tiles for a plan-wide split factor S, in units of
+// one BK slab on one workgroup slot: the greedy schedule (heaviest tile first, next tile
+// to the first free slot — the order the tiles are numbered in) of every tile's slabs
+// plus a fixed per-tile cost, plus the split groups' extra HBM traffic (each partial
+// product written and read back once) at an effective FP64 MFMA : HBM ratio.
+double split_model(const std::vector<GGroup> &base, int64_t S, int bm, int bn, int bk, int64_t slots) {
+    constexpr double TILE_COST = 3.0;      // prologue + epilogue, in slabs (measured fit)
+    constexpr double FLOP_PER_BYTE = 8.0;  // ≈ 35 TFLOP/s effective FP64 MFMA : 4.4 TB/s
+    std::vector<double> cost;
+    double extra = 0;
+    for (GGroup g : base) {
+        split_group(g, S, bk);
+        const int64_t tiles = ((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn);
+        for (int64_t z = 0; z < g.splits; ++z) {
+            const int64_t kz = std::min(g.k, (z + 1) * g.kchunk) - z * g.kchunk;
+            const double c = (double)((kz + bk - 1) / bk) + TILE_COST;
+            for (int64_t t = 0; t < tiles; ++t) cost.push_back(c);
+        }
+        if (g.splits > 1) extra += 2.0 * (double)g.splits * (double)g.m * (double)g.n * sizeof(double);
+    }
+    if (cost.empty()) return 0;
+    std::sort(cost.begin(), cost.end(), std::greater<double>());
+    const size_t w = (size_t)std::max<int64_t>(1, std::min<int64_t>(slots, (int64_t)cost.size()));
+    std::vector<double> load(w, 0.0);  // min-heap of slot finish times
+    for (double c : cost) {
+        std::pop_heap(load.begin(), load.end(), std::greater<double>());
+        load.back() += c;
+        std::push_heap(load.begin(), load.end(), std::greater<double>());
+    }
+    const double makespan = *std::max_element(load.begin(), load.end());
+    return makespan + extra * FLOP_PER_BYTE / (2.0 * bm * bn * bk * (double)slots);
+}
+
 int build_tiles(tkmi_gemm_plan_s *p, int c) {
     TileSet &ts = p->ts[c];
     const int bm = CFGS[c].bm, bn = CFGS[c].bn;
     const int bk = CFGS[c].bk;
     ts.groups.clear();
-    // split-K for long-K groups (≥ LONG_K slabs) when the output tiles alone cannot
-    // fill the chip: aim for ≥ SPLIT_TARGET workgroups, at most MAX_SPLIT splits, at
-    // least MIN_SLABS K slabs per split
-    constexpr int64_t SPLIT_TARGET = 1024, MAX_SPLIT = 8, MIN_SLABS = 16, LONG_K = 32;
-    int64_t tiles_mn_total = 0;
-    for (auto &g : p->base) tiles_mn_total += ((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn);
-    const int64_t S = tiles_mn_total >= SPLIT_TARGET
-                          ? 1
-                          : std::min<int64_t>(MAX_SPLIT, (SPLIT_TARGET + tiles_mn_total - 1) / std::max<int64_t>(1, tiles_mn_total));
+    // plan-wide split-K factor: the one with the shortest modelled launch time (ties:
+    // the smaller factor)
+    const int64_t slots = cfg_slots(c);
+    int64_t S = 1;
+    double best = split_model(p->base, 1, bm, bn, bk, slots);
+    for (int64_t cand = 2; cand <= MAX_SPLIT; ++cand) {
+        const double t = split_model(p->base, cand, bm, bn, bk, slots);
+        if (t < best * 0.98) {
+            best = t;
+            S = cand;
+        }
+    }
+    ts.split = S;
     int64_t start = 0, ws_len = 0;
     ts.rgroups.clear();
     ts.relems = 0;
@@ -716,12 +840,8 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     for (auto g : p->base) {
         g.tiles_m = (g.m + bm - 1) / bm;
         g.tiles_n = (g.n + bn - 1) / bn;
-        const int64_t slabs = (g.k + bk - 1) / bk;
-        int64_t sg = slabs >= LONG_K ? std::max<int64_t>(1, std::min<int64_t>(S, slabs / MIN_SLABS)) : 1;
-        // chunk: whole slabs of this config and a multiple of 32 (the 3×BF16 slab)
-        g.kchunk = std::max<int64_t>(32, (((slabs + sg - 1) / sg) * bk + 31) / 32 * 32);
-        sg = g.k > 0 ? (g.k + g.kchunk - 1) / g.kchunk : 1;
-        g.splits = sg;
+        split_group(g, S, bk);
+        const int64_t sg = g.splits;
         g.ws_off = 0;
         if (sg > 1) {
             g.ws_off = ws_len;
@@ -788,9 +908,8 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
         }
     }
     else {
-        const int64_t pblocks = std::min<int64_t>((p->pack_total + NT - 1) / NT, 16384);
-        hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)pblocks), block, 0, s, p->d_pack,
-                           (int64_t)p->pack.size(), p->pack_total, A, B, p->panels);
+        hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)p->pack_total), block, 0, s, p->d_pack,
+                           (int64_t)p->pack.size(), A, B, p->panels);
         if constexpr (NBUF == 3)
             hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 2>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
                                (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
@@ -965,9 +1084,10 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         P.b_off = G.b_off; P.ldb = G.ldb; P.tb = G.tb;
         P.mp = mp; P.np = np; P.kp = G.kp;
         P.pa = G.pa; P.pb = G.pb; P.sa = G.sa; P.sb = G.sb;
+        const int64_t tk = (G.kp + PACK_T - 1) / PACK_T;  // K tiles of the pack pass
         P.start = p->pack_total;
-        P.nA = mp * (G.kp / 8);
-        p->pack_total += (mp + np) * (G.kp / 8);
+        P.nA = (mp / PACK_T) * tk;
+        p->pack_total += ((mp + np) / PACK_T) * tk;
         p->pack.push_back(P);
     }
     for (int c = 0; c < NCFG && !rc; ++c) rc = build_tiles(p, c);
@@ -1074,6 +1194,14 @@ int tkmi_gemm_plan_split(tkmi_gemm_plan plan, double *flops_mfma, double *flops_
     if (flops_mfma) *flops_mfma = plan->flops_mfma;
     if (flops_skinny) *flops_skinny = plan->flops_skinny;
     if (skinny_groups) *skinny_groups = (int64_t)plan->skinny.groups.size();
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_splitk(tkmi_gemm_plan plan, int32_t config, int64_t *splitk, int64_t *slots) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (config < 0 || config >= NCFG) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: no such tile configuration");
+    if (splitk) *splitk = plan->ts[config].split;
+    if (slots) *slots = cfg_slots(config);
     return TKMI_OK;
 }
 

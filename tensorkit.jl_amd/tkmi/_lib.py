@@ -52,6 +52,7 @@ SIGNATURES = [
                                        c_vp]),
     ("tkmi_gemm_plan_split", C.c_int, [c_vp, c_f64p, c_f64p, c_i64p]),
     ("tkmi_gemm_plan_info", C.c_int, [c_vp, c_i64p, c_f64p]),
+    ("tkmi_gemm_plan_splitk", C.c_int, [c_vp, C.c_int32, c_i64p, c_i64p]),
     ("tkmi_gemm_plan_query", C.c_int, [c_vp, C.c_int32, c_i32p, c_i64p]),
     ("tkmi_gemm_plan_autotune", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, c_vp, C.c_int32, c_vp]),
     ("tkmi_gemm_plan_destroy", C.c_int, [c_vp]),
