@@ -37,6 +37,7 @@ constexpr int MAXD = 12;
 constexpr int NTHREADS = 256;
 constexpr int NCLASS = 4;  // L = 1, 2, 4, 8
 constexpr int MAXW = 8;    // destination trees per work item
+constexpr int64_t SMALL_PLAN = (int64_t)1 << 25;  // written elements below which classes merge
 
 struct TBlock {
     int32_t ndim;
@@ -242,7 +243,9 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
 }
 
 // U[e] = coef[e]·g[idx[e]] (idx < 0: 0): the coefficients of a transform that depend on
-// a small device tensor (a contraction with a small operand folded into the transform)
+// a small device tensor (a contraction with a small operand folded into the transform).
+// A launch of its own ahead of the transform: forming them inside every tile puts two
+// dependent loads in front of each tile's arithmetic, which measured slower.
 template <typename T>
 __global__ __launch_bounds__(NTHREADS) void gather_coeffs(const double *__restrict__ coef,
                                                           const int64_t *__restrict__ idx, const T *__restrict__ g,
@@ -422,6 +425,20 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             sides[c].blocks.clear();
             sides[c].ntiles = 0;
         }
+        // A small plan runs as one launch of its largest class when that class holds at
+        // most 4 trees: the ramp-up and tail of a second launch cost more than the
+        // smaller per-thread work of one-tree items in the bigger class.
+        int cmax = 0;
+        int64_t moved = 0;
+        for (auto &it : items) {
+            const int R = adjoint ? it.nread : it.nwrite;
+            const int Cn = adjoint ? it.nwrite : it.nread;
+            int64_t vol = 1;
+            for (int64_t d : it.size) vol *= d;
+            moved += vol * R;
+            for (int jc = 0; jc < R; jc += MAXW) cmax = std::max(cmax, class_of(Cn, std::min(MAXW, R - jc)));
+        }
+        const bool merge = moved <= SMALL_PLAN && class_L(cmax) <= 4;
         for (auto &it : items) {
             const int R = adjoint ? it.nread : it.nwrite;   // written trees on this side
             const int Cn = adjoint ? it.nwrite : it.nread;  // read trees on this side
@@ -438,7 +455,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
                 b.write_tree = wt + jc;
                 b.u_start = it.u_start + jc;
                 b.u_ld = R;
-                const int c = class_of(b.nread, b.nwrite);
+                const int c = merge ? cmax : class_of(b.nread, b.nwrite);
                 tile_block(b, trees, class_L(c));
                 b.tile_start = sides[c].ntiles;
                 sides[c].ntiles += b.ntW * b.ntR * b.nbatch;
@@ -551,17 +568,30 @@ int64_t total_tiles(const PlanSide *sides) {
 }
 
 int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst, double alpha, double beta,
-                void *stream, bool adjoint) {
+                void *stream, bool adjoint, const void *gsrc = nullptr) {
     if (!p) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
     const PlanSide *sides = adjoint ? p->adj : p->fwd;
     const double *dU = adjoint ? p->d_Ut : p->d_U;
     if (total_tiles(sides) > 0 && (!src || !dst)) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
+    if (dtype != TKMI_F64 && dtype != TKMI_F32)
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
+    // a gather plan's coefficients are formed from gsrc at every call
+    if (p->gather_len > 0 && (adjoint || !gsrc))
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: the plan's coefficients come from a gather source: "
+                                     "use tkmi_transform_exec_gather (no adjoint)");
     hipStream_t s = tkmi::as_stream(stream);
+    if (p->gather_len > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>((p->gather_len + NTHREADS - 1) / NTHREADS, 1024);
+        if (dtype == TKMI_F64)
+            hipLaunchKernelGGL(gather_coeffs<double>, dim3(grid), dim3(NTHREADS), 0, s, p->d_gcoef, p->d_gidx,
+                               (const double *)gsrc, p->gather_len, p->d_U);
+        else
+            hipLaunchKernelGGL(gather_coeffs<float>, dim3(grid), dim3(NTHREADS), 0, s, p->d_gcoef, p->d_gidx,
+                               (const float *)gsrc, p->gather_len, p->d_U);
+    }
     if (dtype == TKMI_F64)
         return launch_side<double>(p, sides, dU, (const double *)src, (double *)dst, alpha, beta, s);
-    if (dtype == TKMI_F32)
-        return launch_side<float>(p, sides, dU, (const float *)src, (float *)dst, alpha, beta, s);
-    TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
+    return launch_side<float>(p, sides, dU, (const float *)src, (float *)dst, alpha, beta, s);
 }
 
 bool in_range(const int64_t *sz, const int64_t *st, int64_t off, int N, int64_t len) {
@@ -730,19 +760,7 @@ int tkmi_transform_exec_gather(tkmi_transform_plan plan, int32_t dtype, const vo
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
     if (!plan->d_gcoef) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_exec_gather: plan has no gather");
     if (plan->gather_len > 0 && !gsrc) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_exec_gather: null gather source");
-    hipStream_t s = tkmi::as_stream(stream);
-    if (plan->gather_len > 0) {
-        const unsigned grid = (unsigned)std::min<int64_t>((plan->gather_len + NTHREADS - 1) / NTHREADS, 1024);
-        if (dtype == TKMI_F64)
-            hipLaunchKernelGGL(gather_coeffs<double>, dim3(grid), dim3(NTHREADS), 0, s, plan->d_gcoef, plan->d_gidx,
-                               (const double *)gsrc, plan->gather_len, plan->d_U);
-        else if (dtype == TKMI_F32)
-            hipLaunchKernelGGL(gather_coeffs<float>, dim3(grid), dim3(NTHREADS), 0, s, plan->d_gcoef, plan->d_gidx,
-                               (const float *)gsrc, plan->gather_len, plan->d_U);
-        else
-            TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
-    }
-    return exec_common(plan, dtype, src, dst, alpha, beta, stream, false);
+    return exec_common(plan, dtype, src, dst, alpha, beta, stream, false, gsrc);
 }
 
 int tkmi_transform_plan_info(tkmi_transform_plan plan, int64_t *ntiles, int64_t *nentries, int64_t *elements) {
