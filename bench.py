@@ -34,8 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
 # Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
 TRAFFIC_PMC = {
-    ("gemm_k4_kernel<double, 64, 64, 16>", 544256): (34909 * 2 + 66218) * 1024,  # L·x
-    ("gemm_k4_kernel<double, 64, 64, 32>", 219136): (53315 * 2 + 23434) * 1024,  # ·R, split-K partials
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): (86757 * 2 + 68107) * 1024,  # L·x
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): (106237 * 2 + 57125) * 1024,  # ·R, split-K 5 partials
 }
 
 

@@ -129,7 +129,10 @@ int tkmi_transform_plan_destroy(tkmi_transform_plan plan);
  * (reshape(view(data, r), d₁, d₂), src/tensors/tensor.jl:474-479).
  * op(A_g) is m×k: transa=0 → A[i + l·lda]; transa=1 → A[l + i·lda]
  * (the adjoint block of an AdjointTensorMap, src/tensors/adjoint.jl:30).
- * op(B_g) is k×n likewise.  k = 0 gives C_g = β·C_g (the rmul! branch).     */
+ * op(B_g) is k×n likewise.  k = 0 gives C_g = β·C_g (the rmul! branch).
+ * Groups on the matrix cores need lda·(transa ? 128 : 32) and
+ * ldb·(transb ? 32 : 128) below 2^29 (32-bit byte offsets inside a K slab);
+ * larger leading dimensions return TKMI_ERR_UNSUPPORTED.                      */
 int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *m,
                           const int64_t *n, const int64_t *k, const int64_t *a_off,
                           const int64_t *lda, const int32_t *transa, const int64_t *b_off,

@@ -163,8 +163,9 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
 
     // Per-thread element coordinates of the A and B slabs (fixed over K): global order is
     // coalesced (contiguous index fastest), the LDS position is [row][k].  Addresses are a
-    // wave-uniform slab base pointer (advanced on the scalar side) plus a 32-bit offset per
-    // element; interior tiles and slabs skip every bounds test.
+    // wave-uniform slab base pointer (advanced on the scalar side) plus a 32-bit byte
+    // offset per element, so the loads take the scalar-base form; interior tiles and
+    // slabs skip every bounds test.
     // element e of this thread: (row i, k) of the A slab, (k, column j) of the B slab;
     // cheap shifts of tid, recomputed where needed rather than held in registers
     auto a_el = [&](int e, int &i, int &k) {
@@ -177,19 +178,22 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
         if (!tb) { k = idx % BK; j = idx / BK; }
         else     { j = idx % BN; k = idx / BN; }
     };
-    uint32_t aoff[AE], boff[BE];
+    uint32_t aoff[AE], boff[BE];  // byte offsets from the slab base
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
         int i, k;
         a_el(e, i, k);
-        aoff[e] = ta ? (uint32_t)(k + i * lda) : (uint32_t)(i + k * lda);
+        aoff[e] = (uint32_t)(sizeof(double) * (ta ? (k + i * lda) : (i + k * lda)));
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
         int j, k;
         b_el(e, j, k);
-        boff[e] = tb ? (uint32_t)(j + k * ldb) : (uint32_t)(k + j * ldb);
+        boff[e] = (uint32_t)(sizeof(double) * (tb ? (j + k * ldb) : (k + j * ldb)));
     }
+    auto ld_at = [](const double *base, uint32_t off) {
+        return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    };
     const int64_t astep = ta ? BK : BK * lda;  // pointer advance per K slab
     const int64_t bstep = tb ? BK * ldb : BK;
     const double *__restrict__ Aslab = A + G.a_off + (ta ? row0 * lda + kbeg : row0 + kbeg * lda);
@@ -201,21 +205,24 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     auto load_into = [&](double (&xa)[AE], double (&xb)[BE], int64_t k0) {
         if (tile_full && k0 + BK <= K) {
 #pragma unroll
-            for (int e = 0; e < AE; ++e) xa[e] = Aslab[aoff[e]];
+            for (int e = 0; e < AE; ++e) xa[e] = ld_at(Aslab, aoff[e]);
 #pragma unroll
-            for (int e = 0; e < BE; ++e) xb[e] = Bslab[boff[e]];
+            for (int e = 0; e < BE; ++e) xb[e] = ld_at(Bslab, boff[e]);
         } else {
+            // edge tile or last slab: 32-bit limits, uniform over the workgroup
+            const int ilim = (int)min((int64_t)BM, m - row0), jlim = (int)min((int64_t)BN, n - col0);
+            const int klim = (int)min((int64_t)BK, K - k0);
 #pragma unroll
             for (int e = 0; e < AE; ++e) {
                 int i, k;
                 a_el(e, i, k);
-                xa[e] = (row0 + i < m && k0 + k < K) ? Aslab[aoff[e]] : 0.0;
+                xa[e] = (i < ilim && k < klim) ? ld_at(Aslab, aoff[e]) : 0.0;
             }
 #pragma unroll
             for (int e = 0; e < BE; ++e) {
                 int j, k;
                 b_el(e, j, k);
-                xb[e] = (col0 + j < n && k0 + k < K) ? Bslab[boff[e]] : 0.0;
+                xb[e] = (j < jlim && k < klim) ? ld_at(Bslab, boff[e]) : 0.0;
             }
         }
         Aslab += astep;
@@ -239,22 +246,27 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     auto store_slab = [&](int buf) { store_from(buf, ra, rb); };
 
     const int64_t nk = (K - kbeg + BK - 1) / BK;
-    // MFMA s of a K slab uses k = 4*(lane/16) + s for both operands: each lane reads
-    // its 4 k values of a row as one contiguous 4-element vector.
-    const int kq = 4 * (lane >> 4);
+    // Fragments are read from LDS as one 16-byte vector per lane and operand: KV = 2
+    // (FP64) or 4 (FP32) consecutive k of one row.  Lane group g = lane/16 holds
+    // k = kk + KV·g + s for MFMA step s < KV — the same map for A and B, so each step's
+    // 4-deep MFMA sums 4 distinct k and a round of KV steps covers KC = 4·KV k.  Reading
+    // 16 rather than 32 bytes per lane halves the live FP64 fragment registers.
+    constexpr int KV = 16 / (int)sizeof(TC), KC = 4 * KV;
+    static_assert(BK % KC == 0, "K slab must hold whole fragment rounds");
+    const int kq = KV * (lane >> 4);
     auto mma_slab = [&](int buf) {
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            typedef TC v4_t __attribute__((ext_vector_type(4)));
-            v4_t af[FM], bfr[FN];
+        for (int kk = 0; kk < BK; kk += KC) {
+            typedef TC vk_t __attribute__((ext_vector_type(KV)));
+            vk_t af[FM], bfr[FN];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
-                af[i] = *reinterpret_cast<const v4_t *>(&As[buf][wm * WM + i * 16 + (lane & 15)][kk + kq]);
+                af[i] = *reinterpret_cast<const vk_t *>(&As[buf][wm * WM + i * 16 + (lane & 15)][kk + kq]);
 #pragma unroll
             for (int j = 0; j < FN; ++j)
-                bfr[j] = *reinterpret_cast<const v4_t *>(&Bs[buf][wn * WN + j * 16 + (lane & 15)][kk + kq]);
+                bfr[j] = *reinterpret_cast<const vk_t *>(&Bs[buf][wn * WN + j * 16 + (lane & 15)][kk + kq]);
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < KV; ++s)
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -1030,7 +1042,8 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         const bool few_tiles = ((G.m + 63) / 64) * ((G.n + 63) / 64) <= 4;
         const bool to_skinny = skinny_shape(G.m, G.n, G.k) && !(any_mfma && few_tiles);
         if (G.k > 0 && G.m > 0 && G.n > 0 && !to_skinny &&
-            (G.lda * (G.ta ? 128 : 32) >= ((int64_t)1 << 31) || G.ldb * (G.tb ? 32 : 128) >= ((int64_t)1 << 31))) {
+            (G.lda * (G.ta ? 128 : 32) >= ((int64_t)1 << 29) || G.ldb * (G.tb ? 32 : 128) >= ((int64_t)1 << 29))) {
+            // the MFMA kernels address a slab with 32-bit byte offsets from its base
             free_gemm(p);
             TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "tkmi_gemm_plan_create: leading dimension too large for 32-bit tile offsets");
         }
