@@ -733,6 +733,11 @@ struct tkmi_gemm_plan_s {
     double flops = 0;
     TileSet ts[NCFG];
     int choice[3] = {-1, -1, -1};  // per precision: config index, -1 = automatic
+    // α = 0: C_g = β·C_g without reading A or B (mul! reduces to rmul!), one list of C
+    // blocks per part (MFMA groups, skinny groups), run through splitk_reduce with no splits
+    std::vector<RGroup> scale[2];
+    int64_t scale_total[2] = {0, 0};
+    RGroup *d_scale[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -989,6 +994,7 @@ void free_gemm(tkmi_gemm_plan_s *p) {
     (void)hipFree(p->panels);
     (void)hipFree(p->d_pack);
     (void)hipFree(p->skinny.d_groups);
+    for (auto *d : p->d_scale) (void)hipFree(d);
     for (auto &t : p->ts) {
         (void)hipFree(t.d_groups);
         (void)hipFree(t.d_rgroups);
@@ -1103,6 +1109,16 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         p->pack_total += ((mp + np) / PACK_T) * tk;
         p->pack.push_back(P);
     }
+    // C blocks of each part for α = 0 calls
+    for (int q = 0; q < 2; ++q) {
+        for (const GGroup &G : q == 0 ? p->base : p->skinny.groups) {
+            if (G.m * G.n == 0) continue;
+            p->scale[q].push_back(RGroup{G.m, G.n, G.c_off, G.ldc, 0, 0, p->scale_total[q]});
+            p->scale_total[q] += G.m * G.n;
+        }
+        if (!rc && !p->scale[q].empty())
+            rc = tkmi::upload((void **)&p->d_scale[q], p->scale[q].data(), p->scale[q].size() * sizeof(RGroup));
+    }
     for (int c = 0; c < NCFG && !rc; ++c) rc = build_tiles(p, c);
     if (rc) {
         free_gemm(p);
@@ -1140,6 +1156,18 @@ int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, 
     if (int rc = check_prec(precision)) return rc;
     hipStream_t s = tkmi::as_stream(stream);
     int rc = TKMI_OK;
+    if (alpha == 0.0) {
+        // BLAS convention: A and B are not read, C = β·C (β = 0 writes zeros)
+        for (int q = 0; q < 2; ++q) {
+            if (!(parts & (1 << q)) || plan->scale_total[q] == 0) continue;
+            const int64_t blocks = std::min<int64_t>((plan->scale_total[q] + NT - 1) / NT, 8192);
+            hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(NT), 0, s, plan->d_scale[q],
+                               (int64_t)plan->scale[q].size(), plan->scale_total[q], (const double *)nullptr, C,
+                               0.0, beta);
+        }
+        TKMI_HIP(hipGetLastError());
+        return TKMI_OK;
+    }
     if (parts & 1) rc = launch_cfg(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s);
     if (rc) return rc;
     if (parts & 2) rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
