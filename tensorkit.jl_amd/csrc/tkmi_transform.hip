@@ -217,14 +217,14 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     }
 }
 
-template <typename T, int L>
+// STAGED: the launch has transpose tiles, which stage through LDS.  A launch of copy
+// tiles only (STAGED = false) declares no LDS, so LDS does not cap how many of its
+// workgroups share a CU (for L = 2 and 4 the staging array is 36 KB).
+template <typename T, int L, bool STAGED>
 __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     const TBlock *__restrict__ blocks, int64_t nblocks, const TTree *__restrict__ trees,
     const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, double alpha,
     double beta, int64_t tile0) {
-    constexpr int LDSN = L * (class_area(L) + class_twmax(L));  // tW·(tR+1) ≤ area + tW per tree
-    __shared__ T lds[LDSN];
-
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
     // binary search: last item with tile_start <= gt
     int64_t lo = 0, hi = nblocks - 1;
@@ -236,10 +236,16 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
             hi = mid - 1;
     }
     const TBlock &B = blocks[lo];
-    if (B.mode == 0)
-        transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
-    else
-        transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
+    if constexpr (STAGED) {
+        constexpr int LDSN = L * (class_area(L) + class_twmax(L));  // tW·(tR+1) ≤ area + tW per tree
+        __shared__ T lds[LDSN];
+        if (B.mode == 0)
+            transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
+        else
+            transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
+    } else {
+        transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, nullptr);
+    }
 }
 
 // U[e] = coef[e]·g[idx[e]] (idx < 0: 0): the coefficients of a transform that depend on
@@ -281,6 +287,7 @@ struct PlanSide {
     std::vector<TBlock> blocks;
     int64_t ntiles = 0;
     TBlock *d_blocks = nullptr;
+    bool staged = false;  // some block uses a transpose tile (LDS staging)
 };
 
 }  // namespace
@@ -424,6 +431,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
         for (int c = 0; c < NCLASS; ++c) {
             sides[c].blocks.clear();
             sides[c].ntiles = 0;
+            sides[c].staged = false;
         }
         // A small plan runs as one launch of its largest class when that class holds at
         // most 4 trees: the ramp-up and tail of a second launch cost more than the
@@ -459,6 +467,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
                 tile_block(b, trees, class_L(c));
                 b.tile_start = sides[c].ntiles;
                 sides[c].ntiles += b.ntW * b.ntR * b.nbatch;
+                sides[c].staged |= (b.mode != 0);
                 sides[c].blocks.push_back(b);
             }
         }
@@ -545,8 +554,13 @@ void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const do
     const int64_t chunk = ((int64_t)1 << 31) / NTHREADS;
     for (int64_t t0 = 0; t0 < side.ntiles; t0 += chunk) {
         const unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
-        hipLaunchKernelGGL((transform_kernel<T, L>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                           (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
+        if (side.staged)
+            hipLaunchKernelGGL((transform_kernel<T, L, true>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
+                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
+        else
+            hipLaunchKernelGGL((transform_kernel<T, L, false>), dim3(grid), dim3(NTHREADS), 0, stream,
+                               side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta,
+                               t0);
     }
 }
 
