@@ -34,8 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
 # Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
 TRAFFIC_PMC = {
-    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): (86757 * 2 + 68107) * 1024,  # L·x
-    ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): (106237 * 2 + 57125) * 1024,  # ·R, split-K 5 partials
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): (87318 * 2 + 67287) * 1024,  # L·x
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): (106296 * 2 + 56895) * 1024,  # ·R, split-K 5 partials
 }
 
 
@@ -358,7 +358,7 @@ def main():
             plan = T.treebraider(Bs.space, Ws, (3, 1), (2, 0))
             su2_report = {"elements": nS, "ms": round(ms_su2, 3), "GB/s": round(gbs, 1),
                           "roofline_frac": round(gbs / PEAK_HBM_GBS, 4), "fusion_blocks": plan.nentries,
-                          "kernel": "transform_kernel<double, L>, one launch per tree class L = 1, 2, 4, 8"}
+                          "kernel": "transform_kernel<double, L, true>, one launch per tree class L = 1, 2, 4, 8"}
             log(f"SU2 permute 82.8 GB: {ms_su2:.2f} ms, {gbs:.0f} GB/s")
             del As, Bs
             torch.cuda.empty_cache()
