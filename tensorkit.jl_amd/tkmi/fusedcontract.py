@@ -160,7 +160,7 @@ def pack_fused(rows):
     }
 
 
-_FUSED_CACHE = {}
+_FUSED_CACHE = TT._LRU(TT.DEFAULT_GLOBALCACHE_SIZE)
 
 
 def fused_plan(C, A, pA, B, pB, pC, twistA_legs, twistB_legs):
@@ -178,7 +178,7 @@ def fused_plan(C, A, pA, B, pB, pC, twistA_legs, twistB_legs):
                    "tkmi_transform_plan_set_gather")
         plan.gather = (coef, idx)
         plan.flops = float(sum(2.0 * m * n_ * k for _, (m, n_, k) in _gemm_shapes(C, A, pA, B, pB)))
-        _FUSED_CACHE[key] = plan
+        _FUSED_CACHE.put(key, plan)
     return plan
 
 

@@ -100,7 +100,7 @@ class TwoSiteHeff:
         return sum(p.flops for p in self._plans())
 
     def _plans(self):
-        return list(T._GEMM_CACHE.values())
+        return list(T._GEMM_CACHE.d.values())
 
     def capture(self, precision=None):
         """Capture the whole chain in one HIP graph (intermediates live in the
@@ -113,8 +113,10 @@ class TwoSiteHeff:
         with torch.cuda.stream(s):
             saved = self.outs
             self.outs = [None] * len(STEPS)
-            with torch.cuda.graph(g, stream=s):
+            # the graph keeps the device addresses of every plan it launches: pin them
+            with T.pinning() as pins, torch.cuda.graph(g, stream=s):
                 y = self.apply(precision=precision)
+            self.graph_pins = pins
             self.graph_outs = self.outs
             self.outs = saved
         torch.cuda.current_stream().wait_stream(s)

@@ -22,7 +22,8 @@ from . import _lib
 from .spaces import HomSpace, SpaceMismatch, IndexError_
 from .sectors import BOSONIC, FERMIONIC
 from .spaces import ProductSpace, SectorMismatch
-from .treetransformers import treebraider, flat_plan, twist_factor, tracer, _make_trace_plan, _TRANSFORMER_CACHE
+from .treetransformers import (treebraider, flat_plan, twist_factor, tracer, _make_trace_plan, _TRANSFORMER_CACHE,
+                               _LRU, pinning, DEFAULT_GLOBALCACHE_SIZE)
 from . import fusedcontract as _FC
 
 # contractions with a tiny operand and skinny per-sector GEMMs run as one fused
@@ -461,7 +462,7 @@ class GemmPlan:
             self.handle = None
 
 
-_GEMM_CACHE = {}
+_GEMM_CACHE = _LRU(DEFAULT_GLOBALCACHE_SIZE)
 
 
 def _block_layout(t):
@@ -503,7 +504,7 @@ def gemm_plan(tC, tA, tB, with_beta_groups=True):
     flops = float(np.sum(2.0 * arr[:, 0] * arr[:, 1] * arr[:, 2])) if len(rows) else 0.0
     plan = GemmPlan(h, flops, len(rows))
     plan.table = arr
-    _GEMM_CACHE[key] = plan
+    _GEMM_CACHE.put(key, plan)
     return plan
 
 

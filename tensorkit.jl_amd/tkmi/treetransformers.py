@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import itertools
 from collections import OrderedDict
+from contextlib import contextmanager
 
 import numpy as np
 
@@ -47,23 +48,54 @@ class TransformPlan:
             self.handle = None
 
 
+# Open pinning() scopes: each is a list that keeps every plan handed out by a cache
+# inside the scope alive (see pinning).
+_PIN_STACK = []
+
+
+def _pin(plan):
+    for pins in _PIN_STACK:
+        pins.append(plan)
+
+
+@contextmanager
+def pinning():
+    """Keep every device plan looked up or created inside the scope alive for as long as
+    the yielded list lives.  A captured HIP graph holds only the device addresses of the
+    plans it launches (tables, split-K workspaces, split panels); a plan evicted from its
+    cache and collected afterwards would be freed under the graph."""
+    pins = []
+    _PIN_STACK.append(pins)
+    try:
+        yield pins
+    finally:
+        # by identity: two scopes' lists can hold the same plans and compare equal
+        del _PIN_STACK[next(i for i, x in enumerate(_PIN_STACK) if x is pins)]
+
+
 class _LRU:
+    """Bounded plan cache (least recently used evicted first); plans it hands out inside
+    a pinning() scope are pinned there."""
+
     def __init__(self, maxsize):
         self.maxsize = maxsize
         self.d = OrderedDict()
         self.hits = 0
         self.misses = 0
+        _ALL_CACHES.append(self)
 
     def get(self, key):
         v = self.d.get(key)
         if v is not None:
             self.d.move_to_end(key)
             self.hits += 1
+            _pin(v)
         else:
             self.misses += 1
         return v
 
     def put(self, key, v):
+        _pin(v)
         self.d[key] = v
         self.d.move_to_end(key)
         while len(self.d) > self.maxsize:
@@ -73,12 +105,15 @@ class _LRU:
         self.d.clear()
 
 
+_ALL_CACHES = []
 _TRANSFORMER_CACHE = _LRU(DEFAULT_GLOBALCACHE_SIZE)
 
 
 def empty_globalcaches():
-    """`empty_globalcaches!()` (`src/auxiliary/caches.jl:2-5`)."""
-    _TRANSFORMER_CACHE.clear()
+    """`empty_globalcaches!()` (`src/auxiliary/caches.jl:2-5`): the transformer cache and
+    this package's other plan caches (grouped GEMMs, fused contractions)."""
+    for c in _ALL_CACHES:
+        c.clear()
 
 
 def global_cache_info():

@@ -1,0 +1,51 @@
+"""CPU: the plan caches (LRU, `src/auxiliary/caches.jl:2-19`) and plan pinning for
+captured graphs (no device: plain objects stand in for plans)."""
+import gc
+import weakref
+
+from tkmi import treetransformers as TT
+
+
+class _Plan:
+    pass
+
+
+def test_lru_evicts_least_recently_used():
+    c = TT._LRU(2)
+    a, b, d = _Plan(), _Plan(), _Plan()
+    c.put("a", a)
+    c.put("b", b)
+    assert c.get("a") is a          # a becomes the most recent
+    c.put("d", d)                   # evicts b
+    assert c.get("b") is None and c.get("a") is a and c.get("d") is d
+    assert c.hits == 3 and c.misses == 1
+
+
+def test_pinning_keeps_evicted_plans_alive():
+    c = TT._LRU(1)
+    p1 = _Plan()
+    c.put("k1", p1)
+    ref = weakref.ref(p1)
+    del p1
+    with TT.pinning() as outer:
+        with TT.pinning() as inner:
+            assert c.get("k1") is ref()
+        c.put("k2", _Plan())        # evicts k1 while pinned
+    gc.collect()
+    assert ref() is not None        # still held by the pins
+    assert ref() in inner and ref() in outer and len(outer) == 2
+    del outer, inner
+    gc.collect()
+    assert ref() is None            # released with the pins
+    assert c.get("k2") is not None  # outside a scope nothing is pinned
+    assert TT._PIN_STACK == []
+
+
+def test_empty_globalcaches_clears_every_plan_cache():
+    from tkmi import fusedcontract as FC
+    from tkmi import tensors as T
+    for cache in (TT._TRANSFORMER_CACHE, T._GEMM_CACHE, FC._FUSED_CACHE):
+        cache.put(("probe",), _Plan())
+    TT.empty_globalcaches()
+    for cache in (TT._TRANSFORMER_CACHE, T._GEMM_CACHE, FC._FUSED_CACHE):
+        assert cache.get(("probe",)) is None

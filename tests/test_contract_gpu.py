@@ -95,3 +95,24 @@ def test_heff_graph_replays(T):
         np.testing.assert_array_equal(y.numpy(), eager)
         H.apply()  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_graph_survives_cache_eviction(T):
+    """A captured graph pins the plans it launches: emptying every plan cache (and
+    collecting the evicted plans) leaves the replay intact."""
+    import gc
+    from tkmi import treetransformers as TT
+    from tkmi.heff import TwoSiteHeff, two_site_spaces
+    V, P, W = two_site_spaces(D=256, qmax=6)
+    H = TwoSiteHeff(V, P, W, np.random.default_rng(6))
+    eager = H.apply().numpy().copy()
+    H.capture()
+    assert H.graph_pins  # the GEMM, transform and fused plans of the chain
+    TT.empty_globalcaches()
+    H.outs = [None] * len(H.outs)  # drop the eager intermediates as well
+    gc.collect()
+    torch.cuda.synchronize()
+    for _ in range(2):
+        y = H.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(y.numpy(), eager)
