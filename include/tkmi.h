@@ -96,8 +96,10 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
                              int64_t dst_len, int64_t src_len);
 
 /* dst = β·dst + α·transform(src).  dtype: TKMI_F64 or TKMI_F32 (both buffers).
- * conj is accepted for API symmetry with TO.tensoradd! and is a no-op on
- * real data.  src and dst must not overlap.                                  */
+ * src and dst must not overlap.  BLAS convention: β = 0 writes dst without reading
+ * it; α = 0 reads no source.  A plan that writes fewer than 2^25 elements and has no
+ * fusion block wider than 4 trees runs as one launch; larger plans run one launch
+ * per tree class (1, 2, 4, 8 trees per block).                               */
 int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
                         double alpha, double beta, void *stream);
 
@@ -114,7 +116,9 @@ int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const v
  * and runs the transform: a contraction C = β·C + α·A·B with a small operand B,
  * folded with the permutes of blas_contract! (src/tensors/tensoroperations.jl:392-462)
  * into one pass over A and C.  U belongs to the plan, so calls on one plan must not
- * overlap on different streams. */
+ * overlap on different streams.  Once a plan has a gather, tkmi_transform_exec and
+ * tkmi_transform_adjoint_exec on it return TKMI_ERR_ARGUMENT (its U has no meaning
+ * without the current g). */
 int tkmi_transform_plan_set_gather(tkmi_transform_plan plan, int64_t n, const double *coef, const int64_t *idx);
 int tkmi_transform_exec_gather(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
                                const void *gsrc, double alpha, double beta, void *stream);
@@ -138,6 +142,11 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
                           const int64_t *lda, const int32_t *transa, const int64_t *b_off,
                           const int64_t *ldb, const int32_t *transb, const int64_t *c_off,
                           const int64_t *ldc);
+/* C_g = α·op(A_g)·op(B_g) + β·C_g for every group, in precision TKMI_PREC_F64,
+ * TKMI_PREC_F32 (inputs rounded to FP32, FP32 accumulation) or TKMI_PREC_BF16X3 (each
+ * input split into three bf16 terms, six products, FP32/FP64 accumulation).  BLAS
+ * convention: β = 0 writes C without reading it; α = 0 reads neither A nor B
+ * (C_g = β·C_g, zeros for β = 0), as mul! reduces to rmul!.                     */
 int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta,
                    const double *A, const double *B, double *C, void *stream);
 /* Tile shape for `precision`: 0 = automatic, 64 / 128 = square tiles with a
