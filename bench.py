@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds-single", type=float, default=8.0,
+                    help="bound of the one-thread CPU sample (0: skip it)")
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-su2", action="store_true")
     # profiling modes for rocprofv3 --pmc passes (no JSON line): a plain device copy of
@@ -450,6 +452,25 @@ def main():
                                "ms_per_step": round(cpu_ms, 3), "gpu_speedup": round(cpu_ms / ms64, 2),
                                "rel_err_vs_gpu": cpu_err}
         log(f"cpu baseline {cpu_ms:.1f} ms/step, GPU speed-up x{cpu_ms/ms64:.1f}, rel err {cpu_err:.2e}")
+        # the reference's own published setting is one thread: the same replay with the
+        # permute loop and the host BLAS each on one thread, on a smaller bounded sample
+        try:
+            from threadpoolctl import threadpool_limits
+        except ImportError:
+            threadpool_limits = None
+        if threadpool_limits is not None and args.cpu_seconds_single > 0:
+            reps1, t1 = 0, 0.0
+            with threadpool_limits(limits=1):
+                while t1 < args.cpu_seconds_single and reps1 < 3:
+                    t0 = time.perf_counter()
+                    cpu_replay(trace, init, 1)
+                    t1 += time.perf_counter() - t0
+                    reps1 += 1
+            ms1 = t1 / reps1 * 1e3
+            out["cpu_baseline"]["single_thread"] = {"value": round(flops / ms1 / 1e6, 2), "unit": "GFLOP/s",
+                                                    "cores": 1, "ms_per_step": round(ms1, 1), "reps": reps1,
+                                                    "gpu_speedup": round(ms1 / ms64, 1)}
+            log(f"cpu baseline, one thread: {ms1:.1f} ms/step ({reps1} applications), GPU speed-up x{ms1/ms64:.0f}")
     print(json.dumps(out), flush=True)
 
 
