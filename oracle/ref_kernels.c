@@ -23,18 +23,20 @@
 
 #define MAXN 16
 
-static void strided_add(int N, const int64_t *sz, const int64_t *dstr, int64_t doff, const int64_t *sstr_dst_order,
-                        int64_t soff, const double *src, double *dst, double a, double b) {
+/* dst[view] = b·dst[view] + a·src[view'] over the outer iterations [o0, o1) of one
+ * strided view (outer = every dimension but the first, first dimension innermost). */
+static void strided_add_range(int N, const int64_t *sz, const int64_t *dstr, int64_t doff,
+                              const int64_t *sstr_dst_order, int64_t soff, const double *src, double *dst, double a,
+                              double b, int64_t o0, int64_t o1) {
     int64_t idx[MAXN];
-    int64_t total = 1;
-    for (int k = 0; k < N; ++k) {
-        idx[k] = 0;
-        total *= sz[k];
+    int64_t rem = o0;
+    for (int k = 1; k < N; ++k) {
+        idx[k] = rem % sz[k];
+        rem /= sz[k];
     }
-    if (total == 0) return;
-    int64_t n0 = N > 0 ? sz[0] : 1;
-    int64_t ds0 = N > 0 ? dstr[0] : 0, ss0 = N > 0 ? sstr_dst_order[0] : 0;
-    for (int64_t outer = 0; outer < total / n0; ++outer) {
+    const int64_t n0 = N > 0 ? sz[0] : 1;
+    const int64_t ds0 = N > 0 ? dstr[0] : 0, ss0 = N > 0 ? sstr_dst_order[0] : 0;
+    for (int64_t outer = o0; outer < o1; ++outer) {
         int64_t dp = doff, sp = soff;
         for (int k = 1; k < N; ++k) {
             dp += idx[k] * dstr[k];
@@ -52,21 +54,62 @@ static void strided_add(int N, const int64_t *sz, const int64_t *dstr, int64_t d
     }
 }
 
-/* structs: nentries × (2N+1) = sizes[N], strides[N], offset */
+static void strided_add(int N, const int64_t *sz, const int64_t *dstr, int64_t doff, const int64_t *sstr_dst_order,
+                        int64_t soff, const double *src, double *dst, double a, double b) {
+    int64_t total = 1;
+    for (int k = 0; k < N; ++k) total *= sz[k];
+    if (total == 0) return;
+    const int64_t n0 = N > 0 ? sz[0] : 1;
+    strided_add_range(N, sz, dstr, doff, sstr_dst_order, soff, src, dst, a, b, 0, total / n0);
+}
+
+/* structs: nentries × (2N+1) = sizes[N], strides[N], offset.  The work is split into
+ * items of about CHUNK elements (whole runs of the first dimension), so that one large
+ * sub-block spreads over every thread. */
+#define CHUNK ((int64_t)1 << 14)
 void oracle_abelian_permute(int N, const int32_t *perm, int64_t nentries, const double *coeffs,
                             const int64_t *dst_structs, const int64_t *src_structs, const double *src, double *dst,
                             double alpha, double beta, int nthreads) {
+    int64_t nitems = 0;
+    for (int64_t e = 0; e < nentries; ++e) {
+        const int64_t *ds = dst_structs + e * (2 * N + 1);
+        int64_t total = 1;
+        for (int k = 0; k < N; ++k) total *= ds[k];
+        if (total == 0) continue;
+        const int64_t n0 = N > 0 ? ds[0] : 1, outer = total / n0;
+        const int64_t step = n0 >= CHUNK ? 1 : CHUNK / n0;
+        nitems += (outer + step - 1) / step;
+    }
+    int64_t *items = (int64_t *)malloc(sizeof(int64_t) * 3 * (nitems > 0 ? nitems : 1));
+    int64_t q = 0;
+    for (int64_t e = 0; e < nentries; ++e) {
+        const int64_t *ds = dst_structs + e * (2 * N + 1);
+        int64_t total = 1;
+        for (int k = 0; k < N; ++k) total *= ds[k];
+        if (total == 0) continue;
+        const int64_t n0 = N > 0 ? ds[0] : 1, outer = total / n0;
+        const int64_t step = n0 >= CHUNK ? 1 : CHUNK / n0;
+        for (int64_t o = 0; o < outer; o += step) {
+            items[3 * q] = e;
+            items[3 * q + 1] = o;
+            items[3 * q + 2] = o + step < outer ? o + step : outer;
+            ++q;
+        }
+    }
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
-    for (int64_t e = 0; e < nentries; ++e) {
+    for (int64_t it = 0; it < nitems; ++it) {
+        const int64_t e = items[3 * it];
         const int64_t *ds = dst_structs + e * (2 * N + 1);
         const int64_t *ss = src_structs + e * (2 * N + 1);
         int64_t sstr[MAXN];
         for (int k = 0; k < N; ++k) sstr[k] = ss[N + perm[k]];
-        strided_add(N, ds, ds + N, ds[2 * N], sstr, ss[2 * N], src, dst, alpha * coeffs[e], beta);
+        strided_add_range(N, ds, ds + N, ds[2 * N], sstr, ss[2 * N], src, dst, alpha * coeffs[e], beta,
+                          items[3 * it + 1], items[3 * it + 2]);
     }
+    free(items);
 }
 
 /* generic: per block b, rows[b]×cols[b] U (column-major), sizes N per block,

@@ -135,3 +135,30 @@ def test_split_k_exact(lib, tile):
                 np.testing.assert_array_equal(out, ref)
             else:
                 assert np.linalg.norm(out - ref) <= 1e-6 * np.linalg.norm(ref)
+
+
+def test_split_k_model_engages_split(lib):
+    """The schedule model splits a lone long-K group (one output tile, 4096-deep K) and
+    keeps a plan with many output tiles and short K whole; the split result stays exact."""
+    rng = np.random.default_rng(22)
+    L = lib.load()
+    for shapes, expect_split in (([(64, 64, 4096)], True), ([(2048, 2048, 64)], False)):
+        tab, na, nb, nc = _make_groups(rng, shapes, [0], [0])
+        arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+        cols = [lib.i64(arr[:, i]) for i in range(11)]
+        ta, tb = lib.i32(arr[:, 5]), lib.i32(arr[:, 8])
+        h = C.c_void_p()
+        lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                          cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1],
+                                          cols[10][1]), "plan")
+        sk, slots = C.c_int64(), C.c_int64()
+        lib.check(L.tkmi_gemm_plan_splitk(h, 5, C.byref(sk), C.byref(slots)), "splitk")
+        L.tkmi_gemm_plan_destroy(h)
+        assert slots.value >= 256  # at least one workgroup per CU
+        assert (sk.value > 1) == expect_split, (shapes, sk.value)
+        if expect_split:
+            A = rng.integers(-4, 5, na).astype(np.float64)
+            B = rng.integers(-4, 5, nb).astype(np.float64)
+            Cm = rng.integers(-4, 5, nc).astype(np.float64)
+            np.testing.assert_array_equal(_run(lib, tab, A, B, Cm, 0, 1.0, -1.0, 1005),
+                                          _ref(tab, A, B, Cm, 1.0, -1.0))
