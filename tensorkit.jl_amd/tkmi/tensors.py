@@ -312,6 +312,56 @@ def _check_transpose(t, p1, p2):
     return p1, p2
 
 
+def _repartition_perm(tsrc, n1):
+    """(p₁, p₂) of a repartition (`indexmanipulations.jl:452-454`): the legs in the
+    cyclic order codomain…, reverse(domain)…, the first n1 into the codomain."""
+    inds = tsrc.codomainind() + tuple(reversed(tsrc.domainind()))
+    return inds[:n1], tuple(reversed(inds[n1:]))
+
+
+def repartition_(tdst, tsrc, alpha=1.0, beta=0.0):
+    """`repartition!(tdst, tsrc, α, β)` (`indexmanipulations.jl:445-457`): move legs
+    between codomain and domain without changing their cyclic order.  The linearised
+    leg order is unchanged, so no leg crosses another: the transform is bends only and
+    coincides with `permute!` for every braiding style."""
+    if tsrc.numind != tdst.numind:
+        raise ValueError("tsrc and tdst should have an equal amount of indices")
+    p1, p2 = _repartition_perm(tsrc, tdst.numout)
+    return permute_(tdst, tsrc, p1, p2, alpha, beta)
+
+
+def repartition(tsrc, n1, n2=None, copy=False):
+    """`repartition(tsrc, N₁, N₂; copy)` (`indexmanipulations.jl:474-484`)."""
+    n2 = tsrc.numind - n1 if n2 is None else n2
+    if n1 + n2 != tsrc.numind:
+        raise ValueError(f"({n1}, {n2}) is not a partition of the {tsrc.numind} indices of {tsrc.space}")
+    p1, p2 = _repartition_perm(tsrc, n1)
+    return permute(tsrc, p1, p2, copy=copy)
+
+
+def _deprecated(old, new):
+    import warnings
+    warnings.warn(f"`{old}` is deprecated, use `{new}` instead", DeprecationWarning, stacklevel=3)
+
+
+def add_permute_(tdst, tsrc, p, alpha, beta):
+    """Deprecated `add_permute!` (`indexmanipulations.jl:542-545`) → `permute!`."""
+    _deprecated("add_permute!", "permute!")
+    return permute_(tdst, tsrc, p[0], p[1], alpha, beta)
+
+
+def add_braid_(tdst, tsrc, p, levels, alpha, beta):
+    """Deprecated `add_braid!` (`indexmanipulations.jl:546-549`) → `braid!`."""
+    _deprecated("add_braid!", "braid!")
+    return braid_(tdst, tsrc, p[0], p[1], levels, alpha, beta)
+
+
+def add_transpose_(tdst, tsrc, p, alpha, beta):
+    """Deprecated `add_transpose!` (`indexmanipulations.jl:550-553`) → `transpose!`."""
+    _deprecated("add_transpose!", "transpose!")
+    return transpose_(tdst, tsrc, p[0], p[1], alpha, beta)
+
+
 def transpose_(tdst, tsrc, p1, p2, alpha=1.0, beta=0.0):
     """`transpose!(tdst, tsrc, (p₁, p₂), α, β)` (`indexmanipulations.jl:380-392`):
     the in-place planar transpose (bosonic sectors: a cyclic `permute!`)."""
