@@ -288,6 +288,7 @@ struct PlanSide {
     int64_t ntiles = 0;
     TBlock *d_blocks = nullptr;
     bool staged = false;  // some block uses a transpose tile (LDS staging)
+    bool reads = false;   // some block reads source trees (blocks with none only scale by β)
 };
 
 }  // namespace
@@ -432,6 +433,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             sides[c].blocks.clear();
             sides[c].ntiles = 0;
             sides[c].staged = false;
+            sides[c].reads = false;
         }
         // A small plan runs as one launch of its largest class when that class holds at
         // most 4 trees: the ramp-up and tail of a second launch cost more than the
@@ -468,6 +470,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
                 b.tile_start = sides[c].ntiles;
                 sides[c].ntiles += b.ntW * b.ntR * b.nbatch;
                 sides[c].staged |= (b.mode != 0);
+                sides[c].reads |= (b.nread > 0);
                 sides[c].blocks.push_back(b);
             }
         }
@@ -586,7 +589,12 @@ int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst
     if (!p) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
     const PlanSide *sides = adjoint ? p->adj : p->fwd;
     const double *dU = adjoint ? p->d_Ut : p->d_U;
-    if (total_tiles(sides) > 0 && (!src || !dst)) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
+    // a source may be empty (null) when no block reads it: e.g. a contraction with an
+    // empty operand into a non-empty C only scales C by β
+    bool reads = false;
+    for (int c = 0; c < NCLASS; ++c) reads |= sides[c].ntiles > 0 && sides[c].reads;
+    if ((total_tiles(sides) > 0 && !dst) || (reads && alpha != 0.0 && !src))
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
     if (dtype != TKMI_F64 && dtype != TKMI_F32)
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
     // a gather plan's coefficients are formed from gsrc at every call

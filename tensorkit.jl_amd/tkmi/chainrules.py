@@ -7,6 +7,8 @@ cotangent of the result to cotangents of the inputs, using the device kernels:
                   matrices) of the same cached plan — `tkmi_transform_adjoint_exec`
                   (`rrule(permute)`, ext/TensorKitChainRulesCoreExt/linalg.jl:62-70;
                   `permute_pullback_dA`, src/pullbacks/indexmanipulations.jl:1-17)
+- flip:           the opposite flip, flip(Δt′, I; inv = !inv)
+                  (ext/TensorKitChainRulesCoreExt/linalg.jl:99-103)
 - tensoradd!:     ext/TensorKitChainRulesCoreExt/tensoroperations.jl:8-58
 - tensorcontract!: ext/TensorKitChainRulesCoreExt/tensoroperations.jl:60-150
                   (dA = ΔC ⋆ twist(B), dB = twist(A) ⋆ ΔC as two more
@@ -58,6 +60,15 @@ def rrule_permute(t, p1, p2):
         _lib.check(lib.tkmi_transform_adjoint_exec(plan.handle, T._DT[dy.dtype], dy.data.data_ptr(),
                                                    dt.data.data_ptr(), 1.0, 0.0, T._stream()), "permute pullback")
         return dt
+
+    return y, pullback
+
+
+def rrule_flip(t, inds, inv=False):
+    y = T.flip(t, inds, inv)
+
+    def pullback(dy):
+        return T.flip(dy, inds, not inv)
 
     return y, pullback
 

@@ -15,6 +15,7 @@ Rules exist for the in-place primals of the path:
   tensors.trace_permute_   ext/TensorKitMooncakeExt/tensoroperations.jl:104-187
   tensors.mul_        ext/TensorKitMooncakeExt/linalg.jl:5-62, ext/TensorKitEnzymeExt/linalg.jl:4-91
   tensors.tr          ext/TensorKitMooncakeExt/linalg.jl:85-105, ext/TensorKitEnzymeExt/linalg.jl:93-153
+  tensors.flip        ext/TensorKitEnzymeExt/indexmanipulations.jl:119-170 (out of place)
 
 Argument annotations (the two packages' vocabulary):
   Duplicated(x, dx)  a tensor with its tangent storage (Mooncake CoDual / Dual, Enzyme
@@ -371,6 +372,39 @@ class _TrRule:
         return T.tr(_v(args[0])), (T.tr(dA) if dA is not None else None)
 
 
+class _FlipRule:
+    """flip(t, inds[, inv]): out of place.  The tape is the output's shadow, created zero,
+    into which the caller accumulates the output's cotangent (Enzyme); a cotangent given
+    to the reverse pass directly (Mooncake's pullback argument) takes its place.  The
+    reverse pass adds flip(Δt′, inds; inv = !inv) to Δt: flip is unitary for the inner
+    product Σ_c dim(c)·⟨b₁, b₂⟩, so its adjoint is its inverse, the opposite flip."""
+
+    @staticmethod
+    def _split(args):
+        inds = _v(args[1])
+        inds = (inds,) if isinstance(inds, int) else tuple(inds)
+        return _v(args[0]), inds, bool(_v(args[2])) if len(args) > 2 else False
+
+    def augmented(self, args):
+        t, inds, inv = self._split(args)
+        out = T.flip(t, inds, inv)
+        return out, T.TensorMap.zeros(out.space, out.dtype)
+
+    def reverse(self, args, cache, dret, restore):
+        t, inds, inv = self._split(args)
+        dt = _tangent(args[0])
+        cot = dret if dret is not None else cache
+        if dt is not None and cot is not None:
+            T.add_(dt, T.flip(cot, inds, not inv), 1.0, 1.0)
+        return (None,) * len(args)
+
+    def forward(self, args):
+        t, inds, inv = self._split(args)
+        out = T.flip(t, inds, inv)
+        dt = _tangent(args[0])
+        return out, (T.flip(dt, inds, inv) if dt is not None else T.TensorMap.zeros(out.space, out.dtype))
+
+
 _RULES = {
     T.permute_: _TransformRule("permute"),
     T.transpose_: _TransformRule("transpose"),
@@ -380,6 +414,7 @@ _RULES = {
     T.trace_permute_: _TraceRule(),
     T.mul_: _MulRule(),
     T.tr: _TrRule(),
+    T.flip: _FlipRule(),
 }
 
 

@@ -277,6 +277,15 @@ class HomSpace:
             raise ValueError(f"{(p1, p2)} is not a valid permutation for {self}")
         return self.select(p1, p2)
 
+    def flip(self, inds):
+        """`flip(W, I)` (`homspace.jl:252-261`): flip the codomain and domain spaces whose
+        linear (0-based) index is in `inds`."""
+        inds = set(inds)
+        n1 = self.numout
+        cod = ProductSpace(*[V.flip() if i in inds else V for i, V in enumerate(self.codomain)])
+        dom = ProductSpace(*[V.flip() if i + n1 in inds else V for i, V in enumerate(self.domain)])
+        return HomSpace(cod, dom, unit=self.unit)
+
     def compose(self, other):
         if self.domain != other.codomain:
             raise SpaceMismatch(f"{self.domain} ≠ {other.codomain}")

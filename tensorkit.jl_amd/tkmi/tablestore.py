@@ -87,6 +87,10 @@ def key_to_json(key):
                 "p2": list(p2), "q1": list(q1), "q2": list(q2)}
     if kind in ("tr", "tr_adj"):
         return {"kind": kind, "space": homspace_to_json(key[1])}
+    if kind == "flip":
+        _, Vd, Vs, inds, inv = key
+        return {"kind": kind, "dst": homspace_to_json(Vd), "src": homspace_to_json(Vs), "inds": list(inds),
+                "inv": bool(inv)}
     if kind == "flat":
         return {"kind": kind, "length": int(key[1])}
     raise TypeError(f"no table-file encoding for cache key kind {kind!r}")
@@ -103,6 +107,8 @@ def key_from_json(d):
                 tuple(d["p2"]), tuple(d["q1"]), tuple(d["q2"]))
     if kind in ("tr", "tr_adj"):
         return (kind, homspace_from_json(d["space"]))
+    if kind == "flip":
+        return ("flip", homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["inds"]), bool(d["inv"]))
     if kind == "flat":
         return ("flat", int(d["length"]))
     raise ValueError(f"unknown cache key kind {kind!r}")

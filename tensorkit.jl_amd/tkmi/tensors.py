@@ -22,6 +22,7 @@ from . import _lib
 from .spaces import HomSpace, SpaceMismatch, IndexError_
 from .sectors import BOSONIC, FERMIONIC
 from .spaces import ProductSpace, SectorMismatch
+from . import treetransformers as TT
 from .treetransformers import (treebraider, flat_plan, twist_factor, tracer, _make_trace_plan, _TRANSFORMER_CACHE,
                                _LRU, pinning, DEFAULT_GLOBALCACHE_SIZE)
 from . import fusedcontract as _FC
@@ -377,14 +378,59 @@ def planarcontract_(C, A, pA, B, pB, pAB, alpha=1.0, beta=0.0, precision=None):
     return contract_(C, A, pA, B, pB, pAB, alpha, beta, precision)
 
 
-def twist_(t, inds):
-    """`twist!(t, inds)`: scale each sub-block by its twist θ (fermionic only;
-    bosonic sectors have θ=1 and nothing is launched)."""
-    if t.space.unit.braiding_style != FERMIONIC or not inds:
+def _legs(t, inds):
+    inds = (inds,) if isinstance(inds, int) else tuple(inds)
+    if not all(0 <= i < t.numind for i in inds):
+        raise ValueError(f"Can't twist indices {inds} of a tensor with only {t.numind} indices.")
+    return inds
+
+
+def has_shared_twist(t, inds):
+    """`has_shared_twist(t, inds)` (`indexmanipulations.jl:34-51`): the twist of those legs
+    is the identity (bosonic braiding, or every sector on them has θ = 1)."""
+    if t.space.unit.braiding_style == BOSONIC:
+        return True
+    W = t.space
+    return all(c.twist() == 1 for i in inds for c in W[i].sectors())
+
+
+def twist_(t, inds, inv=False):
+    """`twist!(t, inds; inv)` (`indexmanipulations.jl:62-78`): scale each sub-block by the
+    product of the twists θ of the legs in `inds` (θ⁻¹ with `inv`; the twists here are
+    ±1, their own inverses).  Nothing is launched when the twist is the identity."""
+    inds = _legs(t, inds)
+    if not inds or has_shared_twist(t, inds):
         return t
     tmp = t.copy()
-    add_transform_(t, tmp, t.codomainind(), t.domainind(), 1.0, 0.0, dst_twist=tuple(inds))
+    add_transform_(t, tmp, t.codomainind(), t.domainind(), 1.0, 0.0, dst_twist=inds)
     return t
+
+
+def twist(t, inds, inv=False, copy=False):
+    """`twist(t, inds; inv, copy)` (`indexmanipulations.jl:90-97`): a twisted copy, or `t`
+    itself when the twist is the identity and `copy` is false."""
+    inds = _legs(t, inds)
+    if not copy and has_shared_twist(t, inds):
+        return t
+    return twist_(t.copy(), inds, inv)
+
+
+def flip(t, inds, inv=False):
+    """`flip(t, I; inv)` (`indexmanipulations.jl:21-29`): a tensor isomorphic to `t` whose
+    legs in I have their arrows flipped (`space(t′, i) = flip(space(t, i))`); each tree
+    pair's sub-block is scaled by the coefficient of `flip((f₁, f₂), I; inv)`.  Not an
+    involution: flip(flip(t, I), I; inv=True) == t."""
+    inds = (inds,) if isinstance(inds, int) else tuple(inds)
+    if not all(0 <= i < t.numind for i in inds):
+        raise ValueError(f"Can't flip indices {inds} of a tensor with only {t.numind} indices.")
+    if isinstance(t, AdjointTensorMap):
+        t = permute(t, t.codomainind(), t.domainind(), copy=True)
+    W2 = t.space.flip(inds)
+    plan = TT.flipper(W2, t.space, inds, inv)
+    out = TensorMap.empty(W2, t.dtype)
+    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[t.dtype], t.data.data_ptr(), out.data.data_ptr(),
+                                               1.0, 0.0, _stream()), "flip")
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -22,7 +22,7 @@ from contextlib import contextmanager
 import numpy as np
 
 from . import _lib
-from .fusiontrees import braid_pair, braid_block, permute_block, split_tree
+from .fusiontrees import FusionTree, braid_pair, braid_block, permute_block, split_tree
 from .sectors import UNIQUE
 
 DEFAULT_GLOBALCACHE_SIZE = 10 ** 4  # src/auxiliary/caches.jl:19
@@ -293,6 +293,66 @@ def adjoint_structure(Vadj):
     for (f1, f2), (sz, st, off) in ps.items():
         out[(f2, f1)] = (tuple(sz[n1p:]) + tuple(sz[:n1p]), tuple(st[n1p:]) + tuple(st[:n1p]), off)
     return out
+
+
+def flip_pair(f1, f2, inds, inv=False):
+    """`flip((f₁, f₂), ind; inv)` (`braiding_manipulations.jl:388-436`): toggle the duality
+    flag of every leg in `inds` (0-based, in order), accumulating per leg
+    codomain leg a:  isdual → χₐ·θₐ, else 1        (inv: isdual → 1, else conj(χₐ·θₐ))
+    domain leg a:    isdual → conj(χₐ), else θₐ    (inv: isdual → conj(θₐ), else χₐ)
+    with χ the Frobenius–Schur phase and θ the twist (real for the sectors here)."""
+    n1 = len(f1.uncoupled)
+    d1, d2 = list(f1.isdual), list(f2.isdual)
+    factor = 1.0
+    for i in inds:
+        if i < n1:
+            a = f1.uncoupled[i]
+            chi, th = a.frobenius_schur_phase(), a.twist()
+            if not inv:
+                factor *= chi * th if d1[i] else 1
+            else:
+                factor *= 1 if d1[i] else chi * th
+            d1[i] = not d1[i]
+        else:
+            j = i - n1
+            a = f2.uncoupled[j]
+            chi, th = a.frobenius_schur_phase(), a.twist()
+            if not inv:
+                factor *= chi if d2[j] else th
+            else:
+                factor *= th if d2[j] else chi
+            d2[j] = not d2[j]
+    g1 = FusionTree(f1.uncoupled, f1.coupled, d1, f1.innerlines, f1.vertices)
+    g2 = FusionTree(f2.uncoupled, f2.coupled, d2, f2.innerlines, f2.vertices)
+    return (g1, g2), float(factor)
+
+
+def flipper(Vdst, Vsrc, inds, inv=False):
+    """Cached device plan of `flip(t, I; inv)` (`indexmanipulations.jl:21-29`): every
+    tree pair keeps its sub-block (same sizes, identity leg order) at the flipped space's
+    offset, scaled by its `flip_pair` coefficient — one entry per tree pair on the
+    abelian transform kernel, for any fusion style (flip never mixes trees)."""
+    key = ("flip", Vdst, Vsrc, tuple(inds), bool(inv))
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is not None:
+        return plan
+    fs, fd = Vsrc.subblockstructure(), Vdst.subblockstructure()
+    coeffs, dst, src = [], [], []
+    for (f1, f2), (sz, st, off) in fs.items():
+        pair, c = flip_pair(f1, f2, inds, inv)
+        szd, std, offd = fd[pair]
+        if tuple(szd) != tuple(sz):
+            raise AssertionError("flip: sub-block sizes differ")
+        coeffs.append(c)
+        dst.append(list(szd) + list(std) + [offd])
+        src.append(list(sz) + list(st) + [off])
+    n = Vsrc.numind
+    tab = {"coeffs": np.array(coeffs, dtype=np.float64),
+           "dst": np.array(dst, dtype=np.int64).reshape(-1, 2 * n + 1),
+           "src": np.array(src, dtype=np.int64).reshape(-1, 2 * n + 1)}
+    plan = _make_abelian_plan(tab, tuple(range(n)), Vdst.dim(), Vsrc.dim())
+    _TRANSFORMER_CACHE.put(key, plan)
+    return plan
 
 
 def flat_plan(length):

@@ -126,3 +126,34 @@ def test_unmatched_blocks_scale_by_beta(T):
         ref = -0.5 * c0[c] + (1.25 * a[c] @ b[c] if c in a and c in b else 0.0)
         assert np.allclose(blk, ref, rtol=1e-13, atol=1e-13), c
     assert any(c not in a for c in got)  # the charge-2 block has no A·B partner
+
+
+def test_empty_operand_into_nonempty_result(T):
+    """A contraction whose operand has no blocks still writes C = β·C (zeros for β = 0):
+    the fused small-operand transform then has destinations without sources and never
+    reads A, so an empty A (no data pointer) is valid — fused and unfused paths agree."""
+    from tkmi.sectors import U1Irrep, FermionParity, ProductSector
+    from tkmi.spaces import GradedSpace
+    f = lambda pairs: GradedSpace({ProductSector(FermionParity(p), U1Irrep(q)): n for (p, q), n in pairs})
+    V1, V2, V3 = f([((0, 0), 2), ((1, 1), 1)]), f([((1, -1), 2), ((0, 0), 1)]), f([((0, 1), 1), ((1, 0), 2)])
+    V4, V5 = f([((0, 0), 1), ((1, 1), 1), ((1, -1), 1)]), f([((1, 0), 1), ((0, -1), 2), ((0, 0), 1)])
+    rng = np.random.default_rng(5)
+    t1 = T.TensorMap.randn(HomSpace(ProductSpace(V1, V2, V3), ProductSpace(V4)), rng)
+    t2 = T.TensorMap.randn(HomSpace(ProductSpace(V2.dual(), V5), ProductSpace(V4.dual(), V1)), rng)
+    assert t1.dim() == 0 and t2.dim() > 0
+    from tkmi import fusedcontract as FC
+    pA, pB, pAB = T.contraction_indices("xyaz", "ybzx", "ab", 1)
+    VC = T.tensorcontract_space(t1, pA, False, t2, pB, False, pAB)
+    assert FC.eligible(T.TensorMap.empty(VC), t1, pA, t2, pB)
+    saved = T.FUSE_SMALL_OPERAND
+    try:
+        for fuse in (True, False):
+            T.FUSE_SMALL_OPERAND = fuse
+            C = T.contract_labels(t1, "xyaz", t2, "ybzx", "ab", 1)
+            assert C.dim() > 0 and not C.numpy().any()
+            C0 = T.TensorMap.randn(C.space, rng)
+            C2 = C0.copy()
+            T.contract_labels(t1, "xyaz", t2, "ybzx", "ab", 1, out=C2, beta=-0.5)
+            assert np.array_equal(C2.numpy(), -0.5 * C0.numpy())
+    finally:
+        T.FUSE_SMALL_OPERAND = saved
