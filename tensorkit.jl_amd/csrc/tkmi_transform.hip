@@ -15,8 +15,11 @@
 //
 // Every thread owns E elements of each of the L trees of a tile (E = 8 for L = 1,
 // else 16/L), all loaded before the first LDS store; the tile area is 256·E
-// elements per tree.  Loads and stores are non-temporal: every element is touched
-// once, so the transform streams through the caches.
+// elements per tree.  Loads are non-temporal: every element is read once.  Stores
+// are non-temporal when a launch writes more than KEEP_BYTES; smaller results are
+// stored normally so that they can stay in the 256 MB Infinity Cache for the kernel
+// that consumes them next (measured on the U(1) D = 2052 two-site H_eff chain: 2%
+// faster per application; non-temporal stores are 11% faster on a 4.8 GB permute).
 //
 // Generic (non-abelian) blocks carry R destination and C source trees of identical
 // shape plus a dense R × C recoupling matrix U; each written element is
@@ -38,6 +41,7 @@ constexpr int NTHREADS = 256;
 constexpr int NCLASS = 4;  // L = 1, 2, 4, 8
 constexpr int MAXW = 8;    // destination trees per work item
 constexpr int64_t SMALL_PLAN = (int64_t)1 << 25;  // written elements below which classes merge
+constexpr int64_t KEEP_BYTES = (int64_t)128 << 20;  // written bytes up to which stores stay cached
 
 struct TBlock {
     int32_t ndim;
@@ -76,8 +80,8 @@ constexpr int class_L(int c) { return 1 << c; }
 template <typename T, int L, bool COPY>
 __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__restrict__ trees,
                                                const double *__restrict__ Umat, const T *__restrict__ src,
-                                               T *__restrict__ dst, double alpha, double beta, int64_t t,
-                                               T *__restrict__ lds) {
+                                               T *__restrict__ dst, double alpha, double beta, bool keep,
+                                               int64_t t, T *__restrict__ lds) {
     constexpr int E = class_elems(L);
     const int nread = B.nread, nwrite = B.nwrite;
     const int dW = B.dW, dR = B.dR;
@@ -199,22 +203,28 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 
     const T tbeta = (T)beta;
     const bool has_beta = (beta != 0.0);
+    auto store_all = [&](auto store) {
 #pragma unroll
-    for (int j = 0; j < L; ++j) {
-        if (j < nwrite) {
-            const TTree &tw = trees[B.write_tree + j];
-            T *dp = dst + corner(tw);
-            const int64_t sW = tw.str[dW], sR = tw.str[dR];
+        for (int j = 0; j < L; ++j) {
+            if (j < nwrite) {
+                const TTree &tw = trees[B.write_tree + j];
+                T *dp = dst + corner(tw);
+                const int64_t sW = tw.str[dW], sR = tw.str[dR];
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                int w, r;
-                if (pos_write(k, w, r)) {
-                    T *a = dp + (int64_t)w * sW + (int64_t)r * sR;
-                    __builtin_nontemporal_store(has_beta ? o[j][k] + tbeta * *a : o[j][k], a);
+                for (int k = 0; k < E; ++k) {
+                    int w, r;
+                    if (pos_write(k, w, r)) {
+                        T *a = dp + (int64_t)w * sW + (int64_t)r * sR;
+                        store(has_beta ? o[j][k] + tbeta * *a : o[j][k], a);
+                    }
                 }
             }
         }
-    }
+    };
+    if (keep)
+        store_all([](T v, T *a) { *a = v; });
+    else
+        store_all([](T v, T *a) { __builtin_nontemporal_store(v, a); });
 }
 
 // STAGED: the launch has transpose tiles, which stage through LDS.  A launch of copy
@@ -224,7 +234,7 @@ template <typename T, int L, bool STAGED>
 __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     const TBlock *__restrict__ blocks, int64_t nblocks, const TTree *__restrict__ trees,
     const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, double alpha,
-    double beta, int64_t tile0) {
+    double beta, int keep, int64_t tile0) {
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
     // binary search: last item with tile_start <= gt
     int64_t lo = 0, hi = nblocks - 1;
@@ -240,11 +250,11 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
         constexpr int LDSN = L * (class_area(L) + class_twmax(L));  // tW·(tR+1) ≤ area + tW per tree
         __shared__ T lds[LDSN];
         if (B.mode == 0)
-            transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
+            transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds);
         else
-            transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, lds);
+            transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds);
     } else {
-        transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, gt - B.tile_start, nullptr);
+        transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, nullptr);
     }
 }
 
@@ -289,6 +299,7 @@ struct PlanSide {
     TBlock *d_blocks = nullptr;
     bool staged = false;  // some block uses a transpose tile (LDS staging)
     bool reads = false;   // some block reads source trees (blocks with none only scale by β)
+    int64_t written = 0;  // elements written by all classes of this side together
 };
 
 }  // namespace
@@ -434,6 +445,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             sides[c].ntiles = 0;
             sides[c].staged = false;
             sides[c].reads = false;
+            sides[c].written = 0;
         }
         // A small plan runs as one launch of its largest class when that class holds at
         // most 4 trees: the ramp-up and tail of a second launch cost more than the
@@ -449,6 +461,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             for (int jc = 0; jc < R; jc += MAXW) cmax = std::max(cmax, class_of(Cn, std::min(MAXW, R - jc)));
         }
         const bool merge = moved <= SMALL_PLAN && class_L(cmax) <= 4;
+        for (int c = 0; c < NCLASS; ++c) sides[c].written = moved;
         for (auto &it : items) {
             const int R = adjoint ? it.nread : it.nwrite;   // written trees on this side
             const int Cn = adjoint ? it.nwrite : it.nread;  // read trees on this side
@@ -552,6 +565,7 @@ int add_item(tkmi_transform_plan_s *p, std::vector<HostItem> &items, const std::
 template <typename T, int L>
 void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const double *dU, const T *src, T *dst,
                   double alpha, double beta, hipStream_t stream) {
+    const int keep = side.written * (int64_t)sizeof(T) <= KEEP_BYTES;
     // the dispatch packet holds the grid size in work-items as 32 bits: keep
     // blocks × NTHREADS below 2^31 per launch and continue at tile0 = t0
     const int64_t chunk = ((int64_t)1 << 31) / NTHREADS;
@@ -559,11 +573,11 @@ void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const do
         const unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
         if (side.staged)
             hipLaunchKernelGGL((transform_kernel<T, L, true>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, t0);
+                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, keep, t0);
         else
             hipLaunchKernelGGL((transform_kernel<T, L, false>), dim3(grid), dim3(NTHREADS), 0, stream,
                                side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta,
-                               t0);
+                               keep, t0);
     }
 }
 
