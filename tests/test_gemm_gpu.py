@@ -162,3 +162,71 @@ def test_split_k_model_engages_split(lib):
             Cm = rng.integers(-4, 5, nc).astype(np.float64)
             np.testing.assert_array_equal(_run(lib, tab, A, B, Cm, 0, 1.0, -1.0, 1005),
                                           _ref(tab, A, B, Cm, 1.0, -1.0))
+
+
+def test_split_k_repeated_calls_and_graph_replays(lib):
+    """A split-K plan reused across calls (β = 0 and β ≠ 0, every precision form) and
+    replayed from a captured graph gives the full result every time: FP64 with integer
+    data is exact, and every form is bitwise repeatable (the partial products are
+    summed in a fixed order)."""
+    rng = np.random.default_rng(23)
+    shapes = [(64, 64, 4096), (200, 150, 3000), (130, 33, 2049)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0], [1, 0, 0])
+    arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+    cols = [lib.i64(arr[:, i]) for i in range(11)]
+    ta, tb = lib.i32(arr[:, 5]), lib.i32(arr[:, 8])
+    L = lib.load()
+    h = C.c_void_p()
+    lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                      cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1], cols[10][1]),
+              "plan")
+    try:
+        sk, slots = C.c_int64(), C.c_int64()
+        lib.check(L.tkmi_gemm_plan_splitk(h, 5, C.byref(sk), C.byref(slots)), "splitk")
+        assert sk.value > 1
+        A = rng.integers(-4, 5, na).astype(np.float64)
+        B = rng.integers(-4, 5, nb).astype(np.float64)
+        C0 = rng.integers(-4, 5, nc).astype(np.float64)
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        dC = torch.from_numpy(C0).cuda()
+        for prec in (0, 1, 2):
+            lib.check(L.tkmi_gemm_plan_set_tile(h, prec, 1005), "tile")
+        s = torch.cuda.current_stream().cuda_stream
+        exact = {}
+        for prec in (0, 1, 2):
+            for rep, beta in enumerate((0.0, -1.0, 0.0, 2.0)):
+                dC.copy_(torch.from_numpy(C0))
+                lib.check(L.tkmi_gemm_exec(h, prec, 1.0, beta, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s), "exec")
+                out = dC.cpu().numpy()
+                ref = _ref(tab, A, B, C0, 1.0, beta)
+                if prec == 0:
+                    np.testing.assert_array_equal(out, ref)
+                else:
+                    assert np.linalg.norm(out - ref) <= 1e-5 * np.linalg.norm(ref), (prec, rep)
+                if (prec, beta) in exact:
+                    assert np.array_equal(out, exact[(prec, beta)]), (prec, rep)
+                exact[(prec, beta)] = out
+        # graph replays of one FP64 call (β = 0), with eager work between the replays
+        dC.zero_()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        cs = torch.cuda.Stream()
+        cs.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(cs), torch.cuda.graph(g, stream=cs):
+            lib.check(L.tkmi_gemm_exec(h, 0, 1.0, 0.0, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(),
+                                       cs.cuda_stream), "exec (capture)")
+        torch.cuda.current_stream().wait_stream(cs)
+        ref = _ref(tab, A, B, C0, 1.0, 0.0)
+        dC0 = torch.from_numpy(C0).cuda()
+        for _ in range(4):
+            dC.copy_(dC0)  # β = 0 overwrites every group's block; the padding keeps C0
+            g.replay()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(dC.cpu().numpy(), ref)
+            dC.copy_(dC0)
+            lib.check(L.tkmi_gemm_exec(h, 0, 1.0, 0.0, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s), "exec")
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(dC.cpu().numpy(), ref)
+    finally:
+        torch.cuda.synchronize()
+        L.tkmi_gemm_plan_destroy(h)
