@@ -289,6 +289,12 @@ def main():
         lib.tkmi_gemm_plan_query(op[1].handle, _lib.PREC_F64, C.byref(c), None)
         cfgs.append(c.value)
     log("FP64 tile configurations per grouped GEMM: " + ",".join(map(str, cfgs)))
+    cfgs_bf = []
+    for op in gemm_ops:
+        c = C.c_int32()
+        lib.tkmi_gemm_plan_query(op[1].handle, _lib.PREC_BF16X3, C.byref(c), None)
+        cfgs_bf.append(c.value)
+    log("3xBF16 tile configurations per grouped GEMM: " + ",".join(map(str, cfgs_bf)))
     y_eager = H.apply()
     torch.cuda.synchronize()
     if args.counter_workload:
