@@ -216,8 +216,10 @@ def cpu_replay(steps, init, nthreads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # a step is ≈0.3 ms: 400 timed steps after 40 warm-up replays measure the sustained
+    # rate (50 after 5 read ≈4% slower, the clocks still ramping)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -401,7 +403,7 @@ def main():
             torch.cuda.empty_cache()
 
     # per-kernel breakdown and roofline of the dominant kernel
-    gemms, perms = kernel_breakdown(trace, T, _lib, torch)
+    gemms, perms = kernel_breakdown(trace, T, _lib, torch, reps=100)
     by_kernel = {}
     for g in gemms:
         d = by_kernel.setdefault(g["kernel"], {"flops": 0.0, "ms": 0.0, "calls": 0, "launches": []})

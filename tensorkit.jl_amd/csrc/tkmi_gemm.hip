@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -254,7 +255,17 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     constexpr int KV = 16 / (int)sizeof(TC), KC = 4 * KV;
     static_assert(BK % KC == 0, "K slab must hold whole fragment rounds");
     const int kq = KV * (lane >> 4);
-    auto mma_slab = [&](int buf) {
+    // Edge tiles skip the 16×16 fragments that lie wholly outside C (zero padding), so the
+    // padding of a ragged tile costs whole 16-row/column fragments rather than whole
+    // tiles; the test is uniform over the wave, and interior tiles run without it.
+    bool frag_ok[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            frag_ok[i][j] = row0 + wm * WM + i * 16 < m && col0 + wn * WN + j * 16 < n;
+    auto mma_slab = [&](int buf, auto edge) {
+        constexpr bool EDGE = decltype(edge)::value;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += KC) {
             typedef TC vk_t __attribute__((ext_vector_type(KV)));
@@ -270,9 +281,11 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i][s], bfr[j][s], acc[i][j]);
+                    for (int j = 0; j < FN; ++j)
+                        if (!EDGE || frag_ok[i][j]) acc[i][j] = MfmaK4<TC>::mma(af[i][s], bfr[j][s], acc[i][j]);
         }
     };
+    auto run = [&](auto edge) {
     if constexpr (NBUF == 2) {
         if (nk > 0) {
             load_slab(kbeg);
@@ -282,7 +295,7 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
         for (int64_t kt = 0; kt < nk; ++kt) {
             const int buf = (int)(kt & 1);
             if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);
-            mma_slab(buf);
+            mma_slab(buf, edge);
             if (kt + 1 < nk) store_slab(buf ^ 1);
             __syncthreads();
         }
@@ -293,7 +306,7 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
             store_slab(0);
             __syncthreads();
             if (kt + 1 < nk) load_slab(kbeg + (kt + 1) * BK);  // lands while the MFMAs run
-            mma_slab(0);
+            mma_slab(0, edge);
         }
     } else {
         // one LDS buffer, two register stages: loads run two slabs ahead of the MFMAs
@@ -305,16 +318,23 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
             store_from(0, ra, rb);
             __syncthreads();
             if (kt + 2 < nk) load_into(ra, rb, kbeg + (kt + 2) * BK);
-            mma_slab(0);
+            mma_slab(0, edge);
             if (kt + 1 < nk) {
                 __syncthreads();
                 store_from(0, ra1, rb1);
                 __syncthreads();
                 if (kt + 3 < nk) load_into(ra1, rb1, kbeg + (kt + 3) * BK);
-                mma_slab(0);
+                mma_slab(0, edge);
             }
         }
     }
+    };
+    // FP64: interior tiles run a copy of the loop without the fragment test; FP32 keeps
+    // one loop with the test (a second copy costs its kernels a wave per SIMD)
+    if (sizeof(TC) == 4 || !tile_full)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 
     // epilogue: C = α·acc + β·C, or this split's raw partial product into the workspace
     const bool partial = G.splits > 1;
@@ -546,6 +566,15 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
         }
     };
 
+    // the 16×16 fragments wholly outside C (zero padding of edge tiles) are skipped; the
+    // test is uniform over the wave
+    bool frag_ok[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            frag_ok[i][j] = row0 + wm * WM + i * 16 < m && col0 + wn * WN + j * 16 < n;
+
     const int64_t nk = (K - kbeg + BK - 1) / BK;
 #pragma unroll
     for (int st = 0; st < STAGES; ++st)
@@ -576,6 +605,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
+                if (!frag_ok[i][j]) continue;
                 f4_t h = f4_t{0, 0, 0, 0};
                 h = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], h, 0, 0, 0);
                 f4_t l = lo[i][j];
