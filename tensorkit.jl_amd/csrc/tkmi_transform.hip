@@ -4,8 +4,10 @@
 //   * dims of size 1 are dropped, dims that stay contiguous on both sides are merged;
 //   * dW = the dim with the smallest stride on the written side,
 //     dR = the dim with the smallest stride on the read side;
-//   * dW == dR  -> "copy" mode: a tile is a run of whole rows along dW, numbered
-//                  element after element, lanes consecutive on both sides;
+//   * dW == dR  -> "copy" mode: the (dW, dR) plane of an item is numbered element
+//                  after element along dW (contiguous on both sides) and a tile is the
+//                  next `area` elements of that numbering, across row ends, so every
+//                  tile but an item's last is full; lanes consecutive on both sides;
 //     dW != dR  -> "transpose" mode: a (tW × tR) tile (powers of two) is read with
 //                  lanes along dR, staged in LDS, and written with lanes along dW.
 // Work items are split into classes by the number of trees they touch (L = 1, 2, 4,
@@ -52,6 +54,7 @@ struct TBlock {
     int32_t lgW, lgR;
     int32_t nother;
     int32_t u_ld;  // leading dimension of the (column-major) coefficient matrix
+    double inv_w;  // copy mode: 1 / size[dW], for the row of a tile's first element
     int32_t odims[MAXD];
     int64_t size[MAXD];
     int64_t ntW, ntR, nbatch;
@@ -91,9 +94,29 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     t /= B.ntW;
     const int64_t iR = t % B.ntR;
     const int64_t bt = t / B.ntR;
-    const int64_t w0 = iW * tW, r0 = iR * tR;
-    const int nw = (int)min((int64_t)tW, B.size[dW] - w0);
-    const int nr = (int)min((int64_t)tR, B.size[dR] - r0);
+    int64_t w0, r0;
+    int nw, nr;
+    // copy mode: tile iR holds elements [iR·area, iR·area + cnt) of the item's plane
+    // numbered along dW, starting at row r0, column w0 (r0 from the reciprocal, then
+    // corrected); element e lies ws + e columns on, q = ⌊(w0 + e) / W⌋ rows down
+    const int64_t W = B.size[dW];
+    int cnt = 0;
+    if (COPY) {
+        constexpr int AREA = class_area(L);
+        const int64_t f0 = iR * AREA;
+        int64_t rs = (int64_t)((double)f0 * B.inv_w);
+        if (rs * W > f0) --rs;
+        else if ((rs + 1) * W <= f0) ++rs;
+        r0 = rs;
+        w0 = f0 - rs * W;
+        cnt = (int)min((int64_t)AREA, W * B.size[dR] - f0);
+        nw = nr = 0;
+    } else {
+        w0 = iW * tW;
+        r0 = iR * tR;
+        nw = (int)min((int64_t)tW, B.size[dW] - w0);
+        nr = (int)min((int64_t)tR, B.size[dR] - r0);
+    }
     const int tid = threadIdx.x;
 
     // offset of the tile corner in one tree (the batch index bt is decomposed
@@ -119,15 +142,29 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     // have tW, tR ≤ NTHREADS, so on the read side r is fixed per thread and w steps
     // by NTHREADS/tR with k (on the write side w is fixed and r steps): addresses
     // advance by a uniform stride and only the loaded values stay live.
-    const float inv = 1.0f / (float)nw;
+    // Copy mode returns w = e and r = q: relative to the tile corner the element sits
+    // at e·sW + q·(sR − W·sW) (the copy stride pair below), which keeps both small.
+    // Rows shorter than the tile: q from a float reciprocal (w0 + e < 2·area, exact
+    // after the ±1 correction); longer rows hold a tile in at most two rows.
+    const bool long_rows = W >= class_area(L);
+    const int ws = long_rows ? 0 : (int)w0;
+    const int wshort = long_rows ? 1 : (int)W;
+    const float inv = 1.0f / (float)(COPY ? wshort : nw);
     auto pos_read = [&](int k, int &w, int &r) {
         if (COPY) {
             const int e = tid + k * NTHREADS;
-            r = (int)((float)e * inv);
-            w = e - r * nw;
-            if (w < 0) { --r; w += nw; }
-            else if (w >= nw) { ++r; w -= nw; }
-            return e < nw * nr;
+            if (long_rows) {
+                r = (int64_t)e >= W - w0;
+            } else {
+                const int c = ws + e;
+                int q = (int)((float)c * inv);
+                const int rem = c - q * wshort;
+                if (rem < 0) --q;
+                else if (rem >= wshort) ++q;
+                r = q;
+            }
+            w = e;
+            return e < cnt;
         }
         r = tid & (tR - 1);
         w = (tid >> B.lgR) + k * (NTHREADS >> B.lgR);
@@ -155,7 +192,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
             if (i < nc) {
                 const TTree &tr = trees[B.read_tree + ic + i];
                 const T *sp = src + corner(tr);
-                const int64_t sW = tr.str[dW], sR = tr.str[dR];
+                const int64_t sW = tr.str[dW], sR = COPY ? tr.str[dR] - W * tr.str[dW] : tr.str[dR];
 #pragma unroll
                 for (int k = 0; k < E; ++k) {
                     int w, r;
@@ -209,7 +246,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
             if (j < nwrite) {
                 const TTree &tw = trees[B.write_tree + j];
                 T *dp = dst + corner(tw);
-                const int64_t sW = tw.str[dW], sR = tw.str[dR];
+                const int64_t sW = tw.str[dW], sR = COPY ? tw.str[dR] - W * tw.str[dW] : tw.str[dR];
 #pragma unroll
                 for (int k = 0; k < E; ++k) {
                     int w, r;
@@ -373,15 +410,12 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
         }
         if (second < 0) second = (dW == 0) ? 1 : 0;
         dR = second;
-        // tiles of whole runs: up to `area` elements; runs longer than half of it are split
-        if (b.size[dW] >= area / 2) {
-            b.tW = area;
-            b.tR = 1;
-        } else {
-            b.tW = (int)b.size[dW];
-            b.tR = (int)std::max<int64_t>(1, std::min<int64_t>(b.size[dR], area / b.size[dW]));
-        }
+        // tiles of `area` consecutive elements of the plane numbered along dW (the kernel
+        // places them from the flat index): one tile column, ntR tiles per batch index
+        b.tW = (int)std::min<int64_t>(b.size[dW], INT32_MAX);
+        b.tR = 1;
         b.lgW = b.lgR = 0;
+        b.inv_w = 1.0 / (double)b.size[dW];
     } else {
         b.mode = 1;
         choose_tile(b.size[dW], b.size[dR], area, twmax, b.tW, b.tR);
@@ -390,8 +424,13 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
     }
     b.dW = dW;
     b.dR = dR;
-    b.ntW = (b.size[dW] + b.tW - 1) / b.tW;
-    b.ntR = (b.size[dR] + b.tR - 1) / b.tR;
+    if (b.mode == 0) {
+        b.ntW = 1;
+        b.ntR = (b.size[dW] * b.size[dR] + area - 1) / area;
+    } else {
+        b.ntW = (b.size[dW] + b.tW - 1) / b.tW;
+        b.ntR = (b.size[dR] + b.tR - 1) / b.tR;
+    }
     b.nother = 0;
     b.nbatch = 1;
     for (int d = 0; d < b.ndim; ++d) {
