@@ -42,8 +42,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
 # Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
 TRAFFIC_PMC = {
-    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): (87318 * 2 + 67287) * 1024,  # L·x
-    ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): (106296 * 2 + 56895) * 1024,  # ·R, split-K 5 partials
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): round((80849.4 * 2 + 68539.3) * 1024),  # L·x
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): round((96597.6 * 2 + 57122.2) * 1024),  # ·R, split-K 5 partials
 }
 
 
@@ -103,10 +103,21 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
                     continue
                 if part == 1 and nt.value == 0:
                     continue
-                fn = lambda plan=plan, prec=prec, part=part, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
-                    plan.handle, prec, part, a, b, pa, pb, pc, s)
+                # the MFMA launch alone (part 1 | 4: without its split-K reduction, the
+                # kernel a trace reports under this name); the reduction timed on its own
+                bits = 5 if part == 1 else part
+                fn = lambda plan=plan, prec=prec, bits=bits, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
+                    plan.handle, prec, bits, a, b, pa, pb, pc, s)
                 ms = time_launches(fn, reps, torch)
-                gemms.append({"flops": flops, "ms": ms, "kernel": name, "tiles": nt.value if part == 1 else None,
+                red_ms = None
+                if part == 1 and sk.value > 1:
+                    fr = lambda plan=plan, prec=prec, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
+                        plan.handle, prec, 8, a, b, pa, pb, pc, s)
+                    red_ms = time_launches(fr, reps, torch)
+                    # leave C as a full call writes it
+                    L.tkmi_gemm_exec_parts(plan.handle, prec, 1, a, b, pa, pb, pc, s)
+                gemms.append({"flops": flops, "ms": ms, "reduce_ms": red_ms, "kernel": name,
+                              "tiles": nt.value if part == 1 else None,
                               "splitk": sk.value if part == 1 else None, "slots": slots.value if part == 1 else None,
                               "grid_x": nt.value * 256 if part == 1 else None,
                               "alg_bytes": alg_bytes if part == 1 else None})
@@ -425,7 +436,9 @@ def main():
     log("gemm launches: " + "; ".join(f"{g['kernel']} {g['flops']/1e9:.3f} GF {g['ms']*1e3:.1f} us "
                                        f"{g['flops']/g['ms']/1e9:.2f} TF/s"
                                        + (f" ({g['tiles']} tiles, split-K {g['splitk']}, {g['slots']} slots)"
-                                          if g.get('tiles') else "") for g in gemms))
+                                          if g.get('tiles') else "")
+                                       + (f" + splitk_reduce {g['reduce_ms']*1e3:.1f} us" if g.get('reduce_ms') else "")
+                                       for g in gemms))
     log("permutes: " + "; ".join(f"{p['elements']} el {p['ms']*1e3:.1f} us {p['bytes']/p['ms']/1e6:.0f} GB/s"
                                  for p in perms))
 
@@ -459,7 +472,9 @@ def main():
                      "traffic": traffic, "traffic_unit": "B/launch",
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "launch_grids": [g["grid_x"] for g in dom["launches"]], "calls_per_step": dom["calls"],
-                     "avg_launch_us": round(dom["ms"] / dom["calls"] * 1e3, 2)},
+                     "avg_launch_us": round(dom["ms"] / dom["calls"] * 1e3, 2),
+                     "splitk_reduce_us": [round(g["reduce_ms"] * 1e3, 2) if g.get("reduce_ms") else None
+                                          for g in dom["launches"]]},
         "graph_vs_eager_maxabs": eager_err,
         "svd_trunc": svd_report,
         "permute_su2_full": su2_report,

@@ -157,9 +157,11 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
  * stage, gemm_k4s_kernel; 3: one LDS buffer and two register stages, gemm_k4p_kernel). */
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
- * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64), bit 1 = the
- * bandwidth-bound skinny groups; parts = 3 is tkmi_gemm_exec.  Used to time the
- * two kernels separately.                                                     */
+ * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64) with their split-K
+ * reduction, bit 1 = the bandwidth-bound skinny groups; parts = 3 is tkmi_gemm_exec.
+ * For timing the kernels one by one: bit 2 (with bit 0) leaves out the split-K
+ * reduction launch, bit 3 (without bit 0) runs that reduction alone; either leaves
+ * the split groups of C without their full sums.                              */
 int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alpha,
                          double beta, const double *A, const double *B, double *C, void *stream);
 /* Algorithmic flops routed to the MFMA kernel and to the skinny kernel.       */

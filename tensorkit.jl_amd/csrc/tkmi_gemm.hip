@@ -927,10 +927,11 @@ int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? 
 
 template <int BM, int BN, int BK, int NBUF>
 void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const double *A, const double *B, double *C,
-              double alpha, double beta, hipStream_t s) {
+              double alpha, double beta, hipStream_t s, bool tiles, bool reduce) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
-    if (prec != TKMI_PREC_BF16X3) {
+    if (!tiles) {
+    } else if (prec != TKMI_PREC_BF16X3) {
         if constexpr (NBUF == 3) {
             if (prec == TKMI_PREC_F64)
                 hipLaunchKernelGGL((gemm_k4p_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
@@ -964,7 +965,7 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
             hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 1>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
                                (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
     }
-    if (ts.relems > 0) {
+    if (reduce && ts.relems > 0) {
         const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
         hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
                            (int64_t)ts.rgroups.size(), ts.relems, ts.ws, C, alpha, beta);
@@ -972,7 +973,7 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
 }
 
 int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const double *B, double *C,
-               double alpha, double beta, hipStream_t s) {
+               double alpha, double beta, hipStream_t s, bool tiles = true, bool reduce = true) {
     const TileSet &ts = p->ts[c];
     if (ts.ntiles == 0) return TKMI_OK;
     if (prec == TKMI_PREC_BF16X3 && !p->panels) {
@@ -988,15 +989,15 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const doub
     if (ts.ntiles * NT >= ((int64_t)1 << 31))
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more output tiles than one launch can hold (2^31 work-items)");
     switch (c) {
-        case 0: launch_k<64, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 1: launch_k<128, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 2: launch_k<128, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 3: launch_k<64, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 4: launch_k<64, 64, 32, 2>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 5: launch_k<64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 6: launch_k<64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        case 7: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s); break;
-        default: launch_k<64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s); break;
+        case 0: launch_k<64, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 1: launch_k<128, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 2: launch_k<128, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 3: launch_k<64, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 4: launch_k<64, 64, 32, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 5: launch_k<64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 6: launch_k<64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 7: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        default: launch_k<64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
@@ -1198,7 +1199,11 @@ int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, 
         TKMI_HIP(hipGetLastError());
         return TKMI_OK;
     }
-    if (parts & 1) rc = launch_cfg(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s);
+    // bit 2: the MFMA launch without its split-K reduction; bit 3: that reduction alone
+    // (timing only: either leaves the split groups of C without their sums)
+    const bool tiles = (parts & 1) != 0, reduce = (parts & 1) ? !(parts & 4) : (parts & 8) != 0;
+    if (tiles || reduce)
+        rc = launch_cfg(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s, tiles, reduce);
     if (rc) return rc;
     if (parts & 2) rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
     return rc;
