@@ -370,6 +370,12 @@ def transpose_(tdst, tsrc, p1, p2, alpha=1.0, beta=0.0):
     return permute_(tdst, tsrc, p1, p2, alpha, beta)
 
 
+def planaradd_(C, A, p, alpha=1.0, beta=0.0):
+    """`planaradd!(C, A, (p₁, p₂), α, β)` (`src/planar/planaroperations.jl:29-36`): the
+    planar transpose `transpose!(C, A, p, α, β)`."""
+    return transpose_(C, A, p[0], p[1], alpha, beta)
+
+
 def planarcontract_(C, A, pA, B, pB, pAB, alpha=1.0, beta=0.0, precision=None):
     """`planarcontract!` (`src/planar/planaroperations.jl:150-190`): bosonic
     sectors forward to `contract!`."""

@@ -115,3 +115,23 @@ def test_deprecated_add_wrappers(T):
         T.add_transpose_(Ct, A, pt, 1.0, 0.0)
     assert any(issubclass(x.category, DeprecationWarning) for x in w)
     assert np.array_equal(Ct.numpy(), reft.numpy())
+
+
+@pytest.mark.parametrize("kind", ["u1", "su2"])
+def test_planaradd_is_transpose(T, kind):
+    """planaradd! (src/planar/planaroperations.jl:29-36) is transpose!: same result for a
+    cyclic transpose, α and β included; a non-cyclic index order is rejected."""
+    rng = np.random.default_rng(35)
+    if kind == "u1":
+        V = U1Space((0, 2), (1, 1), (-1, 1))
+    else:
+        V = SU2Space((0, 1), (0.5, 1), (1, 1))
+    A = T.TensorMap.randn(HomSpace(ProductSpace(V, V), ProductSpace(V, V)), rng)
+    p = ((1, 3), (0, 2))  # the cyclic order (0 1 | 3 2) rotated by one
+    C0 = T.TensorMap.randn(A.space.permute(*p), rng)
+    ref, C = C0.copy(), C0.copy()
+    T.transpose_(ref, A, p[0], p[1], 0.5, -1.0)
+    T.planaradd_(C, A, p, 0.5, -1.0)
+    assert np.array_equal(C.numpy(), ref.numpy())
+    with pytest.raises(Exception):
+        T.planaradd_(T.TensorMap.empty(A.space.permute((0, 2), (1, 3))), A, ((0, 2), (1, 3)))
