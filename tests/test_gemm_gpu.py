@@ -230,3 +230,41 @@ def test_split_k_repeated_calls_and_graph_replays(lib):
     finally:
         torch.cuda.synchronize()
         L.tkmi_gemm_plan_destroy(h)
+
+
+def test_exec_parts_split_reduction_separately(lib):
+    """tkmi_gemm_exec_parts timing bits: the MFMA launch without its split-K reduction
+    (bits 0 | 2) followed by the reduction alone (bit 3) and the skinny part (bit 1)
+    gives exactly what tkmi_gemm_exec gives (the same kernels in the same order)."""
+    rng = np.random.default_rng(24)
+    shapes = [(64, 64, 4096), (200, 150, 3000), (130, 33, 2049), (300, 8, 20), (40, 70, 300)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0, 0, 1], [1, 0, 0, 1, 0])
+    arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+    cols = [lib.i64(arr[:, i]) for i in range(11)]
+    ta, tb = lib.i32(arr[:, 5]), lib.i32(arr[:, 8])
+    L = lib.load()
+    h = C.c_void_p()
+    lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                      cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1], cols[10][1]),
+              "plan")
+    try:
+        A, B, C0 = rng.standard_normal(na), rng.standard_normal(nb), rng.standard_normal(nc)
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        s = torch.cuda.current_stream().cuda_stream
+        for prec in (0, 1, 2):
+            lib.check(L.tkmi_gemm_plan_set_tile(h, prec, 1005), "tile")
+            sk, slots = C.c_int64(), C.c_int64()
+            lib.check(L.tkmi_gemm_plan_splitk(h, 5, C.byref(sk), C.byref(slots)), "splitk")
+            assert sk.value > 1
+            for beta in (0.0, -0.5):
+                full = torch.from_numpy(C0).cuda()
+                lib.check(L.tkmi_gemm_exec(h, prec, 1.25, beta, dA.data_ptr(), dB.data_ptr(), full.data_ptr(), s),
+                          "exec")
+                parts = torch.from_numpy(C0).cuda()
+                for bits in (5, 8, 2):
+                    lib.check(L.tkmi_gemm_exec_parts(h, prec, bits, 1.25, beta, dA.data_ptr(), dB.data_ptr(),
+                                                     parts.data_ptr(), s), f"parts {bits}")
+                assert torch.equal(parts, full), (prec, beta)
+    finally:
+        torch.cuda.synchronize()
+        L.tkmi_gemm_plan_destroy(h)
