@@ -4,10 +4,11 @@
 //   * dims of size 1 are dropped, dims that stay contiguous on both sides are merged;
 //   * dW = the dim with the smallest stride on the written side,
 //     dR = the dim with the smallest stride on the read side;
-//   * dW == dR  -> "copy" mode: the (dW, dR) plane of an item is numbered element
-//                  after element along dW (contiguous on both sides) and a tile is the
-//                  next `area` elements of that numbering, across row ends, so every
-//                  tile but an item's last is full; lanes consecutive on both sides;
+//   * dW == dR  -> "copy" mode: the plane of dW and the dim with the next smallest
+//                  written stride (stored as dR) is numbered element after element
+//                  along dW (contiguous on both sides) and a tile is the next `area`
+//                  elements of that numbering, across row ends, so every tile but an
+//                  item's last is full; lanes consecutive on both sides;
 //     dW != dR  -> "transpose" mode: a (tW × tR) tile (powers of two) is read with
 //                  lanes along dR, staged in LDS, and written with lanes along dW.
 // Work items are split into classes by the number of trees they touch (L = 1, 2, 4,
@@ -98,7 +99,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     int nw, nr;
     // copy mode: tile iR holds elements [iR·area, iR·area + cnt) of the item's plane
     // numbered along dW, starting at row r0, column w0 (r0 from the reciprocal, then
-    // corrected); element e lies ws + e columns on, q = ⌊(w0 + e) / W⌋ rows down
+    // corrected); element e of the tile is e places further along that numbering
     const int64_t W = B.size[dW];
     int cnt = 0;
     if (COPY) {
