@@ -257,7 +257,7 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     const int kq = KV * (lane >> 4);
     // Edge tiles skip the 16×16 fragments that lie wholly outside C (zero padding), so the
     // padding of a ragged tile costs whole 16-row/column fragments rather than whole
-    // tiles; the test is uniform over the wave, and interior tiles run without it.
+    // tiles; the test is uniform over the wave (FP64 interior tiles run without it).
     bool frag_ok[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
