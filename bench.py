@@ -12,7 +12,10 @@ Reported:
   roofline     dominant kernel (FP64 grouped GEMM) achieved TFLOP/s vs 78.6 peak,
                from HIP events around each of its launches on the launch stream
   cpu_baseline the oracle's CPU restatement (table-driven C permute loop + numpy/
-               OpenBLAS dgemm per coupled sector) replaying the same kernel sequence
+               OpenBLAS dgemm per coupled sector) replaying the same kernel sequence;
+               its su2_permute entry times the oracle's generic permute loop on the
+               SU(2) rank-4 permute with degeneracies ÷4, beside the device transform
+               on the same tensor
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -524,6 +527,47 @@ def main():
                                                     "cores": 1, "ms_per_step": round(ms1, 1), "reps": reps1,
                                                     "gpu_speedup": round(ms1 / ms64, 1)}
             log(f"cpu baseline, one thread: {ms1:.1f} ms/step ({reps1} applications), GPU speed-up x{ms1/ms64:.0f}")
+        # the non-abelian permute on the host: the oracle's table-driven generic loop
+        # (extract → ·Uᵀ → insert per fusion block, OpenMP over blocks) on the SU(2) rank-4
+        # permute ((3,1),(2,0)) with every degeneracy ÷4 (the same 269 fusion blocks,
+        # 40.5 M elements), and the device transform on the same tensor
+        from tkmi.spaces import HomSpace as _HS, ProductSpace as _PS, SU2Space
+        from tkmi.treetransformers import generic_table
+        from oracle import kernels as OK
+        Vq = SU2Space((0, 16), (0.5, 24), (1, 24), (1.5, 16), (2, 8))
+        Wq = _HS(_PS(Vq, Vq), _PS(Vq, Vq))
+        q1, q2 = (3, 1), (2, 0)
+        Wqd = Wq.permute(q1, q2)
+        nq = Wq.dim()
+        xq = np.random.default_rng(3).standard_normal(nq)
+        yq = np.empty(Wqd.dim())
+        rows_q = generic_table(Wqd, Wq, q1, q2)
+        OK.generic_permute(rows_q, q1 + q2, xq, yq, nthreads=nthreads)
+        tq = {}
+        for nt, budget in ((nthreads, 4.0), (1, 3.0)):
+            reps_q, t_q = 0, 0.0
+            while t_q < budget and reps_q < 20:
+                t0 = time.perf_counter()
+                OK.generic_permute(rows_q, q1 + q2, xq, yq, nthreads=nt)
+                t_q += time.perf_counter() - t0
+                reps_q += 1
+            tq[nt] = (t_q / reps_q * 1e3, reps_q)
+        Aq = T.TensorMap(torch.from_numpy(xq).cuda(), Wq)
+        Bq = T.TensorMap.empty(Wqd)
+        T.permute_(Bq, Aq, q1, q2)
+        torch.cuda.synchronize()
+        ms_qg = time_launches(lambda: T.permute_(Bq, Aq, q1, q2), 20, torch)
+        diff_q = float(np.max(np.abs(Bq.numpy() - yq)) / np.max(np.abs(yq)))
+        ms_qc, reps_qc = tq[nthreads]
+        out["cpu_baseline"]["su2_permute"] = {
+            "value": round(16 * nq / ms_qc / 1e6, 2), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "sample": f"SU(2) rank-4 permute ((3,1),(2,0)) with degeneracies ÷4: {nq} elements, 269 fusion "
+                      f"blocks, {reps_qc} runs of the oracle's generic loop (OpenMP over blocks)",
+            "ms": round(ms_qc, 2), "single_thread_ms": round(tq[1][0], 2),
+            "gpu_ms_same_sample": round(ms_qg, 4), "gpu_GBs_same_sample": round(16 * nq / ms_qg / 1e6, 1),
+            "gpu_speedup": round(ms_qc / ms_qg, 1), "max_rel_diff_vs_gpu": diff_q}
+        log(f"cpu SU2 permute (÷4 sample, {nq} el): {ms_qc:.1f} ms on {nthreads} threads, {tq[1][0]:.1f} ms on 1; "
+            f"GPU {ms_qg:.3f} ms; max rel diff {diff_q:.1e}")
     print(json.dumps(out), flush=True)
 
 
