@@ -143,6 +143,104 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
     return gemms, perms
 
 
+# The reference's published CPU timings of its U1 MPO-environment contraction
+# (benchmark/comparisons/results/tensorkit_mpo_u1.txt, medians of 100 runs; cases from
+# benchmark/comparisons/bonds.txt:16-20; one BLAS thread, hardware not stated).
+PUBLISHED_MPO_U1_MS = {(2560, 5, 3): 495.4, (6120, 5, 3): 5293.0}
+
+
+def distribute_u1(D, p=0.25):
+    """U1 degeneracies of the reference's benchmarks (`distributeU1_poisson`,
+    benchmark/comparisons/benchtools.jl:28-44): charge 0 holds ⌈p·D⌉ (made even-sized
+    remainder), charges ±n hold ⌈p·λⁿ/n!·D⌉ with λ = log((1/p + 1)/2) until D is used."""
+    import math
+    lam = math.log((1 / p + 1) / 2)
+    d0 = math.ceil(p * D)
+    if (D - d0) % 2:
+        d0 = 2 if d0 == 1 else d0 - 1
+    sectors, rem, n = [(0, d0)], D - d0, 1
+    while rem > 0:
+        dn = math.ceil(p * lam ** n / math.factorial(n) * D)
+        sectors += [(n, dn), (-n, dn)]
+        rem -= 2 * dn
+        n += 1
+    return sorted(sectors)
+
+
+def mpo_u1_case(T, torch, np, case, cpu_seconds):
+    """The reference's MPO-environment contraction, `@tensor C = FL[4,2,1] * A[1,3,6] *
+    M[2,5,3,7] * conj(A[4,5,8]) * FR[6,7,8]` (benchmark/comparisons/tensorkit_timers.jl:6-17),
+    contracted left to right as @tensor does, on random operands of the reference's spaces;
+    the chain is captured in one graph and replayed (plans cached, as the reference's
+    transformer caches are warm after its first run).  With cpu_seconds > 0 the oracle's
+    replay of the same kernel sequence is timed on one thread as well."""
+    from tkmi.spaces import HomSpace as _HS, ProductSpace as _PS, U1Space
+    Dmps, Dmpo, Dphys = case
+    Vmps, Vmpo, Vphys = (U1Space(*distribute_u1(d)) for d in case)
+    rng = np.random.default_rng(5)
+    A = T.TensorMap.randn(_HS(_PS(Vmps, Vphys, Vmps.dual()), _PS()), rng)
+    M = T.TensorMap.randn(_HS(_PS(Vmpo, Vphys, Vphys.dual(), Vmpo.dual()), _PS()), rng)
+    FL = T.TensorMap.randn(_HS(_PS(Vmps, Vmpo.dual(), Vmps.dual()), _PS()), rng)
+    FR = T.TensorMap.randn(_HS(_PS(Vmps, Vmpo, Vmps.dual()), _PS()), rng)
+    outs = [None] * 4
+
+    def chain():
+        outs[0] = T.contract_labels(FL, (4, 2, 1), A, (1, 3, 6), (4, 2, 3, 6), 4, out=outs[0])
+        outs[1] = T.contract_labels(outs[0], (4, 2, 3, 6), M, (2, 5, 3, 7), (4, 6, 5, 7), 4, out=outs[1])
+        outs[2] = T.contract_labels(outs[1], (4, 6, 5, 7), A, (4, 5, 8), (6, 7, 8), 3, conjB=True, out=outs[2])
+        outs[3] = T.contract_labels(outs[2], (6, 7, 8), FR, (6, 7, 8), (), 0, out=outs[3])
+
+    chain()
+    T.start_trace()
+    chain()
+    trace = T.stop_trace()
+    torch.cuda.synchronize()
+    flops = sum(op[1].flops for op in trace if op[0] in ("gemm", "fused"))
+    value = float(outs[3].numpy()[0])
+    g = torch.cuda.CUDAGraph()
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        with T.pinning() as pins, torch.cuda.graph(g, stream=st):
+            chain()
+    torch.cuda.current_stream().wait_stream(st)
+    for _ in range(5):
+        g.replay()
+    torch.cuda.synchronize()
+    replay_ok = float(outs[3].numpy()[0]) == value
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del pins
+    pub = PUBLISHED_MPO_U1_MS.get(tuple(case))
+    rep = {"case": {"Dmps": Dmps, "Dmpo": Dmpo, "Dphys": Dphys, "dims": [Vmps.dim(), Vmpo.dim(), Vphys.dim()]},
+           "ms": round(ms, 4), "gflop": round(flops / 1e9, 3), "GFLOP/s": round(flops / ms / 1e6, 1),
+           "kernels": [op[0] for op in trace], "graph_replay_matches_eager": replay_ok,
+           "published_reference_cpu_ms": pub,
+           "published_reference_cpu_note": "median of 100 runs, one BLAS thread, hardware not stated",
+           "speedup_vs_published": round(pub / ms, 1) if pub else None}
+    if cpu_seconds > 0:
+        init = {t.data.data_ptr(): t.numpy().copy() for t in (A, M, FL, FR)}
+        steps = prepare_cpu_replay(trace)
+        from threadpoolctl import threadpool_limits
+        n, t = 0, 0.0
+        with threadpool_limits(limits=1):
+            while t < cpu_seconds and n < 5:
+                t0 = time.perf_counter()
+                bufs = cpu_replay(steps, init, 1)
+                t += time.perf_counter() - t0
+                n += 1
+        cval = float(bufs[outs[3].data.data_ptr()][0])
+        rep["cpu_port_one_thread_ms"] = round(t / n * 1e3, 1)
+        rep["cpu_port_rel_diff"] = abs(cval - value) / max(abs(value), 1e-300)
+    return rep
+
+
 def prepare_cpu_replay(trace):
     """Host-side tables of the replay, built once outside the timed region (the reference
     caches its transformer tables; table construction is excluded on both sides): the
@@ -241,6 +339,8 @@ def main():
                     help="bound of the one-thread CPU sample (0: skip it)")
     ap.add_argument("--no-svd", action="store_true")
     ap.add_argument("--no-su2", action="store_true")
+    ap.add_argument("--no-mpo", action="store_true",
+                    help="skip the reference's own U1 MPO-environment contraction cases")
     # profiling modes for rocprofv3 --pmc passes (no JSON line): a plain device copy of
     # known size to calibrate FETCH_SIZE / WRITE_SIZE, and N eager applications of the
     # autotuned FP64 chain (the kernels and tile configurations the graph replays)
@@ -416,6 +516,18 @@ def main():
             del As, Bs
             torch.cuda.empty_cache()
 
+    # the reference's own U1 MPO-environment contraction, beside its published CPU timings
+    mpo_report = None
+    if not args.no_mpo:
+        mpo_report = []
+        for case in ((2560, 5, 3), (6120, 5, 3)):
+            r = mpo_u1_case(T, torch, np, case, 0.0 if args.no_cpu or case[0] > 3000 else 4.0)
+            mpo_report.append(r)
+            log(f"MPO U1 {case}: {r['ms']:.3f} ms ({r['GFLOP/s']:.0f} GFLOP/s) vs published reference CPU "
+                f"{r['published_reference_cpu_ms']} ms" + (f"; CPU port, one thread: {r['cpu_port_one_thread_ms']} ms"
+                                                            if "cpu_port_one_thread_ms" in r else ""))
+        torch.cuda.empty_cache()
+
     # per-kernel breakdown and roofline of the dominant kernel
     gemms, perms = kernel_breakdown(trace, T, _lib, torch, reps=100)
     by_kernel = {}
@@ -481,6 +593,7 @@ def main():
         "graph_vs_eager_maxabs": eager_err,
         "svd_trunc": svd_report,
         "permute_su2_full": su2_report,
+        "reference_mpo_u1": mpo_report,
     }
 
     if not args.no_cpu:

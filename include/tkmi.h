@@ -146,7 +146,11 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
  * TKMI_PREC_F32 (inputs rounded to FP32, FP32 accumulation) or TKMI_PREC_BF16X3 (each
  * input split into three bf16 terms, six products, FP32/FP64 accumulation).  BLAS
  * convention: β = 0 writes C without reading it; α = 0 reads neither A nor B
- * (C_g = β·C_g, zeros for β = 0), as mul! reduces to rmul!.                     */
+ * (C_g = β·C_g, zeros for β = 0), as mul! reduces to rmul!.  The bandwidth-bound
+ * groups (min(m, n) ≤ 32 with k ≤ 64, and 1×1 results with longer k, i.e. full
+ * contractions to a scalar) compute in FP64 for every precision form; a 1×1 result
+ * is a dot product of fixed summation order (partials over k chunks, then their sum),
+ * with partial sums in a plan-owned buffer.                                    */
 int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta,
                    const double *A, const double *B, double *C, void *stream);
 /* Tile shape for `precision`: 0 = automatic, 64 / 128 = square tiles with a
@@ -158,7 +162,8 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
  * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64) with their split-K
- * reduction, bit 1 = the bandwidth-bound skinny groups; parts = 3 is tkmi_gemm_exec.
+ * reduction, bit 1 = the bandwidth-bound groups (skinny ones, and 1×1 results with
+ * k > 64, which run as two-pass dot products); parts = 3 is tkmi_gemm_exec.
  * For timing the kernels one by one: bit 2 (with bit 0) leaves out the split-K
  * reduction launch, bit 3 (without bit 0) runs that reduction alone; either leaves
  * the split groups of C without their full sums.                              */

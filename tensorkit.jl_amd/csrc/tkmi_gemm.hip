@@ -718,6 +718,76 @@ __global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restric
     }
 }
 
+// --------------------------------------------------------------------------------
+// Dot groups (m = n = 1 with K > SKINNY_K: a contraction down to a scalar, whose only
+// coupled sector is the unit): HBM-bound reductions in two passes of fixed order, FP64
+// for every precision form.  Pass 1: one workgroup per K chunk of a group writes its
+// partial sum; pass 2: one workgroup per group adds its partials in order and applies
+// α and β.
+// --------------------------------------------------------------------------------
+struct DGroup {
+    int64_t k, a_off, a_inc, b_off, b_inc, c_off;  // op(A)(0, i) = A[a_off + i·a_inc], op(B)(i, 0) likewise
+    int64_t part_start, nparts, chunk;
+};
+
+__global__ __launch_bounds__(NT) void gemm_dot_partial(const DGroup *__restrict__ dg, int64_t ng,
+                                                       const double *__restrict__ A, const double *__restrict__ B,
+                                                       double *__restrict__ parts) {
+    __shared__ double sh[NT];
+    const int64_t part = (int64_t)blockIdx.x;
+    int64_t lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (dg[mid].part_start <= part) lo = mid; else hi = mid - 1;
+    }
+    const DGroup &D = dg[lo];
+    const int64_t k0 = (part - D.part_start) * D.chunk, k1 = min(D.k, k0 + D.chunk);
+    const double *a = A + D.a_off, *b = B + D.b_off;
+    const int64_t ai = D.a_inc, bi = D.b_inc;
+    // two independent sums per thread keep two load pairs in flight; the order of every
+    // addition is fixed by (k, chunk), so results are reproducible
+    double s0 = 0.0, s1 = 0.0;
+    int64_t i = k0 + threadIdx.x;
+    for (; i + NT < k1; i += 2 * NT) {
+        s0 += a[i * ai] * b[i * bi];
+        s1 += a[(i + NT) * ai] * b[(i + NT) * bi];
+    }
+    for (; i < k1; i += NT) s0 += a[i * ai] * b[i * bi];
+    sh[threadIdx.x] = s0 + s1;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[part] = sh[0];
+}
+
+__global__ __launch_bounds__(NT) void gemm_dot_final(const DGroup *__restrict__ dg, const double *__restrict__ parts,
+                                                     double *__restrict__ C, double alpha, double beta) {
+    __shared__ double sh[NT];
+    const DGroup &D = dg[blockIdx.x];
+    double s = 0.0;
+    for (int64_t j = threadIdx.x; j < D.nparts; j += NT) s += parts[D.part_start + j];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *c = C + D.c_off;
+        const double r = alpha * sh[0];
+        *c = beta != 0.0 ? r + beta * *c : r;
+    }
+}
+
+struct DotSet {
+    std::vector<DGroup> groups;
+    int64_t nparts = 0;
+    DGroup *d_groups = nullptr;
+    double *parts = nullptr;  // plan-owned partial sums
+};
+
 struct TileSet {
     std::vector<GGroup> groups;
     int64_t ntiles = 0;
@@ -759,6 +829,8 @@ struct tkmi_gemm_plan_s {
     PackGroup *d_pack = nullptr;
     __bf16 *panels = nullptr;
     SkinnySet skinny;
+    DotSet dots;
+    std::vector<GGroup> skinny_c;  // the dot groups as 1×1 C blocks (α = 0 calls scale them)
     double flops_mfma = 0, flops_skinny = 0;
     double flops = 0;
     TileSet ts[NCFG];
@@ -1020,11 +1092,24 @@ int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double 
     return TKMI_OK;
 }
 
+int launch_dots(const DotSet &ds, const double *A, const double *B, double *C, double alpha, double beta,
+                hipStream_t s) {
+    if (ds.groups.empty()) return TKMI_OK;
+    hipLaunchKernelGGL(gemm_dot_partial, dim3((unsigned)ds.nparts), dim3(NT), 0, s, ds.d_groups,
+                       (int64_t)ds.groups.size(), A, B, ds.parts);
+    hipLaunchKernelGGL(gemm_dot_final, dim3((unsigned)ds.groups.size()), dim3(NT), 0, s, ds.d_groups, ds.parts, C,
+                       alpha, beta);
+    TKMI_HIP(hipGetLastError());
+    return TKMI_OK;
+}
+
 void free_gemm(tkmi_gemm_plan_s *p) {
     if (!p) return;
     (void)hipFree(p->panels);
     (void)hipFree(p->d_pack);
     (void)hipFree(p->skinny.d_groups);
+    (void)hipFree(p->dots.d_groups);
+    (void)hipFree(p->dots.parts);
     for (auto *d : p->d_scale) (void)hipFree(d);
     for (auto &t : p->ts) {
         (void)hipFree(t.d_groups);
@@ -1056,8 +1141,10 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
     // skinny kernel keeps the plans made of skinny groups only (the MPO steps).
     auto skinny_shape = [](int64_t m, int64_t n, int64_t k) { return std::min(m, n) <= SKINNY && k <= SKINNY_K; };
     bool any_mfma = false;
+    auto dot_shape = [](int64_t m, int64_t n, int64_t k) { return m == 1 && n == 1 && k > SKINNY_K; };
     for (int64_t g = 0; g < ngroups; ++g)
-        if (m[g] > 0 && n[g] > 0 && k[g] > 0 && !skinny_shape(m[g], n[g], k[g])) any_mfma = true;
+        if (m[g] > 0 && n[g] > 0 && k[g] > 0 && !skinny_shape(m[g], n[g], k[g]) && !dot_shape(m[g], n[g], k[g]))
+            any_mfma = true;
     for (int64_t g = 0; g < ngroups; ++g) {
         GGroup G;
         std::memset(&G, 0, sizeof(G));
@@ -1078,7 +1165,7 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         }
         const bool few_tiles = ((G.m + 63) / 64) * ((G.n + 63) / 64) <= 4;
         const bool to_skinny = skinny_shape(G.m, G.n, G.k) && !(any_mfma && few_tiles);
-        if (G.k > 0 && G.m > 0 && G.n > 0 && !to_skinny &&
+        if (G.k > 0 && G.m > 0 && G.n > 0 && !to_skinny && !dot_shape(G.m, G.n, G.k) &&
             (G.lda * (G.ta ? 128 : 32) >= ((int64_t)1 << 29) || G.ldb * (G.tb ? 32 : 128) >= ((int64_t)1 << 29))) {
             // the MFMA kernels address a slab with 32-bit byte offsets from its base
             free_gemm(p);
@@ -1094,6 +1181,26 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         const double f = 2.0 * (double)G.m * (double)G.n * (double)G.k;
         p->flops += f;
         if (G.m == 0 || G.n == 0) continue;
+        if (dot_shape(G.m, G.n, G.k)) {
+            // a dot product: op(A) is 1×k, op(B) is k×1
+            constexpr int64_t DOT_CHUNK = 8192, DOT_MAXPARTS = 1024;
+            DGroup D;
+            std::memset(&D, 0, sizeof(D));
+            D.k = G.k;
+            D.a_off = G.a_off;
+            D.a_inc = G.ta ? 1 : G.lda;
+            D.b_off = G.b_off;
+            D.b_inc = G.tb ? G.ldb : 1;
+            D.c_off = G.c_off;
+            D.nparts = std::min<int64_t>(DOT_MAXPARTS, (G.k + DOT_CHUNK - 1) / DOT_CHUNK);
+            D.chunk = (G.k + D.nparts - 1) / D.nparts;
+            D.part_start = p->dots.nparts;
+            p->dots.nparts += D.nparts;
+            p->dots.groups.push_back(D);
+            p->flops_skinny += f;
+            p->skinny_c.push_back(G);  // its C element, for α = 0 calls
+            continue;
+        }
         if (to_skinny) {
             p->flops_skinny += f;
             G.skinny_mode = (G.n <= G.m) ? 0 : 1;
@@ -1113,6 +1220,10 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
     if (!p->skinny.groups.empty())
         rc = tkmi::upload((void **)&p->skinny.d_groups, p->skinny.groups.data(),
                           p->skinny.groups.size() * sizeof(GGroup));
+    if (!rc && !p->dots.groups.empty()) {
+        rc = tkmi::upload((void **)&p->dots.d_groups, p->dots.groups.data(), p->dots.groups.size() * sizeof(DGroup));
+        if (!rc) rc = tkmi::upload((void **)&p->dots.parts, nullptr, (size_t)p->dots.nparts * sizeof(double));
+    }
     // largest groups first: their tiles start earliest
     std::stable_sort(p->base.begin(), p->base.end(), [](const GGroup &a, const GGroup &b) {
         return a.m * a.n * std::max<int64_t>(a.k, 1) > b.m * b.n * std::max<int64_t>(b.k, 1);
@@ -1142,10 +1253,14 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
     }
     // C blocks of each part for α = 0 calls
     for (int q = 0; q < 2; ++q) {
-        for (const GGroup &G : q == 0 ? p->base : p->skinny.groups) {
-            if (G.m * G.n == 0) continue;
-            p->scale[q].push_back(RGroup{G.m, G.n, G.c_off, G.ldc, 0, 0, p->scale_total[q]});
-            p->scale_total[q] += G.m * G.n;
+        for (const std::vector<GGroup> *list : q == 0 ? std::vector<const std::vector<GGroup> *>{&p->base}
+                                                      : std::vector<const std::vector<GGroup> *>{&p->skinny.groups,
+                                                                                                 &p->skinny_c}) {
+            for (const GGroup &G : *list) {
+                if (G.m * G.n == 0) continue;
+                p->scale[q].push_back(RGroup{G.m, G.n, G.c_off, G.ldc, 0, 0, p->scale_total[q]});
+                p->scale_total[q] += G.m * G.n;
+            }
         }
         if (!rc && !p->scale[q].empty())
             rc = tkmi::upload((void **)&p->d_scale[q], p->scale[q].data(), p->scale[q].size() * sizeof(RGroup));
@@ -1206,6 +1321,7 @@ int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, 
         rc = launch_cfg(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s, tiles, reduce);
     if (rc) return rc;
     if (parts & 2) rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
+    if (!rc && (parts & 2)) rc = launch_dots(plan->dots, A, B, C, alpha, beta, s);
     return rc;
 }
 

@@ -289,7 +289,8 @@ class HomSpace:
     def compose(self, other):
         if self.domain != other.codomain:
             raise SpaceMismatch(f"{self.domain} ≠ {other.codomain}")
-        return HomSpace(self.codomain, other.domain)
+        # a full contraction composes to a scalar space, which keeps the sector type
+        return HomSpace(self.codomain, other.domain, unit=self.unit)
 
     # ---------------- structure ----------------
     def blocksectors(self):

@@ -116,3 +116,24 @@ def test_graph_survives_cache_eviction(T):
         y = H.replay()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(y.numpy(), eager)
+
+
+def test_full_contraction_to_a_scalar(T):
+    """@tensor C = A[i j k] * B[i j k]-style full contraction (the reference's MPO benchmark
+    ends in one, benchmark/comparisons/tensorkit_timers.jl:6-17): the result lives in the
+    scalar space (one 1×1 block of the unit sector) and equals the dense contraction; its
+    GEMM is a single 1×1 group with a long K, computed as a dot product."""
+    rng = np.random.default_rng(7)
+    a = U1Space((0, 40), (1, 30), (-1, 30), (2, 11))
+    b = U1Space((0, 3), (1, 2), (-1, 2))
+    A = T.TensorMap.randn(HomSpace(ProductSpace(a, b, a.dual()), ProductSpace()), rng)
+    B = T.TensorMap.randn(HomSpace(ProductSpace(a.dual(), b.dual(), a), ProductSpace()), rng)
+    C = T.contract_labels(A, "ijk", B, "ijk", "", 0)
+    assert C.space.dim() == 1 and C.numout == 0 and C.numin == 0
+    ref = float(np.einsum("ijk,ijk->", dense(A), dense(B)))
+    assert abs(float(C.numpy()[0]) - ref) <= 1e-12 * np.linalg.norm(A.numpy()) * np.linalg.norm(B.numpy())
+    # α/β into an existing scalar, and conj of the second operand (real data: the same)
+    C2 = T.TensorMap.randn(C.space, rng)
+    c0 = float(C2.numpy()[0])
+    T.contract_labels(A, "ijk", B, "ijk", "", 0, conjB=False, out=C2, alpha=2.0, beta=-1.0)
+    assert abs(float(C2.numpy()[0]) - (2.0 * ref - c0)) <= 1e-12 * (abs(ref) + abs(c0) + 1)

@@ -268,3 +268,32 @@ def test_exec_parts_split_reduction_separately(lib):
     finally:
         torch.cuda.synchronize()
         L.tkmi_gemm_plan_destroy(h)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_dot_groups(lib, ta, tb):
+    """1×1 results with a long K (full contractions to a scalar) run as two-pass dot
+    products in FP64 for every precision form: exact for integer data (any order), beside
+    skinny groups in the same plan; α = 0 scales C; a repeated call is bit-identical."""
+    rng = np.random.default_rng(40 + 2 * ta + tb)
+    shapes = [(1, 1, 65), (1, 1, 8192), (1, 1, 8193), (1, 1, 100003), (1, 1, 2_500_000), (40, 8, 30), (1, 1, 7)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [ta] * len(shapes), [tb] * len(shapes))
+    A = rng.integers(-3, 4, na).astype(np.float64)
+    B = rng.integers(-3, 4, nb).astype(np.float64)
+    Cm = rng.integers(-3, 4, nc).astype(np.float64)
+    for prec in (0, 1, 2):
+        for alpha, beta in ((1.0, 0.0), (2.0, -1.0), (0.0, 3.0)):
+            out = _run(lib, tab, A, B, Cm, prec, alpha, beta)
+            np.testing.assert_array_equal(out, _ref(tab, A, B, Cm, alpha, beta))
+    # a k×1 operand with ld = k beyond the MFMA kernel's 32-bit slab offsets: fine here
+    big = [(1, 1, 5_000_000)]
+    tb_, na_, nb_, nc_ = _make_groups(rng, big, [0], [0])
+    Ab, Bb = rng.integers(-2, 3, na_).astype(np.float64), rng.integers(-2, 3, nb_).astype(np.float64)
+    Cb = np.zeros(nc_)
+    np.testing.assert_array_equal(_run(lib, tb_, Ab, Bb, Cb, 0, 1.0, 0.0), _ref(tb_, Ab, Bb, Cb, 1.0, 0.0))
+    Af, Bf = rng.standard_normal(na), rng.standard_normal(nb)
+    o1 = _run(lib, tab, Af, Bf, Cm, 0, 1.0, 0.0)
+    o2 = _run(lib, tab, Af, Bf, Cm, 0, 1.0, 0.0)
+    assert np.array_equal(o1, o2)
+    ref = _ref(tab, Af, Bf, Cm, 1.0, 0.0)
+    assert np.linalg.norm(o1 - ref) <= 1e-12 * np.linalg.norm(ref)
