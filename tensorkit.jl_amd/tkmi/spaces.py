@@ -16,6 +16,7 @@ what every device kernel addresses; it must match the reference bit for bit.
 """
 from __future__ import annotations
 
+from collections import OrderedDict
 from functools import lru_cache
 
 from .fusiontrees import FusionTreeBlock, fusiontrees, fuse_set
@@ -122,6 +123,33 @@ def Vect(pairs, dual=False):
     return GradedSpace(dict(pairs), dual)
 
 
+# Structure of spaces, shared by every equal space object: TensorKit keeps it in global
+# caches keyed by the space (`src/spaces/structure.jl`, `src/auxiliary/caches.jl`), so the
+# HomSpace objects a contraction builds afresh for its operands and result do not
+# recompute their block and sub-block structure.  Bounded LRU tables.
+_SPACE_CACHE_SIZE = 10_000
+_HOM_STRUCTURE = OrderedDict()  # (codomain, domain, unit of a scalar space) -> structure dict
+_PRODUCT_BLOCKSECTORS = OrderedDict()  # (ProductSpace, unit) -> sorted coupled sectors
+
+
+def _shared(table, key, make):
+    v = table.get(key)
+    if v is None:
+        v = make()
+        table[key] = v
+        if len(table) > _SPACE_CACHE_SIZE:
+            table.popitem(last=False)
+    else:
+        table.move_to_end(key)
+    return v
+
+
+def clear_space_caches():
+    """Drop the shared space structure (part of `empty_globalcaches`)."""
+    _HOM_STRUCTURE.clear()
+    _PRODUCT_BLOCKSECTORS.clear()
+
+
 def _product_first_fastest(lists):
     if not lists:
         yield ()
@@ -187,13 +215,17 @@ class ProductSpace:
         n = len(self.spaces)
         if n == 0:
             return [unit]
-        bs = set()
-        if n == 1:
-            bs.update(self.spaces[0].sectors())
-        else:
-            for s in self.sectors():
-                bs.update(fuse_set(s, unit))
-        return sorted(bs)
+
+        def make():
+            bs = set()
+            if n == 1:
+                bs.update(self.spaces[0].sectors())
+            else:
+                for s in self.sectors():
+                    bs.update(fuse_set(s, unit))
+            return tuple(sorted(bs))
+
+        return list(_shared(_PRODUCT_BLOCKSECTORS, (self, unit), make))
 
     def fusiontrees(self, c):
         return fusiontrees([V.sectors() for V in self.spaces], c, self.isdual())
@@ -226,7 +258,8 @@ class HomSpace:
         self.codomain = codomain
         self.domain = domain
         self._h = hash((codomain, domain))
-        self._cache = {}
+        scalar = len(codomain) == 0 and len(domain) == 0
+        self._cache = _shared(_HOM_STRUCTURE, (codomain, domain, unit if scalar else None), dict)
         self._unit = unit
 
     numout = property(lambda self: len(self.codomain))

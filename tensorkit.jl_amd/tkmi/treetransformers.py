@@ -24,6 +24,7 @@ import numpy as np
 from . import _lib
 from .fusiontrees import FusionTree, braid_pair, braid_block, permute_block, split_tree
 from .sectors import UNIQUE
+from .spaces import clear_space_caches
 
 DEFAULT_GLOBALCACHE_SIZE = 10 ** 4  # src/auxiliary/caches.jl:19
 
@@ -110,10 +111,12 @@ _TRANSFORMER_CACHE = _LRU(DEFAULT_GLOBALCACHE_SIZE)
 
 
 def empty_globalcaches():
-    """`empty_globalcaches!()` (`src/auxiliary/caches.jl:2-5`): the transformer cache and
-    this package's other plan caches (grouped GEMMs, fused contractions)."""
+    """`empty_globalcaches!()` (`src/auxiliary/caches.jl:2-5`): the transformer cache,
+    this package's other plan caches (grouped GEMMs, fused contractions) and the shared
+    space structure."""
     for c in _ALL_CACHES:
         c.clear()
+    clear_space_caches()
 
 
 def global_cache_info():
