@@ -49,3 +49,24 @@ def test_empty_globalcaches_clears_every_plan_cache():
     TT.empty_globalcaches()
     for cache in (TT._TRANSFORMER_CACHE, T._GEMM_CACHE, FC._FUSED_CACHE):
         assert cache.get(("probe",)) is None
+
+
+def test_space_structure_is_shared_and_cleared():
+    """Equal spaces share their block structure (the reference's global caches keyed by
+    the space); scalar spaces of different sector types do not; empty_globalcaches drops
+    it, and the structure recomputed afterwards is the same."""
+    from tkmi import treetransformers as TT
+    from tkmi.sectors import U1Irrep, SU2Irrep
+    from tkmi.spaces import HomSpace, ProductSpace, U1Space
+    V = U1Space((0, 2), (1, 1), (-1, 3))
+    W1 = HomSpace(ProductSpace(V, V), ProductSpace(V))
+    ds = W1.degeneracystructure()
+    W2 = HomSpace(ProductSpace(U1Space((0, 2), (1, 1), (-1, 3)), V), ProductSpace(V))
+    assert W2 == W1 and W2._cache is W1._cache and "ds" in W2._cache
+    s1 = HomSpace(ProductSpace(), ProductSpace(), unit=U1Irrep(0))
+    s2 = HomSpace(ProductSpace(), ProductSpace(), unit=SU2Irrep(0))
+    assert s1._cache is not s2._cache
+    TT.empty_globalcaches()
+    W3 = HomSpace(ProductSpace(V, V), ProductSpace(V))
+    assert W3._cache is not W1._cache and not W3._cache
+    assert W3.degeneracystructure() == ds
