@@ -10,7 +10,9 @@
 //                  elements of that numbering, across row ends, so every tile but an
 //                  item's last is full; lanes consecutive on both sides;
 //     dW != dR  -> "transpose" mode: a (tW × tR) tile (powers of two) is read with
-//                  lanes along dR, staged in LDS, and written with lanes along dW.
+//                  lanes along dR, staged in LDS, and written with lanes along dW;
+//                  when tW·tR leaves the tile's area unused (small dims), the tile
+//                  takes tB slices along a third dim dB as well.
 // Work items are split into classes by the number of trees they touch (L = 1, 2, 4,
 // 8); each class is one launch of transform_kernel<T, L>.  Inside a launch the tiles
 // of all items are numbered consecutively (tile_start prefix sums) and each
@@ -81,11 +83,11 @@ constexpr int class_L(int c) { return 1 << c; }
 // One tile of one work item.  COPY: the tile is nr runs of nw elements along dW
 // (contiguous on both sides), numbered run after run; otherwise a tW × tR tile
 // read with lanes along dR and written with lanes along dW through LDS.
-template <typename T, int L, bool COPY>
+template <typename T, int L, bool COPY, bool SLICED>
 __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__restrict__ trees,
                                                const double *__restrict__ Umat, const T *__restrict__ src,
                                                T *__restrict__ dst, double alpha, double beta, bool keep,
-                                               int64_t t, T *__restrict__ lds) {
+                                               int64_t t, T *__restrict__ lds, int dB_in, int lgB) {
     constexpr int E = class_elems(L);
     const int nread = B.nread, nwrite = B.nwrite;
     const int dW = B.dW, dR = B.dR;
@@ -94,7 +96,19 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     const int64_t iW = t % B.ntW;
     t /= B.ntW;
     const int64_t iR = t % B.ntR;
-    const int64_t bt = t / B.ntR;
+    t /= B.ntR;
+    // slices along dB only in the SLICED instantiation (the others keep their registers)
+    const int dB = SLICED ? dB_in : -1;
+    const int tB = SLICED ? 1 << lgB : 1;
+    int64_t iB = 0;
+    if (SLICED && dB >= 0) {
+        const int64_t ntB = (B.size[dB] + tB - 1) >> lgB;
+        iB = t % ntB;
+        t /= ntB;
+    }
+    const int64_t bt = t;
+    const int64_t b0 = iB * tB;
+    const int nb = dB >= 0 ? (int)min((int64_t)tB, B.size[dB] - b0) : 1;
     int64_t w0, r0;
     int nw, nr;
     // copy mode: tile iR holds elements [iR·area, iR·area + cnt) of the item's plane
@@ -123,7 +137,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     // offset of the tile corner in one tree (the batch index bt is decomposed
     // again per tree: cheaper than holding the multi-index in scalar registers)
     auto corner = [&](const TTree &tr) {
-        int64_t o = tr.off + w0 * tr.str[dW] + r0 * tr.str[dR];
+        int64_t o = tr.off + w0 * tr.str[dW] + r0 * tr.str[dR] + (dB >= 0 ? b0 * tr.str[dB] : 0);
         int64_t b = bt;
         for (int k = 0; k < B.nother; ++k) {
             const int d = B.odims[k];
@@ -151,7 +165,8 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     const int ws = long_rows ? 0 : (int)w0;
     const int wshort = long_rows ? 1 : (int)W;
     const float inv = 1.0f / (float)(COPY ? wshort : nw);
-    auto pos_read = [&](int k, int &w, int &r) {
+    auto pos_read = [&](int k, int &w, int &r, int &b) {
+        b = 0;
         if (COPY) {
             const int e = tid + k * NTHREADS;
             if (long_rows) {
@@ -167,15 +182,30 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
             w = e;
             return e < cnt;
         }
+        // lanes along dR; with slices the index along the other side runs over tB
+        // slices of tW
         r = tid & (tR - 1);
-        w = (tid >> B.lgR) + k * (NTHREADS >> B.lgR);
-        return w < nw && r < nr;
+        const int wv = (tid >> B.lgR) + k * (NTHREADS >> B.lgR);
+        if (!SLICED) {
+            w = wv;
+            return w < nw && r < nr;
+        }
+        w = wv & (tW - 1);
+        b = wv >> B.lgW;
+        return w < nw && r < nr && b < nb;
     };
-    auto pos_write = [&](int k, int &w, int &r) {
-        if (COPY) return pos_read(k, w, r);
+    auto pos_write = [&](int k, int &w, int &r, int &b) {
+        if (COPY) return pos_read(k, w, r, b);
+        b = 0;
         w = tid & (tW - 1);
-        r = (tid >> B.lgW) + k * (NTHREADS >> B.lgW);
-        return w < nw && r < nr;
+        const int rv = (tid >> B.lgW) + k * (NTHREADS >> B.lgW);
+        if (!SLICED) {
+            r = rv;
+            return w < nw && r < nr;
+        }
+        r = rv & (tR - 1);
+        b = rv >> B.lgR;
+        return w < nw && r < nr && b < nb;
     };
 
     const bool has_alpha = (alpha != 0.0);  // α = 0: the source is not read (BLAS convention)
@@ -194,11 +224,13 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 const TTree &tr = trees[B.read_tree + ic + i];
                 const T *sp = src + corner(tr);
                 const int64_t sW = tr.str[dW], sR = COPY ? tr.str[dR] - W * tr.str[dW] : tr.str[dR];
+                const int64_t sB = dB >= 0 ? tr.str[dB] : 0;
 #pragma unroll
                 for (int k = 0; k < E; ++k) {
-                    int w, r;
-                    x[i][k] = pos_read(k, w, r) ? __builtin_nontemporal_load(&sp[(int64_t)w * sW + (int64_t)r * sR])
-                                                 : (T)0;
+                    int w, r, b;
+                    x[i][k] = pos_read(k, w, r, b)
+                                  ? __builtin_nontemporal_load(&sp[(int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB])
+                                  : (T)0;
                 }
             }
         }
@@ -211,8 +243,8 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 if (i < nc)
 #pragma unroll
                     for (int k = 0; k < E; ++k) {
-                        int w, r;
-                        if (pos_read(k, w, r)) lds[(i * tW + w) * pitch + r] = x[i][k];
+                        int w, r, b;
+                        if (pos_read(k, w, r, b)) lds[((i * tB + b) * tW + w) * pitch + r] = x[i][k];
                     }
             __syncthreads();
 #pragma unroll
@@ -220,8 +252,8 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 if (i < nc)
 #pragma unroll
                     for (int k = 0; k < E; ++k) {
-                        int w, r;
-                        x[i][k] = pos_write(k, w, r) ? lds[(i * tW + w) * pitch + r] : (T)0;
+                        int w, r, b;
+                        x[i][k] = pos_write(k, w, r, b) ? lds[((i * tB + b) * tW + w) * pitch + r] : (T)0;
                     }
         }
 #pragma unroll
@@ -248,11 +280,12 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 const TTree &tw = trees[B.write_tree + j];
                 T *dp = dst + corner(tw);
                 const int64_t sW = tw.str[dW], sR = COPY ? tw.str[dR] - W * tw.str[dW] : tw.str[dR];
+                const int64_t sB = dB >= 0 ? tw.str[dB] : 0;
 #pragma unroll
                 for (int k = 0; k < E; ++k) {
-                    int w, r;
-                    if (pos_write(k, w, r)) {
-                        T *a = dp + (int64_t)w * sW + (int64_t)r * sR;
+                    int w, r, b;
+                    if (pos_write(k, w, r, b)) {
+                        T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
                         store(has_beta ? o[j][k] + tbeta * *a : o[j][k], a);
                     }
                 }
@@ -268,11 +301,13 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 // STAGED: the launch has transpose tiles, which stage through LDS.  A launch of copy
 // tiles only (STAGED = false) declares no LDS, so LDS does not cap how many of its
 // workgroups share a CU (for L = 2 and 4 the staging array is 36 KB).
-template <typename T, int L, bool STAGED>
+// SLICED: the launch has transpose tiles with slices along a third dim, described per
+// block in `slices` (dB | lgB << 8, or -1), which only this instantiation reads.
+template <typename T, int L, bool STAGED, bool SLICED>
 __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     const TBlock *__restrict__ blocks, int64_t nblocks, const TTree *__restrict__ trees,
     const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, double alpha,
-    double beta, int keep, int64_t tile0) {
+    double beta, int keep, int64_t tile0, const int32_t *__restrict__ slices) {
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
     // binary search: last item with tile_start <= gt
     int64_t lo = 0, hi = nblocks - 1;
@@ -287,12 +322,20 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     if constexpr (STAGED) {
         constexpr int LDSN = L * (class_area(L) + class_twmax(L));  // tW·(tR+1) ≤ area + tW per tree
         __shared__ T lds[LDSN];
-        if (B.mode == 0)
-            transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds);
-        else
-            transform_tile<T, L, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds);
+        if (B.mode == 0) {
+            transform_tile<T, L, true, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds,
+                                              -1, 0);
+        } else if (SLICED) {
+            const int32_t sl = slices[lo];
+            transform_tile<T, L, false, true>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds,
+                                              sl < 0 ? -1 : (sl & 0xff), sl < 0 ? 0 : (sl >> 8));
+        } else {
+            transform_tile<T, L, false, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds,
+                                               -1, 0);
+        }
     } else {
-        transform_tile<T, L, true>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, nullptr);
+        transform_tile<T, L, true, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, nullptr,
+                                          -1, 0);
     }
 }
 
@@ -337,6 +380,9 @@ struct PlanSide {
     TBlock *d_blocks = nullptr;
     bool staged = false;  // some block uses a transpose tile (LDS staging)
     bool reads = false;   // some block reads source trees (blocks with none only scale by β)
+    bool sliced = false;  // some transpose tile takes slices along a third dim
+    std::vector<int32_t> slices;  // per block: dB | lgB << 8, or -1 (uploaded when sliced)
+    int32_t *d_slices = nullptr;
     int64_t written = 0;  // elements written by all classes of this side together
 };
 
@@ -385,8 +431,14 @@ void choose_tile(int64_t sW, int64_t sR, int area, int twmax, int &tW, int &tR) 
     }
 }
 
+// Tiles of a block: ntW · ntR · (slices along dB) · nbatch.
+int64_t block_tiles(const TBlock &b, int dB, int lgB) {
+    const int64_t ntB = dB >= 0 ? (b.size[dB] + ((int64_t)1 << lgB) - 1) >> lgB : 1;
+    return b.ntW * b.ntR * ntB * b.nbatch;
+}
+
 // Fill tiling fields of a block given which trees are read/written.
-void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
+void tile_block(TBlock &b, const std::vector<TTree> &trees, int L, bool allow_slices, int &dB, int &lgB) {
     const TTree &tw = trees[b.write_tree];
     const TTree &tr = trees[b.read_tree];
     const int area = (L == 1) ? class_area(1) : (L == 2 ? class_area(2) : (L == 4 ? class_area(4) : class_area(8)));
@@ -423,6 +475,29 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
         b.lgW = log2i(b.tW);
         b.lgR = log2i(b.tR);
     }
+    // with allow_slices, a transpose tile that leaves at least half its area unused
+    // (small dims) takes slices along one more dim: the other dim with the smallest read
+    // stride, tB of them (a power of two) with tW·tR·tB within the area and tW·tB within
+    // twmax (the LDS staging bound, so the launch's LDS size does not grow)
+    dB = -1;
+    lgB = 0;
+    if (allow_slices && b.mode == 1 && (int64_t)b.tW * b.tR * 2 <= area) {
+        int64_t best = INT64_MAX;
+        for (int d = 0; d < b.ndim; ++d) {
+            if (d == dW || d == dR || b.size[d] <= 1) continue;
+            const int64_t sr = std::llabs(tr.str[d]);
+            if (sr < best) { best = sr; dB = d; }
+        }
+        if (dB >= 0) {
+            int tb = 1;
+            while ((int64_t)b.tW * b.tR * tb * 2 <= area && b.tW * tb * 2 <= twmax && tb < b.size[dB]) tb *= 2;
+            if (tb > 1) {
+                lgB = log2i(tb);
+            } else {
+                dB = -1;
+            }
+        }
+    }
     b.dW = dW;
     b.dR = dR;
     if (b.mode == 0) {
@@ -435,7 +510,7 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L) {
     b.nother = 0;
     b.nbatch = 1;
     for (int d = 0; d < b.ndim; ++d) {
-        if (d == dW || d == dR) continue;
+        if (d == dW || d == dR || d == dB) continue;
         b.odims[b.nother++] = d;
         b.nbatch *= b.size[d];
     }
@@ -485,6 +560,8 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             sides[c].ntiles = 0;
             sides[c].staged = false;
             sides[c].reads = false;
+            sides[c].sliced = false;
+            sides[c].slices.clear();
             sides[c].written = 0;
         }
         // A small plan runs as one launch of its largest class when that class holds at
@@ -502,6 +579,39 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
         }
         const bool merge = moved <= SMALL_PLAN && class_L(cmax) <= 4;
         for (int c = 0; c < NCLASS; ++c) sides[c].written = moved;
+        // Slicing is decided per class: its launch then uses the kernel instantiation with
+        // slices, which holds more registers (the L = 1 kernel drops from 7 to 6 waves per
+        // SIMD), so a class takes it only when slices cut its tile count by at least 60%
+        // (measured: 2.65× on an SU(2) rank-4 permute with 8–24 element dims, whose tiles
+        // drop to 0.24 of their number; a class at 0.6 with 16–48 element dims and the
+        // 82.8 GB permute with 32–96 element dims ran slower in the sliced kernel).
+        bool slice_class[NCLASS];
+        {
+            int64_t plain[NCLASS] = {0}, sliced[NCLASS] = {0};
+            for (auto &it : items) {
+                const int R = adjoint ? it.nread : it.nwrite;
+                const int Cn = adjoint ? it.nwrite : it.nread;
+                const int64_t rt = adjoint ? it.write_tree : it.read_tree;
+                const int64_t wt = adjoint ? it.read_tree : it.write_tree;
+                for (int jc = 0; jc < R; jc += MAXW) {
+                    TBlock b;
+                    std::memset(&b, 0, sizeof(b));
+                    b.ndim = (int)it.size.size();
+                    for (int d = 0; d < b.ndim; ++d) b.size[d] = it.size[d];
+                    b.nread = Cn;
+                    b.nwrite = std::min(MAXW, R - jc);
+                    b.read_tree = rt;
+                    b.write_tree = wt + jc;
+                    const int c = merge ? cmax : class_of(b.nread, b.nwrite);
+                    int sdb, slg;
+                    tile_block(b, trees, class_L(c), false, sdb, slg);
+                    plain[c] += block_tiles(b, sdb, slg);
+                    tile_block(b, trees, class_L(c), true, sdb, slg);
+                    sliced[c] += block_tiles(b, sdb, slg);
+                }
+            }
+            for (int c = 0; c < NCLASS; ++c) slice_class[c] = sliced[c] * 10 <= plain[c] * 4;
+        }
         for (auto &it : items) {
             const int R = adjoint ? it.nread : it.nwrite;   // written trees on this side
             const int Cn = adjoint ? it.nwrite : it.nread;  // read trees on this side
@@ -519,11 +629,14 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
                 b.u_start = it.u_start + jc;
                 b.u_ld = R;
                 const int c = merge ? cmax : class_of(b.nread, b.nwrite);
-                tile_block(b, trees, class_L(c));
+                int sdb, slg;
+                tile_block(b, trees, class_L(c), slice_class[c], sdb, slg);
                 b.tile_start = sides[c].ntiles;
-                sides[c].ntiles += b.ntW * b.ntR * b.nbatch;
+                sides[c].ntiles += block_tiles(b, sdb, slg);
+                sides[c].slices.push_back(sdb >= 0 ? (int32_t)(sdb | (slg << 8)) : -1);
                 sides[c].staged |= (b.mode != 0);
                 sides[c].reads |= (b.nread > 0);
+                sides[c].sliced |= (sdb >= 0);
                 sides[c].blocks.push_back(b);
             }
         }
@@ -542,6 +655,9 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             if ((rc = tkmi::upload((void **)&side->d_blocks, side->blocks.data(),
                                    side->blocks.size() * sizeof(TBlock))))
                 return rc;
+            if (side->sliced && (rc = tkmi::upload((void **)&side->d_slices, side->slices.data(),
+                                                   side->slices.size() * sizeof(int32_t))))
+                return rc;
         }
     }
     return TKMI_OK;
@@ -557,6 +673,8 @@ void free_plan(tkmi_transform_plan_s *p) {
     for (int c = 0; c < NCLASS; ++c) {
         (void)hipFree(p->fwd[c].d_blocks);
         (void)hipFree(p->adj[c].d_blocks);
+        (void)hipFree(p->fwd[c].d_slices);
+        (void)hipFree(p->adj[c].d_slices);
     }
     delete p;
 }
@@ -611,13 +729,18 @@ void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const do
     const int64_t chunk = ((int64_t)1 << 31) / NTHREADS;
     for (int64_t t0 = 0; t0 < side.ntiles; t0 += chunk) {
         const unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
-        if (side.staged)
-            hipLaunchKernelGGL((transform_kernel<T, L, true>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, keep, t0);
-        else
-            hipLaunchKernelGGL((transform_kernel<T, L, false>), dim3(grid), dim3(NTHREADS), 0, stream,
+        if (side.staged && side.sliced)
+            hipLaunchKernelGGL((transform_kernel<T, L, true, true>), dim3(grid), dim3(NTHREADS), 0, stream,
                                side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta,
-                               keep, t0);
+                               keep, t0, (const int32_t *)side.d_slices);
+        else if (side.staged)
+            hipLaunchKernelGGL((transform_kernel<T, L, true, false>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
+                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, keep, t0,
+                               (const int32_t *)nullptr);
+        else
+            hipLaunchKernelGGL((transform_kernel<T, L, false, false>), dim3(grid), dim3(NTHREADS), 0, stream,
+                               side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta,
+                               keep, t0, (const int32_t *)nullptr);
     }
 }
 
