@@ -173,11 +173,16 @@ def test_su2_permute_dense(T, k):
         assert np.linalg.norm(dB - ref) <= 1e-13 * np.linalg.norm(ref)
 
 
-def test_su2_permute_vs_c_loop_cfg3_scaled(T):
-    """cfg-3 leg space with degeneracies ÷16: K2 against the oracle's table loop."""
+@pytest.mark.parametrize("div", [16, 4])
+def test_su2_permute_vs_c_loop_cfg3_scaled(T, div):
+    """cfg-3 leg space with degeneracies ÷16 (dims 2-6) and ÷4 (dims 8-24, 40.5 M
+    elements): K2 against the oracle's table loop.  The last two permutations have
+    plans whose transpose tiles take slices along a third dim (÷16: 1190 tiles against
+    5268 without slices; ÷4: 20 320 against 84 288), with partial slices at the ends of
+    the 6- and 24-wide dims; the first permutation has none."""
     from tkmi.spaces import SU2Space
     from oracle import kernels as OK
-    V = SU2Space((0, 4), (0.5, 6), (1, 6), (1.5, 4), (2, 2))
+    V = SU2Space(*((j, n // div) for j, n in ((0, 64), (0.5, 96), (1, 96), (1.5, 64), (2, 32))))
     W = HomSpace(ProductSpace(V, V), ProductSpace(V, V))
     rng = np.random.default_rng(2)
     A = T.TensorMap.randn(W, rng)
