@@ -15,8 +15,18 @@
  *  - `perm` is the linearised index permutation p = (p1..., p2...) of
  *    TO.tensoradd!: dst index k takes source index perm[k] (0-based).
  *  - Streams are hipStream_t passed as void*; NULL is the default stream.
- *  - Plans are immutable after creation and may be used concurrently from
- *    several host threads on different streams.
+ *  - Concurrency.  Abelian and generic transform plans without a gather are
+ *    immutable after creation and may be executed concurrently from several host
+ *    threads on different streams.  The other plan kinds own device state that
+ *    every execution writes, so each serves ONE stream at a time (calls on one plan
+ *    from different streams, or overlapping in time, race silently):
+ *      - a GEMM plan: its split-K workspace, dot-product partials and 3×BF16 panels
+ *        (tkmi_gemm_exec, tkmi_gemm_exec_parts, tkmi_gemm_plan_autotune);
+ *      - a factorisation plan: its lanes' streams, handles and workspaces
+ *        (tkmi_svd_exec, tkmi_eigh_exec);
+ *      - a transform plan with a gather: its coefficient matrix, rewritten by every
+ *        tkmi_transform_exec_gather.
+ *    A binding that shares plans between tasks keeps one such plan per stream.
  */
 #ifndef TKMI_H
 #define TKMI_H
@@ -96,10 +106,21 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
                              int64_t dst_len, int64_t src_len);
 
 /* dst = β·dst + α·transform(src).  dtype: TKMI_F64 or TKMI_F32 (both buffers).
- * src and dst must not overlap.  BLAS convention: β = 0 writes dst without reading
- * it; α = 0 reads no source.  A plan that writes fewer than 2^25 elements and has no
- * fusion block wider than 4 trees runs as one launch; larger plans run one launch
- * per tree class (1, 2, 4, 8 trees per block).                               */
+ * Rounding follows the reference's tensoradd! (Julia evaluates
+ * scale(C, β) + scale(A, α) without a fused multiply-add):
+ *   - a block with one source tree (every abelian entry): dst = β·dst + (α·coeff)·src,
+ *     with α·coeff, the product with src and β·dst each rounded, then added;
+ *   - a block with several source trees: t = Σ_i U[j,i]·src_i (the recoupling GEMM),
+ *     then dst = β·dst + α·t with β·dst and α·t rounded separately.
+ * α and β are Float64; FP32 data is promoted (as Julia promotes Float32 data by
+ * Float64 scalars) and rounded to FP32 once, when stored.  So abelian permutes are
+ * bit-exact for every α and β.
+ * src and dst must not overlap (TKMI_ERR_ARGUMENT), except src == dst on an
+ * elementwise plan (identity permutation, equal tables: add!/scale! in place) and
+ * α = 0.  BLAS convention: β = 0 writes dst without reading it; α = 0 reads no
+ * source.  A plan that writes fewer than 2^25 elements and has no fusion block wider
+ * than 4 trees runs as one launch; larger plans run one launch per tree class
+ * (1, 2, 4, 8 trees per block).                                               */
 int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
                         double alpha, double beta, void *stream);
 
@@ -115,8 +136,8 @@ int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const v
  * of a generic plan (n = Σ rows·cols).  tkmi_transform_exec_gather refreshes U from g
  * and runs the transform: a contraction C = β·C + α·A·B with a small operand B,
  * folded with the permutes of blas_contract! (src/tensors/tensoroperations.jl:392-462)
- * into one pass over A and C.  U belongs to the plan, so calls on one plan must not
- * overlap on different streams.  Once a plan has a gather, tkmi_transform_exec and
+ * into one pass over A and C.  U belongs to the plan: a plan with a gather serves one
+ * stream at a time (calls on one plan must not overlap).  Once a plan has a gather, tkmi_transform_exec and
  * tkmi_transform_adjoint_exec on it return TKMI_ERR_ARGUMENT (its U has no meaning
  * without the current g). */
 int tkmi_transform_plan_set_gather(tkmi_transform_plan plan, int64_t n, const double *coef, const int64_t *idx);
@@ -150,7 +171,8 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
  * groups (min(m, n) ≤ 32 with k ≤ 64, and 1×1 results with longer k, i.e. full
  * contractions to a scalar) compute in FP64 for every precision form; a 1×1 result
  * is a dot product of fixed summation order (partials over k chunks, then their sum),
- * with partial sums in a plan-owned buffer.                                    */
+ * with partial sums in a plan-owned buffer.  The plan's workspaces (split-K partials,
+ * dot partials, 3×BF16 panels) are written by every call: one stream at a time.  */
 int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta,
                    const double *A, const double *B, double *C, void *stream);
 /* Tile shape for `precision`: 0 = automatic, 64 / 128 = square tiles with a
@@ -194,6 +216,7 @@ int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);
  * ld r_c) — the layouts of U: codomain←V_r, Vᴴ: V_r←domain as TensorMaps.
  * The blocks run concurrently on streams owned by the plan, ordered after the
  * work already on `stream`; `stream` waits for all of them before continuing.
+ * The plan's lanes (streams, handles, workspaces) serve one call at a time.
  * Replaces MAK.svd_compact! per block (src/factorizations/matrixalgebrakit.jl:34-51). */
 int tkmi_svd_plan_create(tkmi_svd_plan *plan, int64_t nblocks, const int64_t *m, const int64_t *n,
                          const int64_t *a_off, const int64_t *lda);

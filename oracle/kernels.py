@@ -16,7 +16,9 @@ _lib = None
 
 def build():
     """Compile ref_kernels.c with gcc/OpenMP into oracle/liboracle.so."""
-    cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-o", SO, SRC]
+    # -ffp-contract=off: no fused multiply-add, each product rounded on its own as in
+    # the reference's tensoradd! (gcc contracts a*b + c into an FMA by default)
+    cmd = ["gcc", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-o", SO, SRC]
     subprocess.run(cmd, check=True)
     return SO
 

@@ -9,6 +9,9 @@
  *            for each fusion block: buffer_dst = buffer_src·Uᵀ, then
  *            dst_j = β·dst_j + α·permutedims(buffer_dst[:, j], p)
  *
+ * Built with -ffp-contract=off: β·dst and (α·coeff)·src are rounded separately and
+ * then added, as Julia evaluates scale(C, β) + scale(A, α) (no fused multiply-add).
+ *
  * Entries are distributed over OpenMP threads (the reference's tforeach /
  * @tasks over sub-blocks when TRANSFORMER_THREADS > 1).  Strided loops use an
  * odometer over the destination index; the destination's fastest dim is the
@@ -134,6 +137,15 @@ void oracle_generic_permute(int N, const int32_t *perm, int64_t nblocks, const i
     for (int64_t b = 0; b < nblocks; ++b) {
         const int64_t R = rows[b], C = cols[b];
         const int64_t *szs = sz_src + b * N, *szd = sz_dst + b * N;
+        if (R == 1 && C == 1) {
+            /* a block of one tree: tensoradd! with α·coeff, no recoupling GEMM
+             * (indexmanipulations.jl:684-690) */
+            const int64_t *sod = so_dst + toff_d[b] * (N + 1), *sos = so_src + toff_s[b] * (N + 1);
+            int64_t sstr[MAXN];
+            for (int k = 0; k < N; ++k) sstr[k] = sos[perm[k]];
+            strided_add(N, szd, sod, sod[N], sstr, sos[N], src, dst, alpha * U[uoff[b]], beta);
+            continue;
+        }
         int64_t bs = 1;
         for (int k = 0; k < N; ++k) bs *= szs[k];
         /* 1. extract: buffer_src[:, i] = src_i in canonical (column-major) order */

@@ -93,6 +93,9 @@ struct tkmi_svd_plan_s {
 namespace {
 void free_svd(tkmi_svd_plan_s *p) {
     if (!p) return;
+    // work still queued on a lane uses its workspace: drain every lane first
+    for (auto &l : p->lanes)
+        if (l.stream) (void)hipStreamSynchronize(l.stream);
     for (auto &l : p->lanes) {
         if (l.handle) rocblas_destroy_handle(l.handle);
         if (l.done) (void)hipEventDestroy(l.done);

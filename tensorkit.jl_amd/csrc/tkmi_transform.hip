@@ -209,11 +209,19 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     };
 
     const bool has_alpha = (alpha != 0.0);  // α = 0: the source is not read (BLAS convention)
-    T o[L][E];
+    // One source tree (every abelian entry, and generic blocks of one tree): the
+    // reference scales by α·coeff (tensoradd! with α*coeff, indexmanipulations.jl:658-669,
+    // 684-690), so α is folded into the coefficient.  Several source trees: the reference
+    // forms Σ_i U[j,i]·src_i (the recoupling GEMM, :710-711) and scales that by α when it
+    // inserts the column (:715-720), so α is applied to the sum at the store.
+    // Sums are FP64 for both element types: FP32 data with the Float64 scalars of the
+    // reference's tensoradd! is promoted to Float64 and rounded once when stored.
+    const bool single = (nread == 1);
+    double o[L][E];
 #pragma unroll
     for (int j = 0; j < L; ++j)
 #pragma unroll
-        for (int k = 0; k < E; ++k) o[j][k] = (T)0;
+        for (int k = 0; k < E; ++k) o[j][k] = 0.0;
 
     for (int ic = 0; ic < nread && has_alpha; ic += L) {
         const int nc = min(L, nread - ic);
@@ -262,17 +270,28 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 #pragma unroll
                 for (int i = 0; i < L; ++i) {
                     if (i < nc) {
-                        const T u = (T)(alpha * Umat[B.u_start + j + (int64_t)(ic + i) * B.u_ld]);
+                        const double uc = Umat[B.u_start + j + (int64_t)(ic + i) * B.u_ld];
+                        const double u = single ? alpha * uc : uc;
 #pragma unroll
-                        for (int k = 0; k < E; ++k) o[j][k] += u * x[i][k];
+                        for (int k = 0; k < E; ++k) o[j][k] += u * (double)x[i][k];
                     }
                 }
             }
         }
     }
 
-    const T tbeta = (T)beta;
     const bool has_beta = (beta != 0.0);
+    // dst = β·dst + v with β·dst and v rounded separately, as the reference's
+    // tensoradd! (scale(C, β) + scale(A, α), no fused multiply-add in Julia)
+    auto combine = [&](double v, double c) {
+#pragma clang fp contract(off)
+        if (!single) v = alpha * v;
+        if (has_beta) {
+            const double bc = beta * c;
+            v = bc + v;
+        }
+        return v;
+    };
     auto store_all = [&](auto store) {
 #pragma unroll
         for (int j = 0; j < L; ++j) {
@@ -286,7 +305,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                     int w, r, b;
                     if (pos_write(k, w, r, b)) {
                         T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
-                        store(has_beta ? o[j][k] + tbeta * *a : o[j][k], a);
+                        store((T)combine(o[j][k], has_beta ? (double)*a : 0.0), a);
                     }
                 }
             }
@@ -396,6 +415,9 @@ struct tkmi_transform_plan_s {
     double *d_gcoef = nullptr;
     int64_t *d_gidx = nullptr;
     int64_t dst_len = 0, src_len = 0, elements = 0, nitems = 0;
+    // every element is read and written at the same offset by the same thread (an
+    // elementwise y = β·y + α·x: add!/scale!), so the source may be the destination
+    bool inplace = false;
     std::vector<TTree> trees;
     std::vector<double> U, Ut;
     PlanSide fwd[NCLASS], adj[NCLASS];
@@ -774,6 +796,16 @@ int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst
         TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
     if (dtype != TKMI_F64 && dtype != TKMI_F32)
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
+    // the source and destination vectors must not overlap (tiles read and write them in
+    // no particular order), except an elementwise plan run in place, or α = 0 (no reads)
+    if (reads && alpha != 0.0 && src && dst) {
+        const uintptr_t es = dtype == TKMI_F64 ? 8 : 4;
+        const uintptr_t s0 = (uintptr_t)src, d0 = (uintptr_t)dst;
+        const uintptr_t s1 = s0 + es * (uintptr_t)(adjoint ? p->dst_len : p->src_len);
+        const uintptr_t d1 = d0 + es * (uintptr_t)(adjoint ? p->src_len : p->dst_len);
+        if (s0 < d1 && d0 < s1 && !(p->inplace && s0 == d0))
+            TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: source and destination overlap");
+    }
     // a gather plan's coefficients are formed from gsrc at every call
     if (p->gather_len > 0 && (adjoint || !gsrc))
         TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: the plan's coefficients come from a gather source: "
@@ -822,10 +854,13 @@ int tkmi_abelian_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
     p->N = N;
     p->dst_len = dst_len;
     p->src_len = src_len;
+    p->inplace = dst_len == src_len;
+    for (int k = 0; k < N; ++k) p->inplace &= perm[k] == k;
     std::vector<HostItem> items;
     for (int64_t e = 0; e < nentries; ++e) {
         const int64_t *ds = dst_structs + e * (2 * N + 1);
         const int64_t *ss = src_structs + e * (2 * N + 1);
+        p->inplace &= std::equal(ds, ds + 2 * N + 1, ss);
         std::vector<int64_t> sizes(N);
         std::vector<std::vector<int64_t>> strs(2, std::vector<int64_t>(N));
         for (int k = 0; k < N; ++k) {

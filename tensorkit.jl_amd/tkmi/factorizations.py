@@ -20,7 +20,7 @@ import torch
 from . import _lib
 from . import tensors as T
 from .spaces import GradedSpace, HomSpace, ProductSpace
-from .treetransformers import _make_abelian_plan
+from .treetransformers import _make_abelian_plan, _LRU
 
 
 class SVDPlan:
@@ -46,7 +46,11 @@ class SVDPlan:
             self.handle = None
 
 
-_SVD_CACHE = {}
+# Factorisation plans own streams, rocBLAS handles and device workspace, and a sweep
+# meets a new space at every truncation: the caches are bounded (least recently used
+# plan evicted and destroyed) and emptied by empty_globalcaches().
+FACTORIZATION_CACHE_SIZE = 16
+_SVD_CACHE = _LRU(FACTORIZATION_CACHE_SIZE)
 
 # MatrixAlgebraKit's rocSOLVER SVD algorithms (the `alg` of svd_compact / svd_trunc)
 SVD_ALGORITHMS = {"qr_iteration": _lib.SVD_QRITERATION, "divide_and_conquer": _lib.SVD_DIVIDEANDCONQUER,
@@ -69,7 +73,7 @@ def svd_compact(t, alg=None):
     plan = _SVD_CACHE.get(W)
     if plan is None:
         plan = SVDPlan(W)
-        _SVD_CACHE[W] = plan
+        _SVD_CACHE.put(W, plan)
     counts = {c: r for (c, _, _, _), r in zip(plan.blocks, plan.r)}
     Vr = _mid_space(W, counts)
     U = T.TensorMap.empty(HomSpace(W.codomain, ProductSpace(Vr)))
@@ -182,7 +186,7 @@ class EighPlan:
             self.handle = None
 
 
-_EIGH_CACHE = {}
+_EIGH_CACHE = _LRU(FACTORIZATION_CACHE_SIZE)
 
 
 def eigh_full(t, alg=None):
@@ -197,7 +201,7 @@ def eigh_full(t, alg=None):
     plan = _EIGH_CACHE.get(W)
     if plan is None:
         plan = EighPlan(W)
-        _EIGH_CACHE[W] = plan
+        _EIGH_CACHE.put(W, plan)
     counts = {c: r for (c, _, _, _), r in zip(plan.blocks, plan.r)}
     Vr = _mid_space(W, counts)
     V = T.TensorMap.empty(HomSpace(W.codomain, ProductSpace(Vr)))

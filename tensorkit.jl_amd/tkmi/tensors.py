@@ -709,6 +709,9 @@ def blas_contract_(C, A, pA, B, pB, pAB, alpha, beta, precision=None):
     GEMM form (fermionic twists folded into the permute), grouped GEMM, and a
     final permute of the result when C is not in GEMM layout."""
     TC = C.dtype
+    if A.dtype != TC or B.dtype != TC:
+        # the kernels read every operand with C's element type: no promotion on the device
+        raise TypeError(f"contract: element types differ (A {A.dtype}, B {B.dtype}, C {TC})")
     copyA = not (_has_shared_permute(A, pA) and A.dtype == TC)
     copyB = not (_has_shared_permute(B, pB) and B.dtype == TC)
     fermionic = C.space.unit.braiding_style == FERMIONIC

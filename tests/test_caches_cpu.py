@@ -70,3 +70,17 @@ def test_space_structure_is_shared_and_cleared():
     W3 = HomSpace(ProductSpace(V, V), ProductSpace(V))
     assert W3._cache is not W1._cache and not W3._cache
     assert W3.degeneracystructure() == ds
+
+
+def test_factorization_caches_are_bounded():
+    """svd / eigh plans own streams, handles and workspace: their caches are LRUs that
+    evict beyond a fixed size and that empty_globalcaches() empties."""
+    from tkmi import factorizations as F
+    for cache in (F._SVD_CACHE, F._EIGH_CACHE):
+        assert isinstance(cache, TT._LRU) and cache in TT._ALL_CACHES
+        assert cache.maxsize == F.FACTORIZATION_CACHE_SIZE
+        for i in range(cache.maxsize + 5):
+            cache.put(("space", i), _Plan())
+        assert len(cache.d) == cache.maxsize
+        TT.empty_globalcaches()
+        assert len(cache.d) == 0

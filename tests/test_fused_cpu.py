@@ -102,3 +102,29 @@ def test_fused_equals_unfused(case):
         ref = _unfused(C, A, pA, B, pB, pC, twA, twB, a, b, 0.75, -1.25, c0)
         got = _fused(C, A, pA, B, pB, pC, twA, twB, a, b, 0.75, -1.25, c0)
         assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_mixed_element_types_are_refused():
+    """A tiny float32 B with float64 A and C: the fused path would read B with C's
+    element type, so it must not apply; the contraction raises before any launch."""
+    import torch
+    from tkmi import tensors as T
+    V = U1Space((0, 3), (1, 2), (-1, 2))
+    P = U1Space((0, 1), (1, 1))
+    WA = HomSpace(ProductSpace(V, P), ProductSpace(V))
+    WB = HomSpace(ProductSpace(P), ProductSpace(P))
+    WC = HomSpace(ProductSpace(V, P), ProductSpace(V))
+
+    def mk(W, dt):  # host tensors: nothing here reaches the device
+        return T.TensorMap(torch.zeros(W.dim(), dtype=dt), W)
+
+    pA, pB = ((0, 2), (1,)), ((0,), (1,))
+    A, C = mk(WA, torch.float64), mk(WC, torch.float64)
+    assert FC.eligible(C, A, pA, mk(WB, torch.float64), pB)
+    for dA, dB, dC in [(torch.float64, torch.float32, torch.float64),
+                       (torch.float32, torch.float64, torch.float64),
+                       (torch.float32, torch.float32, torch.float64)]:
+        A, B, C = mk(WA, dA), mk(WB, dB), mk(WC, dC)
+        assert not FC.eligible(C, A, pA, B, pB)
+        with pytest.raises(TypeError):
+            T.blas_contract_(C, A, pA, B, pB, ((0, 2), (1,)), 1.0, 0.0)

@@ -35,8 +35,26 @@ def test_cfg1_permute(T):
     np.testing.assert_array_equal(B.numpy(), _ref_permute(W, x, (2, 0), (1,)))
 
 
+# α, β include values that are not powers of two: the reference's tensoradd! rounds
+# β·C and (α·coeff)·A separately (Julia does not fuse a*b + c), which a fused
+# multiply-add in the kernel would not reproduce (the CPU table tests show these draws
+# tell the two apart)
+_ALPHAS = [1.0, -2.0, 0.5, 0.3, -1.7]
+_BETAS = [0.0, 1.0, -0.25, 0.7, 1.0 / 3.0]
+
+
+def _expected(ref64, c0, beta, dtype):
+    """β·C + ref in FP64 with separately rounded terms; FP32 data is promoted by the
+    Float64 scalars and rounded once when stored."""
+    c = c0.astype(np.float64)
+    out = beta * c + ref64 if beta != 0.0 else ref64
+    return out.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
 @pytest.mark.parametrize("seed", range(20))
-def test_random_u1_permutes(T, seed):
+def test_random_u1_permutes(T, seed, dtype):
+    import torch
     rng = np.random.default_rng(1000 + seed)
     n1, n2 = int(rng.integers(1, 4)), int(rng.integers(0, 4))
     W = HomSpace(ProductSpace(*[random_u1_space(rng, maxdeg=7) for _ in range(n1)]),
@@ -44,16 +62,15 @@ def test_random_u1_permutes(T, seed):
     if W.dim() == 0:
         pytest.skip("empty")
     p1, p2 = random_perm(rng, n1, n2)
-    A = T.TensorMap.randn(W, rng)
-    x = A.numpy()
-    alpha = float(rng.choice([1.0, -2.0, 0.5]))
-    beta = float(rng.choice([0.0, 1.0, -0.25]))
-    C = T.TensorMap.randn(W.permute(p1, p2), rng)
+    tdt = getattr(torch, dtype)
+    A = T.TensorMap.randn(W, rng, dtype=tdt)
+    x = A.numpy().astype(np.float64)
+    alpha = _ALPHAS[seed % 5]
+    beta = _BETAS[(seed // 5 + seed) % 5]
+    C = T.TensorMap.randn(W.permute(p1, p2), rng, dtype=tdt)
     c0 = C.numpy()
     T.permute_(C, A, p1, p2, alpha, beta)
-    ref = _ref_permute(W, x, p1, p2, alpha)
-    if beta != 0.0:
-        ref = beta * c0 + ref
+    ref = _expected(_ref_permute(W, x, p1, p2, alpha), c0, beta, np.dtype(dtype))
     np.testing.assert_array_equal(C.numpy(), ref)
 
 
@@ -118,6 +135,13 @@ def test_fermionic_permute(T, seed):
     A = T.TensorMap.randn(W, rng)
     B = T.permute(A, p1, p2, copy=True)
     np.testing.assert_array_equal(B.numpy(), _ref_permute(W, A.numpy(), p1, p2))
+    # with scalars: C = β·C + α·permute(A), signs ±1 folded into α·coeff
+    alpha, beta = ((0.3, 0.7), (-1.7, 1.0 / 3.0), (2.0, 1.0))[seed % 3]
+    C = T.TensorMap.randn(B.space, rng)
+    c0 = C.numpy()
+    T.permute_(C, A, p1, p2, alpha, beta)
+    ref = _expected(_ref_permute(W, A.numpy(), p1, p2, alpha), c0, beta, np.float64)
+    np.testing.assert_array_equal(C.numpy(), ref)
 
 
 def test_scale_and_add(T):
@@ -180,19 +204,40 @@ def test_su2_permute_vs_c_loop_cfg3_scaled(T, div):
     plans whose transpose tiles take slices along a third dim (÷16: 1190 tiles against
     5268 without slices; ÷4: 20 320 against 84 288), with partial slices at the ends of
     the 6- and 24-wide dims; the first permutation has none."""
+    import ctypes
+    import torch
     from tkmi.spaces import SU2Space
+    from tkmi import _lib, chainrules as CR
     from oracle import kernels as OK
     V = SU2Space(*((j, n // div) for j, n in ((0, 64), (0.5, 96), (1, 96), (1.5, 64), (2, 32))))
     W = HomSpace(ProductSpace(V, V), ProductSpace(V, V))
     rng = np.random.default_rng(2)
     A = T.TensorMap.randn(W, rng)
     x = A.numpy()
-    for p1, p2 in [((0, 2), (1, 3)), ((3, 1), (2, 0)), ((1, 0, 3), (2,))]:
+    A32 = T.TensorMap.from_numpy(x, W, torch.float32)
+    x32 = A32.numpy().astype(np.float64)
+    sliced_bound = {16: 1190, 4: 20320}[div]
+    for n, (p1, p2) in enumerate([((0, 2), (1, 3)), ((3, 1), (2, 0)), ((1, 0, 3), (2,))]):
         B = T.permute(A, p1, p2)
         plan = T.treebraider(B.space, W, p1, p2)
         y = np.zeros(B.space.dim())
         OK.generic_permute(plan.table, p1 + p2, x, y)
         assert np.linalg.norm(B.numpy() - y) <= 1e-14 * np.linalg.norm(y)
+        if n > 0:
+            # these plans take slices along a third dim (4.4x fewer tiles than without)
+            nt = ctypes.c_int64()
+            _lib.check(_lib.load().tkmi_transform_plan_info(plan.handle, ctypes.byref(nt), None, None), "info")
+            assert nt.value <= sliced_bound, (div, p1, p2, nt.value)
+            # the adjoint exec of the same plan: ⟨d, permute(A)⟩ = ⟨pullback(d), A⟩
+            yb, pb = CR.rrule_permute(A, p1, p2)
+            d = T.TensorMap.randn(yb.space, rng)
+            lhs, rhs = CR.inner(d, yb), CR.inner(pb(d), A)
+            assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs))
+            # FP32 through the same plan (sums in FP64, one rounding at the store)
+            B32 = T.permute(A32, p1, p2)
+            y32 = np.zeros(B.space.dim())
+            OK.generic_permute(plan.table, p1 + p2, x32, y32)
+            assert np.linalg.norm(B32.numpy() - y32) <= 1e-7 * np.linalg.norm(y32)
         # round trip
         inv = [list(p1 + p2).index(i) for i in range(4)]
         C = T.permute(B, tuple(inv[:2]), tuple(inv[2:]))

@@ -25,8 +25,35 @@ def _check_table(W, p1, p2, rng, alpha=1.0, beta=0.0):
     ref, ncod, ndom = O.permute(src, cod, dom, p1, p2, kind_of(W), alpha)
     assert (ncod, ndom) == oracle_legs(Wd)
     if beta != 0.0:
+        # numpy rounds β·dst and (α·coeff)·src separately, as Julia's
+        # scale(C, β) + scale(A, α) does in the reference's tensoradd!
         ref = beta * dst0 + ref
     np.testing.assert_array_equal(dst, ref)
+    return dst0, ref
+
+
+def _fused(beta, c, o):
+    """round(β·c + o) with one rounding (a fused multiply-add), exactly."""
+    from fractions import Fraction
+    return np.array([float(Fraction(beta) * Fraction(float(ci)) + Fraction(float(oi))) for ci, oi in zip(c, o)])
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.3, 0.7), (-1.7, 1.0 / 3.0)])
+def test_nondyadic_scalars_are_not_fused(alpha, beta):
+    """β·dst + (α·coeff)·src with β, α not powers of two: the oracle loop (built with
+    -ffp-contract=off) rounds both products, and these draws tell that apart from a
+    fused multiply-add, so a kernel that fuses fails the same check on the GPU."""
+    V, P = cfg1_spaces()
+    W = HomSpace(ProductSpace(V, P), ProductSpace(V))
+    rng = np.random.default_rng(11)
+    Wd = W.permute((2, 0), (1,))
+    dst0, ref = _check_table(W, (2, 0), (1,), rng, alpha, beta)
+    cod, dom = oracle_legs(W)
+    src = np.random.default_rng(11).standard_normal(W.dim())
+    o, _, _ = O.permute(src, cod, dom, (2, 0), (1,), kind_of(W), alpha)
+    assert Wd.dim() == len(o)
+    fused = _fused(beta, dst0, o)
+    assert np.count_nonzero(fused != ref) > 0
 
 
 def test_cfg1_permute_table():
@@ -44,7 +71,8 @@ def test_random_u1_tables(seed):
     if W.dim() == 0:
         pytest.skip("empty space")
     p1, p2 = random_perm(rng, n1, n2)
-    _check_table(W, p1, p2, rng, alpha=float(rng.choice([1.0, -0.5])), beta=float(rng.choice([0.0, 2.0])))
+    alphas, betas = [1.0, -0.5, 0.3, -1.7], [0.0, 2.0, 0.7, 1.0 / 3.0]
+    _check_table(W, p1, p2, rng, alpha=alphas[seed % 4], beta=betas[(seed // 4) % 4])
 
 
 def _fz2u1_space(rng, dual):
@@ -61,7 +89,7 @@ def test_fermionic_tables_koszul(seed):
     if W.dim() == 0:
         pytest.skip("empty space")
     p1, p2 = random_perm(rng, n1, n2)
-    _check_table(W, p1, p2, rng)
+    _check_table(W, p1, p2, rng, *((1.0, 0.0), (0.3, 0.7))[seed % 2])
 
 
 def test_oracle_invariants():
