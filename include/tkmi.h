@@ -260,6 +260,16 @@ int tkmi_truncate_select(int64_t nvals, const double *S, const double *weights, 
  * reverse-mode rules.                                                          */
 int tkmi_dot(int32_t dtype, const void *a, const void *b, int64_t n, double *work, void *stream);
 
+/* result[0] = Σ_i a[i]·b[i], the same kernels and summation order as tkmi_dot (so the
+ * same bits), with no workspace argument and no allocation of any kind.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The partial
+ * sums live in a static device array of 1025 doubles per GPU, kept until the process
+ * ends; any n fits (at most 1024 partials).  Every call on one GPU shares that array:
+ * calls that overlap in time on one GPU (different streams, or host threads) race, so
+ * order them (one stream, or events).  `result` is a caller-owned device double.     */
+int tkmi_dot_noalloc(int32_t dtype, const void *a, const void *b, int64_t n, double *result, void *stream);
+
 /* ---- hipGraph capture of a chain of launches (one sweep step) ------------ */
 int tkmi_graph_begin(void *stream);
 int tkmi_graph_end(void *stream, tkmi_graph *graph);

@@ -33,4 +33,7 @@ int upload(void **dptr, const void *host, size_t bytes);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Per-device set-up of the reductions' static scratch (called by tkmi_init).
+int reduce_setup();
+
 }  // namespace tkmi

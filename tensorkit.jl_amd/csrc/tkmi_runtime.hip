@@ -43,7 +43,7 @@ int tkmi_init(int device) {
     TKMI_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_init: device index out of range");
     TKMI_HIP(hipSetDevice(device));
-    return TKMI_OK;
+    return tkmi::reduce_setup();
 }
 
 int tkmi_alloc(void **ptr, size_t bytes) {

@@ -67,6 +67,7 @@ SIGNATURES = [
     ("tkmi_eigh_exec", C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_truncate_select", C.c_int, [C.c_int64, c_vp, c_vp, C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_dot", C.c_int, [C.c_int32, c_vp, c_vp, C.c_int64, c_vp, c_vp]),
+    ("tkmi_dot_noalloc", C.c_int, [C.c_int32, c_vp, c_vp, C.c_int64, c_vp, c_vp]),
     ("tkmi_graph_begin", C.c_int, [c_vp]),
     ("tkmi_graph_end", C.c_int, [c_vp, C.POINTER(c_vp)]),
     ("tkmi_graph_launch", C.c_int, [c_vp, c_vp]),
