@@ -90,3 +90,14 @@ def random_perm(rng, n1, n2):
     p = list(rng.permutation(n1 + n2))
     k = int(rng.integers(0, n1 + n2 + 1))
     return tuple(int(x) for x in p[:k]), tuple(int(x) for x in p[k:])
+
+
+def dense_heff(L, x, M1, M2, R):
+    """Dense two-site H_eff application y[a s t; b] = L[a w A]·x[A S T B]·M1[w s S v]
+    ·M2[v t T u]·R[B u b] as pairwise BLAS contractions (the same sums as
+    np.einsum("awA,ASTB,wsSv,vtTu,Bub->astb"), without its unblocked loops)."""
+    t1 = np.tensordot(L, x, axes=([2], [0]))            # a w S T B
+    t2 = np.tensordot(t1, M1, axes=([1, 2], [0, 2]))    # a T B s v
+    t3 = np.tensordot(t2, M2, axes=([1, 4], [2, 0]))    # a B s t u
+    t4 = np.tensordot(t3, R, axes=([1, 4], [0, 1]))     # a s t b
+    return t4

@@ -8,7 +8,7 @@ import torch
 from tkmi.spaces import HomSpace, ProductSpace, U1Space
 
 from oracle import abelian as O
-from helpers import cfg1_spaces, cfg2_spaces, oracle_legs, random_u1_space
+from helpers import cfg1_spaces, cfg2_spaces, oracle_legs, random_u1_space, dense_heff
 
 pytestmark = pytest.mark.gpu
 
@@ -76,8 +76,7 @@ def test_heff_chain_vs_dense(T):
     V, P, W = two_site_spaces(D=96, qmax=4)
     H = TwoSiteHeff(V, P, W, np.random.default_rng(1))
     y = H.apply()
-    ref = np.einsum("awA,ASTB,wsSv,vtTu,Bub->astb", dense(H.L), dense(H.x), dense(H.M1), dense(H.M2),
-                    dense(H.R), optimize=True)
+    ref = dense_heff(dense(H.L), dense(H.x), dense(H.M1), dense(H.M2), dense(H.R))
     assert relerr(y.numpy(), undense(ref, y.space)) < TOL
 
 
