@@ -385,6 +385,274 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
     gemm_k4_body<TC, BM, BN, BK, 3>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
 }
 
+// --------------------------------------------------------------------------------
+// Persistent FP64 / FP32 kernel.  The grid is the device's workgroup slots (occupancy ×
+// CUs); workgroup w computes the tiles sched[slot_start[w] .. slot_start[w+1]) in order, a
+// longest-processing-time assignment of every tile to a slot made on the host, so the
+// launch has no tail of half-empty rounds and no workgroup starts cold.  The (tile, K
+// slab) steps of a workgroup form one pipeline: the global loads of the next step — the
+// next slab of this tile or the first slab of the next tile — are issued before the
+// MFMAs of the current step, so a tile's first slab arrives while the previous tile is
+// still being multiplied.  One LDS buffer, the next step waits in registers (two
+// barriers per step: the staging of gemm_k4s_kernel, four workgroups per CU at 64×64).
+// Measured against gemm_k4s_kernel (64×64×16, FP64): 58 against 63 TFLOP/s at 4096³,
+// 48 against 49 on 16 sectors of 2048×2048×290, 107 against 100 µs on the two-site H_eff
+// L·x launch: the host schedule removes the tail and the prefetch the cold start of each
+// tile, but neither is what bounds these launches, so the autotuner keeps gemm_k4s_kernel.
+// --------------------------------------------------------------------------------
+template <typename TC, int BM, int BN, int BK>
+__device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, int64_t ngroups,
+                                             const int32_t *__restrict__ sched,
+                                             const int32_t *__restrict__ slot_start, const double *__restrict__ A,
+                                             const double *__restrict__ Bm, double *__restrict__ C,
+                                             double *__restrict__ ws, double alpha, double beta) {
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int AE = BM * BK / NT;
+    constexpr int BE = BN * BK / NT;
+    constexpr int KP = BK + 16 / (int)sizeof(TC);
+    __shared__ __attribute__((aligned(16))) TC As[1][BM][KP];
+    __shared__ __attribute__((aligned(16))) TC Bs[1][BN][KP];
+    using acc_t = typename MfmaK4<TC>::acc_t;
+
+    const int32_t s0 = slot_start[blockIdx.x], s1 = slot_start[blockIdx.x + 1];
+    if (s0 >= s1) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    // the tile of a step: its group's fields (copied by value: read through the
+    // restrict-qualified table they stay scalar loads, while a pointer kept across steps
+    // would turn them into vector loads whose waits also wait for the prefetch), its
+    // corner, K range [kbeg, K) and slab count (a K = 0 group runs one all-zero slab, so
+    // its epilogue still applies β)
+    struct Tile {
+        int64_t m, n, a_off, lda, b_off, ldb, c_off, ldc, ws_off;
+        int ta, tb;
+        bool partial;
+        int64_t row0, col0, kbeg, K, z;
+        int nk;
+        bool full;
+    };
+    auto decode = [&](int32_t pos) {
+        Tile t;
+        const int64_t tile = sched[pos];
+        const int gi = find_group(groups, ngroups, tile);
+        const int64_t lt = tile - groups[gi].tile_start;
+        const int64_t tiles_n = groups[gi].tiles_n;
+        const int64_t tiles_mn = groups[gi].tiles_m * tiles_n;
+        t.m = groups[gi].m;
+        t.n = groups[gi].n;
+        t.a_off = groups[gi].a_off;
+        t.lda = groups[gi].lda;
+        t.b_off = groups[gi].b_off;
+        t.ldb = groups[gi].ldb;
+        t.c_off = groups[gi].c_off;
+        t.ldc = groups[gi].ldc;
+        t.ws_off = groups[gi].ws_off;
+        t.ta = groups[gi].ta;
+        t.tb = groups[gi].tb;
+        t.partial = groups[gi].splits > 1;
+        const int64_t kchunk = groups[gi].kchunk;
+        t.z = lt / tiles_mn;
+        const int64_t lmn = lt - t.z * tiles_mn;
+        t.row0 = (lmn / tiles_n) * BM;
+        t.col0 = (lmn % tiles_n) * BN;
+        t.kbeg = t.z * kchunk;
+        t.K = min(groups[gi].k, t.kbeg + kchunk);
+        t.nk = (int)max((int64_t)1, (t.K - t.kbeg + BK - 1) / BK);
+        t.full = (t.row0 + BM <= t.m) && (t.col0 + BN <= t.n);
+        return t;
+    };
+    // element e of this thread: (row i, k) of the A slab, (k, column j) of the B slab;
+    // global order is coalesced (contiguous index fastest), the LDS position [row][k]
+    auto a_el = [&](int ta, int e, int &i, int &k) {
+        const int idx = e * NT + tid;
+        if (!ta) { i = idx % BM; k = idx / BM; }
+        else     { k = idx % BK; i = idx / BK; }
+    };
+    auto b_el = [&](int tb, int e, int &j, int &k) {
+        const int idx = e * NT + tid;
+        if (!tb) { k = idx % BK; j = idx / BK; }
+        else     { j = idx % BN; k = idx / BN; }
+    };
+
+    // load side: the tile whose slabs are being fetched, its per-thread byte offsets from
+    // the slab base (set once per tile) and the registers the slab lands in; addresses
+    // are a scalar slab base plus a 32-bit offset, and interior slabs skip every bounds
+    // test (as in gemm_k4s_kernel)
+    Tile L;
+    int32_t lpos = s0;
+    int lkt = 0;
+    uint32_t aoff[AE], boff[BE];
+    auto set_offsets = [&]() {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, k;
+            a_el(L.ta, e, i, k);
+            aoff[e] = (uint32_t)(sizeof(double) * (L.ta ? (k + i * L.lda) : (i + k * L.lda)));
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int j, k;
+            b_el(L.tb, e, j, k);
+            boff[e] = (uint32_t)(sizeof(double) * (L.tb ? (j + k * L.ldb) : (k + j * L.ldb)));
+        }
+    };
+    auto ld_at = [](const double *base, uint32_t off) {
+        return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    };
+    double ra[AE], rb[BE];
+    auto load_step = [&]() {
+        const int64_t k0 = L.kbeg + (int64_t)lkt * BK;
+        const double *Aslab = A + L.a_off + (L.ta ? L.row0 * L.lda + k0 : L.row0 + k0 * L.lda);
+        const double *Bslab = Bm + L.b_off + (L.tb ? L.col0 + k0 * L.ldb : L.col0 * L.ldb + k0);
+        if (L.full && k0 + BK <= L.K) {
+#pragma unroll
+            for (int e = 0; e < AE; ++e) ra[e] = ld_at(Aslab, aoff[e]);
+#pragma unroll
+            for (int e = 0; e < BE; ++e) rb[e] = ld_at(Bslab, boff[e]);
+        } else {
+            const int ilim = (int)min((int64_t)BM, L.m - L.row0), jlim = (int)min((int64_t)BN, L.n - L.col0);
+            const int klim = (int)max((int64_t)0, min((int64_t)BK, L.K - k0));
+#pragma unroll
+            for (int e = 0; e < AE; ++e) {
+                int i, k;
+                a_el(L.ta, e, i, k);
+                ra[e] = (i < ilim && k < klim) ? ld_at(Aslab, aoff[e]) : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < BE; ++e) {
+                int j, k;
+                b_el(L.tb, e, j, k);
+                rb[e] = (j < jlim && k < klim) ? ld_at(Bslab, boff[e]) : 0.0;
+            }
+        }
+        ++lkt;
+    };
+    // the registers hold a step of tile T: stored with T's operand layouts
+    auto store_step = [&](int buf, const Tile &Tt) {
+        const int ta = Tt.ta, tb = Tt.tb;
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, k;
+            a_el(ta, e, i, k);
+            As[buf][i][k] = (TC)ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int j, k;
+            b_el(tb, e, j, k);
+            Bs[buf][j][k] = (TC)rb[e];
+        }
+    };
+    // MFMA side: the tile being multiplied
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    constexpr int KV = 16 / (int)sizeof(TC), KC = 4 * KV;
+    static_assert(BK % KC == 0, "K slab must hold whole fragment rounds");
+    const int kq = KV * (lane >> 4);
+    // One MFMA path for every tile: the fragments of an edge tile that lie outside C
+    // multiply the zeros its masked loads left in LDS, and the epilogue stores nothing
+    // outside C.  (Skipping them needs a second copy of the step loop, and the merge of
+    // its accumulator registers with the first costs a copy of every accumulator behind
+    // the last MFMA of each step.)
+    auto mma_slab = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += KC) {
+            typedef TC vk_t __attribute__((ext_vector_type(KV)));
+            vk_t af[FM], bfr[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+                af[i] = *reinterpret_cast<const vk_t *>(&As[buf][wm * WM + i * 16 + (lane & 15)][kk + kq]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                bfr[j] = *reinterpret_cast<const vk_t *>(&Bs[buf][wn * WN + j * 16 + (lane & 15)][kk + kq]);
+#pragma unroll
+            for (int s = 0; s < KV; ++s)
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i][s], bfr[j][s], acc[i][j]);
+        }
+    };
+    auto epilogue = [&](const Tile &M) {
+        const int64_t m = M.m, n = M.n;
+        const bool partial = M.partial;
+        double *Cg = partial ? ws + M.ws_off + M.z * m * n : C + M.c_off;
+        const int64_t ldc = partial ? m : M.ldc;
+        const bool has_beta = !partial && (beta != 0.0);
+        const double a_ = partial ? 1.0 : alpha;
+        // the tile's corner on the scalar side, rows and columns inside it 32-bit
+        double *Ct = Cg + M.row0 + M.col0 * ldc;
+        const int rlim = (int)min((int64_t)BM, m - M.row0), clim = (int)min((int64_t)BN, n - M.col0);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int col = wn * WN + j * 16 + (lane & 15);
+                if (col < clim) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
+                        if (row < rlim) {
+                            const double r = a_ * (double)acc[i][j][v];
+                            double *p = Ct + row + (int64_t)col * ldc;
+                            *p = has_beta ? r + beta * *p : r;
+                        }
+                    }
+                }
+                acc[i][j] = acc_t{0, 0, 0, 0};
+            }
+    };
+
+    L = decode(s0);
+    set_offsets();
+    load_step();  // the first step's slab
+    Tile M = L;
+    int32_t mpos = s0;
+    {
+        // per tile: a tight loop over all but its last slab (the next slab of the same tile
+        // is fetched while the current one is multiplied), then the last slab, during which
+        // the first slab of the next tile is fetched
+        for (bool first = true;; first = false) {
+            for (int kt = 0; kt < M.nk - 1; ++kt) {
+                if (!(first && kt == 0)) __syncthreads();  // every wave is done reading the slab
+                store_step(0, M);
+                __syncthreads();
+                load_step();
+                mma_slab(0);
+            }
+            if (!(first && M.nk == 1)) __syncthreads();
+            store_step(0, M);
+            __syncthreads();
+            const bool more = ++mpos < s1;
+            if (more) {
+                L = decode(mpos);
+                set_offsets();
+                lkt = 0;
+                load_step();
+            }
+            mma_slab(0);
+            epilogue(M);
+            if (!more) break;
+            M = L;
+        }
+    }
+}
+
+template <typename TC, int BM, int BN, int BK>
+__global__ __launch_bounds__(NT, 4) void gemm_pks_kernel(
+    const GGroup *__restrict__ groups, int64_t ngroups, const int32_t *__restrict__ sched,
+    const int32_t *__restrict__ slot_start, const double *__restrict__ A, const double *__restrict__ Bm,
+    double *__restrict__ C, double *__restrict__ ws, double alpha, double beta) {
+    static_assert(BM * BN <= 64 * 64, "four workgroups per CU: 64×64 tiles");
+    gemm_pk_body<TC, BM, BN, BK>(groups, ngroups, sched, slot_start, A, Bm, C, ws, alpha, beta);
+}
+
 // Split-K reduction: one thread per output element of the split groups.
 __global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ rg, int64_t ng, int64_t total,
                                                     const double *__restrict__ ws, double *__restrict__ C,
@@ -797,6 +1065,10 @@ struct TileSet {
     RGroup *d_rgroups = nullptr;
     double *ws = nullptr;  // split-K partial products (plan-owned)
     int64_t split = 1;     // plan-wide split-K factor the schedule model chose
+    // persistent kernel: tiles of slot w are sched[slot_start[w] .. slot_start[w+1])
+    std::vector<int32_t> sched, slot_start;
+    int32_t *d_sched = nullptr, *d_slot_start = nullptr;
+    int64_t nslots = 0;
 };
 
 struct SkinnySet {
@@ -809,11 +1081,12 @@ struct SkinnySet {
 // Tile shapes the autotuner chooses from (BM × BN output tile, BK slab depth; the
 // 3×BF16 kernel always uses BK = 32).
 struct Cfg {
-    int bm, bn, bk, nbuf;  // FP64/FP32 kernel: 2 gemm_k4 (two LDS buffers), 1 gemm_k4s, 3 gemm_k4p
+    int bm, bn, bk, nbuf;  // FP64/FP32 kernel: 2 gemm_k4 (two LDS buffers), 1 gemm_k4s, 3 gemm_k4p, 5 gemm_pks
 };
-constexpr Cfg CFGS[] = {{64, 64, 16, 2}, {128, 128, 16, 2}, {128, 64, 16, 2}, {64, 128, 16, 2},
-                        {64, 64, 32, 2}, {64, 64, 16, 1},  {64, 64, 32, 1},  {128, 128, 16, 1},
-                        {64, 64, 16, 3}};
+// nbuf 5: the persistent kernel with a host-made schedule (gemm_pks_kernel)
+constexpr Cfg CFGS[] = {{64, 64, 16, 2},  {128, 128, 16, 2}, {128, 64, 16, 2}, {64, 128, 16, 2},
+                        {64, 64, 32, 2},  {64, 64, 16, 1},   {64, 64, 32, 1},  {128, 128, 16, 1},
+                        {64, 64, 16, 3},  {64, 64, 16, 5}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
 // of a configuration matter to it, and the prefetch depth of configuration 8
 constexpr bool bf16_duplicate(int c) { return c >= 4 && c != 8; }  // 8: two prefetch stages
@@ -850,7 +1123,10 @@ template <int BM, int BN, int BK, int NBUF>
 int kernel_occupancy() {
     int nb = 0;
     hipError_t e;
-    if constexpr (NBUF == 3)
+    if constexpr (NBUF == 5)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&gemm_pks_kernel<double, BM, BN, BK>), NT, 0);
+    else if constexpr (NBUF == 3)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void *>(&gemm_k4p_kernel<double, BM, BN, BK>), NT, 0);
     else if constexpr (NBUF == 2)
@@ -882,7 +1158,8 @@ int64_t cfg_slots(int c) {
             case 5: occ = kernel_occupancy<64, 64, 16, 1>(); break;
             case 6: occ = kernel_occupancy<64, 64, 32, 1>(); break;
             case 7: occ = kernel_occupancy<128, 128, 16, 1>(); break;
-            default: occ = kernel_occupancy<64, 64, 16, 3>(); break;
+            case 8: occ = kernel_occupancy<64, 64, 16, 3>(); break;
+            default: occ = kernel_occupancy<64, 64, 16, 5>(); break;
         }
         v = occ * cus;
     }
@@ -982,7 +1259,54 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     }
     ts.ntiles = start;
     if (ts.groups.empty()) return TKMI_OK;
-    int rc = tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
+    int rc = TKMI_OK;
+    if (CFGS[c].nbuf == 5) {
+        // Workgroup w runs on XCD w % 8 (workgroups are dealt round-robin to the XCDs).
+        // Chunks of XCH consecutive tiles (one group, neighbouring rows and columns) go to
+        // one XCD, chunk c to XCD c % 8, so they share their A and B panels in that XCD's
+        // L2; within an XCD, longest processing time first: tiles in their heaviest-first
+        // numbering, each to the XCD's slot that frees first (ties: the lowest slot), at
+        // the modelled cost of split_model (K slabs plus a fixed per-tile cost).
+        if (ts.ntiles >= ((int64_t)1 << 31)) TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: too many tiles to schedule");
+        constexpr double TILE_COST = 3.0;
+        constexpr int64_t NX = 8, XCH = 16;
+        const int64_t w = std::max<int64_t>(1, std::min<int64_t>(slots, ts.ntiles));
+        std::vector<std::vector<int32_t>> lists((size_t)w);
+        auto later = [](const std::pair<double, int64_t> &a, const std::pair<double, int64_t> &b) {
+            return a.first > b.first || (a.first == b.first && a.second > b.second);
+        };
+        const int64_t nx = std::min<int64_t>(NX, w);
+        std::vector<std::vector<std::pair<double, int64_t>>> heaps((size_t)nx);
+        for (int64_t i = 0; i < w; ++i) heaps[(size_t)(i % nx)].push_back({0.0, i});
+        for (auto &h : heaps) std::make_heap(h.begin(), h.end(), later);
+        for (const GGroup &g : ts.groups) {
+            const int64_t tmn = g.tiles_m * g.tiles_n;
+            for (int64_t z = 0; z < g.splits; ++z) {
+                const int64_t kz = std::max<int64_t>(0, std::min(g.k, (z + 1) * g.kchunk) - z * g.kchunk);
+                const double cost = (double)std::max<int64_t>(1, (kz + bk - 1) / bk) + TILE_COST;
+                for (int64_t t = 0; t < tmn; ++t) {
+                    const int64_t id = g.tile_start + z * tmn + t;
+                    auto &heap = heaps[(size_t)((id / XCH) % nx)];
+                    std::pop_heap(heap.begin(), heap.end(), later);
+                    heap.back().first += cost;
+                    lists[(size_t)heap.back().second].push_back((int32_t)id);
+                    std::push_heap(heap.begin(), heap.end(), later);
+                }
+            }
+        }
+        ts.sched.clear();
+        ts.slot_start.assign(1, 0);
+        for (auto &l : lists) {
+            ts.sched.insert(ts.sched.end(), l.begin(), l.end());
+            ts.slot_start.push_back((int32_t)ts.sched.size());
+        }
+        ts.nslots = w;
+        rc = tkmi::upload((void **)&ts.d_sched, ts.sched.data(), ts.sched.size() * sizeof(int32_t));
+        if (!rc)
+            rc = tkmi::upload((void **)&ts.d_slot_start, ts.slot_start.data(), ts.slot_start.size() * sizeof(int32_t));
+        if (rc) return rc;
+    }
+    rc = tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
     if (!rc && !ts.rgroups.empty())
         rc = tkmi::upload((void **)&ts.d_rgroups, ts.rgroups.data(), ts.rgroups.size() * sizeof(RGroup));
     if (!rc && ws_len > 0) rc = tkmi::upload((void **)&ts.ws, nullptr, ws_len * sizeof(double));
@@ -1004,7 +1328,15 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
     const int64_t ng = (int64_t)ts.groups.size();
     if (!tiles) {
     } else if (prec != TKMI_PREC_BF16X3) {
-        if constexpr (NBUF == 3) {
+        if constexpr (NBUF == 5) {
+            const dim3 pgrid((unsigned)ts.nslots);
+            if (prec == TKMI_PREC_F64)
+                hipLaunchKernelGGL((gemm_pks_kernel<double, BM, BN, BK>), pgrid, block, 0, s, ts.d_groups, ng,
+                                   ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
+            else
+                hipLaunchKernelGGL((gemm_pks_kernel<float, BM, BN, BK>), pgrid, block, 0, s, ts.d_groups, ng,
+                                   ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
+        } else if constexpr (NBUF == 3) {
             if (prec == TKMI_PREC_F64)
                 hipLaunchKernelGGL((gemm_k4p_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
                                    ts.ntiles, A, B, C, ts.ws, alpha, beta);
@@ -1069,7 +1401,8 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const doub
         case 5: launch_k<64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         case 6: launch_k<64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         case 7: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        default: launch_k<64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 8: launch_k<64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        default: launch_k<64, 64, 16, 5>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
@@ -1115,6 +1448,8 @@ void free_gemm(tkmi_gemm_plan_s *p) {
         (void)hipFree(t.d_groups);
         (void)hipFree(t.d_rgroups);
         (void)hipFree(t.ws);
+        (void)hipFree(t.d_sched);
+        (void)hipFree(t.d_slot_start);
     }
     delete p;
 }
