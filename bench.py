@@ -50,6 +50,38 @@ TRAFFIC_PMC = {
 }
 
 
+def host_cpu_share():
+    """The host cores this process may use and how they are known: scheduler affinity,
+    the cgroup CPU quota (cpu.max), OMP_NUM_THREADS; the thread count used is the smallest
+    of those that are set.  Also the CPU model and nproc, for the record."""
+    import math
+    info = {"nproc": os.cpu_count(), "affinity": None, "cgroup_cpus": None,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "model": None}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_cpus"] = float(quota) / float(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cands = [c for c in (info["affinity"], info["cgroup_cpus"] and int(math.floor(info["cgroup_cpus"])))
+             if c]
+    if info["omp_num_threads_env"] and info["omp_num_threads_env"].isdigit():
+        cands.append(int(info["omp_num_threads_env"]))
+    info["threads_used"] = max(1, min(cands) if cands else (os.cpu_count() or 1))
+    return info
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -249,6 +281,8 @@ def main():
     # rate (50 after 5 read ≈4% slower, the clocks still ramping)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--settle-ms", type=float, default=300.0,
+                    help="untimed replays before the warm-up steps, reported as clock_settle")
     ap.add_argument("--D", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -339,6 +373,17 @@ def main():
 
     # FP64 graph
     y64 = H.capture(precision=_lib.PREC_F64)
+    # Clock settling, reported on its own (not part of `warmup`): the GPU's clocks ramp up
+    # over the first few hundred milliseconds of work, and a short timed window right after
+    # the warm-up steps reads several percent slow.  Untimed replays for at least
+    # args.settle_ms (bounded by a replay count).
+    t_settle, n_settle = time.perf_counter(), 0
+    while args.settle_ms > 0 and (time.perf_counter() - t_settle) * 1e3 < args.settle_ms and n_settle < 20000:
+        for _ in range(50):
+            H.replay()
+        n_settle += 50
+        torch.cuda.synchronize()
+    settle = {"replays": n_settle, "ms": round((time.perf_counter() - t_settle) * 1e3, 1)}
     for _ in range(args.warmup):
         H.replay()
     torch.cuda.synchronize()
@@ -481,6 +526,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "clock_settle": settle,
         "ms_per_step": round(ms64, 5),
         "higher_is_better": True,
         "scaling": "weak",
@@ -515,8 +561,13 @@ def main():
 
     if not args.no_cpu:
         from oracle import replay as _replay  # the CPU baseline leg: the oracle's host replay
-        nthreads = min(16, os.cpu_count() or 1)
-        os.environ.setdefault("OMP_NUM_THREADS", str(nthreads))
+        share = host_cpu_share()
+        nthreads = share["threads_used"]
+        try:
+            from threadpoolctl import threadpool_limits, threadpool_info
+        except ImportError:
+            threadpool_limits = threadpool_info = None
+        blas_ctx = threadpool_limits(limits=nthreads, user_api="blas") if threadpool_limits else None
         init = {}
         for t in [H.L, H.M1, H.M2, H.R, H.x]:
             init[t.data.data_ptr()] = t.numpy().copy()
@@ -528,6 +579,10 @@ def main():
             bufs = _replay.cpu_replay(steps, init, nthreads)
             t_cpu += time.perf_counter() - t0
             reps += 1
+        blas = [{k: d.get(k) for k in ("internal_api", "version", "num_threads", "threading_layer")}
+                for d in (threadpool_info() if threadpool_info else []) if d.get("user_api") == "blas"]
+        if blas_ctx is not None:
+            blas_ctx.restore_original_limits()
         ycpu = bufs[y_eager.data.data_ptr()]
         cpu_err = float(np.linalg.norm(ycpu - y_eager.numpy()) / np.linalg.norm(ycpu))
         cpu_ms = t_cpu / reps * 1e3
@@ -535,16 +590,13 @@ def main():
                                "kind": "port",
                                "sample": f"{reps} full H_eff applications (same kernel sequence) on the host, "
                                          f"{cpu_ms:.1f} ms each; C permute loop with {nthreads} OpenMP threads, "
-                                         f"numpy/OpenBLAS dgemm per sector",
+                                         f"numpy/OpenBLAS dgemm per sector with {nthreads} BLAS threads",
                                "ms_per_step": round(cpu_ms, 3), "gpu_speedup": round(cpu_ms / ms64, 2),
-                               "rel_err_vs_gpu": cpu_err}
+                               "rel_err_vs_gpu": cpu_err,
+                               "host": {**share, "blas": blas}}
         log(f"cpu baseline {cpu_ms:.1f} ms/step, GPU speed-up x{cpu_ms/ms64:.1f}, rel err {cpu_err:.2e}")
         # the reference's own published setting is one thread: the same replay with the
         # permute loop and the host BLAS each on one thread, on a smaller bounded sample
-        try:
-            from threadpoolctl import threadpool_limits
-        except ImportError:
-            threadpool_limits = None
         if threadpool_limits is not None and args.cpu_seconds_single > 0:
             reps1, t1 = 0, 0.0
             with threadpool_limits(limits=1):

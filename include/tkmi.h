@@ -205,6 +205,15 @@ int tkmi_gemm_plan_splitk(tkmi_gemm_plan plan, int32_t config, int64_t *splitk, 
  * `stream`; meant for warm-up, not for capture.                               */
 int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double *A, const double *B,
                             double *C, int32_t reps, void *stream);
+/* 3×BF16 operands that do not change between calls (an environment tensor of a sweep,
+ * constant across a Lanczos or Davidson run): operands = 1 (A), 2 (B), 3 (both), 0 (none).
+ * The split of a constant operand into its three bf16 planes is made by the first
+ * TKMI_PREC_BF16X3 call and reused while later calls pass the same pointer; a call with
+ * another pointer splits it again.  After changing a constant operand's data in place,
+ * call tkmi_gemm_plan_invalidate, or the next call multiplies the old split.  A captured
+ * graph holds only the splits its capture made: make one eager 3×BF16 call first.     */
+int tkmi_gemm_plan_set_constant(tkmi_gemm_plan plan, int32_t operands);
+int tkmi_gemm_plan_invalidate(tkmi_gemm_plan plan);
 /* Configuration index and tile count the next exec with `precision` uses.      */
 int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *config, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);

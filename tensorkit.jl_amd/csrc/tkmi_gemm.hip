@@ -49,11 +49,13 @@ struct GGroup {
 
 // Work of the split/pack pre-pass for one MFMA group: nA tiles of A rows, then the
 // tiles of B columns (each PACK_T rows × PACK_T of K, K fastest), numbered from start.
+// startA / startB number the A tiles and the B tiles on their own, for a pass that splits
+// only one operand (the other one's panels are still valid: see tkmi_gemm_plan_set_constant).
 struct PackGroup {
     int64_t m, n, k, a_off, lda, b_off, ldb;
     int32_t ta, tb;
     int64_t mp, np, kp, pa, pb, sa, sb;
-    int64_t start, nA;
+    int64_t start, nA, startA, startB, nB;
 };
 
 // Split-K reduction record: C = α·Σ_z ws[ws_off + z·m·n + ·] + β·C, fixed z order.
@@ -696,20 +698,25 @@ typedef uint32_t u4_t __attribute__((ext_vector_type(4)));  // 16 bytes = 8 bf16
 
 constexpr int PACK_T = 64;
 
+// which: 0 = the A and the B tiles of every group, 1 = the A tiles only, 2 = the B tiles only
 __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restrict__ pg, int64_t ng,
                                                         const double *__restrict__ A, const double *__restrict__ Bm,
-                                                        __bf16 *__restrict__ P) {
+                                                        __bf16 *__restrict__ P, int which) {
     __shared__ double tile[PACK_T][PACK_T + 1];
     const int64_t q = blockIdx.x;
+    auto first = [&](const PackGroup &g) { return which == 0 ? g.start : (which == 1 ? g.startA : g.startB); };
     int64_t lo = 0, hi = ng - 1;
     while (lo < hi) {
         const int64_t mid = (lo + hi + 1) >> 1;
-        if (pg[mid].start <= q) lo = mid; else hi = mid - 1;
+        if (first(pg[mid]) <= q) lo = mid; else hi = mid - 1;
     }
     const PackGroup &G = pg[lo];
-    int64_t r = q - G.start;
-    const bool isA = r < G.nA;
-    if (!isA) r -= G.nA;
+    int64_t r = q - first(G);
+    bool isA = which == 1;
+    if (which == 0) {
+        isA = r < G.nA;
+        if (!isA) r -= G.nA;
+    }
     const int64_t tk = (G.kp + PACK_T - 1) / PACK_T;
     const int64_t i0 = (r / tk) * PACK_T, k0 = (r % tk) * PACK_T;
     const int64_t lim = isA ? G.m : G.n;
@@ -1098,8 +1105,13 @@ struct tkmi_gemm_plan_s {
     std::vector<GGroup> base;  // MFMA groups without tiling
     // 3×BF16 split panels (allocated by the first 3×BF16 call) and the pack work list
     std::vector<PackGroup> pack;
-    int64_t pack_total = 0, panel_len = 0;
+    int64_t pack_total = 0, pack_total_a = 0, pack_total_b = 0, panel_len = 0;
     PackGroup *d_pack = nullptr;
+    // operands marked constant (bit 0: A, bit 1: B) and the source pointer whose split is in
+    // the panels for each (nullptr: none): a constant operand is split again only when it
+    // comes from another pointer or after tkmi_gemm_plan_invalidate
+    int32_t const_ops = 0;
+    const void *split_src[2] = {nullptr, nullptr};
     __bf16 *panels = nullptr;
     SkinnySet skinny;
     DotSet dots;
@@ -1322,7 +1334,7 @@ int auto_cfg(const tkmi_gemm_plan_s *) {
 int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
 
 template <int BM, int BN, int BK, int NBUF>
-void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const double *A, const double *B, double *C,
+void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const double *A, const double *B, double *C,
               double alpha, double beta, hipStream_t s, bool tiles, bool reduce) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
@@ -1360,8 +1372,17 @@ void launch_k(const tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const doub
         }
     }
     else {
-        hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)p->pack_total), block, 0, s, p->d_pack,
-                           (int64_t)p->pack.size(), A, B, p->panels);
+        // split the operands whose panels do not already hold them (constant operands from
+        // the same pointer as the last split keep theirs)
+        const bool keepA = (p->const_ops & 1) && p->split_src[0] == A;
+        const bool keepB = (p->const_ops & 2) && p->split_src[1] == B;
+        const int which = !keepA && !keepB ? 0 : (!keepA ? 1 : (!keepB ? 2 : -1));
+        const int64_t npack = which == 0 ? p->pack_total : (which == 1 ? p->pack_total_a : p->pack_total_b);
+        if (which >= 0 && npack > 0)
+            hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)npack), block, 0, s, p->d_pack,
+                               (int64_t)p->pack.size(), A, B, p->panels, which);
+        p->split_src[0] = (p->const_ops & 1) ? (const void *)A : nullptr;
+        p->split_src[1] = (p->const_ops & 2) ? (const void *)B : nullptr;
         if constexpr (NBUF == 3)
             hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 2>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
                                (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
@@ -1583,7 +1604,12 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         const int64_t tk = (G.kp + PACK_T - 1) / PACK_T;  // K tiles of the pack pass
         P.start = p->pack_total;
         P.nA = (mp / PACK_T) * tk;
+        P.nB = (np / PACK_T) * tk;
+        P.startA = p->pack_total_a;
+        P.startB = p->pack_total_b;
         p->pack_total += ((mp + np) / PACK_T) * tk;
+        p->pack_total_a += P.nA;
+        p->pack_total_b += P.nB;
         p->pack.push_back(P);
     }
     // C blocks of each part for α = 0 calls
@@ -1704,6 +1730,21 @@ int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     plan->choice[precision] = best_c;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_set_constant(tkmi_gemm_plan plan, int32_t operands) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (operands < 0 || operands > 3) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: operands is a mask of 1 (A) and 2 (B)");
+    plan->const_ops = operands;
+    if (!(operands & 1)) plan->split_src[0] = nullptr;
+    if (!(operands & 2)) plan->split_src[1] = nullptr;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_invalidate(tkmi_gemm_plan plan) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    plan->split_src[0] = plan->split_src[1] = nullptr;
     return TKMI_OK;
 }
 

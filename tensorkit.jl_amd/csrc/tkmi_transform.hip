@@ -80,6 +80,17 @@ constexpr int class_area(int L) { return NTHREADS * class_elems(L); }
 constexpr int class_twmax(int L) { return L <= 2 ? 256 : (L == 4 ? 128 : 64); }
 constexpr int class_L(int c) { return 1 << c; }
 
+// β·c + v with β·c and the sum rounded separately, as the reference's tensoradd!
+// (scale(C, β) + scale(A, α): Julia does not fuse a multiply and an add).  The empty asm
+// statement takes the rounded product as a register operand, so the compiler cannot fuse
+// the multiply into the add; it emits no instruction.  (A `#pragma clang fp contract(off)`
+// here switched contraction off in the whole calling kernel, the recoupling sums too.)
+__device__ __forceinline__ double scaled_sum(double beta, double c, double v) {
+    double bc = beta * c;
+    asm volatile("" : "+v"(bc));
+    return bc + v;
+}
+
 // One tile of one work item.  COPY: the tile is nr runs of nw elements along dW
 // (contiguous on both sides), numbered run after run; otherwise a tW × tR tile
 // read with lanes along dR and written with lanes along dW through LDS.
@@ -280,18 +291,15 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
         }
     }
 
+    // several source trees: α scales the recoupling sum (α = 1 leaves it as it is)
+    if (!single && alpha != 1.0) {
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+#pragma unroll
+            for (int k = 0; k < E; ++k) o[j][k] = alpha * o[j][k];
+    }
     const bool has_beta = (beta != 0.0);
-    // dst = β·dst + v with β·dst and v rounded separately, as the reference's
-    // tensoradd! (scale(C, β) + scale(A, α), no fused multiply-add in Julia)
-    auto combine = [&](double v, double c) {
-#pragma clang fp contract(off)
-        if (!single) v = alpha * v;
-        if (has_beta) {
-            const double bc = beta * c;
-            v = bc + v;
-        }
-        return v;
-    };
+    auto with_beta = [&](double v, double c) { return scaled_sum(beta, c, v); };
     auto store_all = [&](auto store) {
 #pragma unroll
         for (int j = 0; j < L; ++j) {
@@ -300,12 +308,23 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 T *dp = dst + corner(tw);
                 const int64_t sW = tw.str[dW], sR = COPY ? tw.str[dR] - W * tw.str[dW] : tw.str[dR];
                 const int64_t sB = dB >= 0 ? tw.str[dB] : 0;
+                if (has_beta) {
 #pragma unroll
-                for (int k = 0; k < E; ++k) {
-                    int w, r, b;
-                    if (pos_write(k, w, r, b)) {
-                        T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
-                        store((T)combine(o[j][k], has_beta ? (double)*a : 0.0), a);
+                    for (int k = 0; k < E; ++k) {
+                        int w, r, b;
+                        if (pos_write(k, w, r, b)) {
+                            T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
+                            store((T)with_beta(o[j][k], (double)*a), a);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < E; ++k) {
+                        int w, r, b;
+                        if (pos_write(k, w, r, b)) {
+                            T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
+                            store((T)o[j][k], a);
+                        }
                     }
                 }
             }

@@ -55,6 +55,8 @@ SIGNATURES = [
     ("tkmi_gemm_plan_splitk", C.c_int, [c_vp, C.c_int32, c_i64p, c_i64p]),
     ("tkmi_gemm_plan_query", C.c_int, [c_vp, C.c_int32, c_i32p, c_i64p]),
     ("tkmi_gemm_plan_autotune", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, c_vp, C.c_int32, c_vp]),
+    ("tkmi_gemm_plan_set_constant", C.c_int, [c_vp, C.c_int32]),
+    ("tkmi_gemm_plan_invalidate", C.c_int, [c_vp]),
     ("tkmi_gemm_plan_destroy", C.c_int, [c_vp]),
     ("tkmi_transform_plan_set_gather", C.c_int, [c_vp, C.c_int64, c_vp, c_vp]),
     ("tkmi_transform_exec_gather", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, c_vp, C.c_double, C.c_double, c_vp]),

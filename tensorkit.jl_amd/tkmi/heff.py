@@ -102,10 +102,44 @@ class TwoSiteHeff:
     def _plans(self):
         return list(T._GEMM_CACHE.d.values())
 
+    def mark_constant_operands(self, precision=None):
+        """The environments L and R stay fixed while x changes (a Lanczos or Davidson run):
+        mark them as constant operands of the grouped GEMMs that read them, so the 3×BF16
+        form splits them into bf16 planes once (tkmi_gemm_plan_set_constant).  Call
+        `invalidate_operands` after changing L or R in place."""
+        saved = T._TRACE
+        T.start_trace()
+        try:
+            self.apply(precision=precision)
+        finally:
+            tr = T.stop_trace()
+            T._TRACE = saved
+        lib = T._lib.load()
+        envs = {self.L.data.data_ptr(), self.R.data.data_ptr()}
+        self._const_plans = []
+        for op in tr:
+            if op[0] != "gemm":
+                continue
+            plan, (ta, _), (tb, _) = op[1], op[3], op[4]
+            mask = (1 if ta.data_ptr() in envs else 0) | (2 if tb.data_ptr() in envs else 0)
+            if mask:
+                T._lib.check(lib.tkmi_gemm_plan_set_constant(plan.handle, mask), "set constant")
+                self._const_plans.append(plan)
+
+    def invalidate_operands(self):
+        """L or R changed in place: the next 3×BF16 application splits them again."""
+        lib = T._lib.load()
+        for plan in getattr(self, "_const_plans", []):
+            T._lib.check(lib.tkmi_gemm_plan_invalidate(plan.handle), "invalidate")
+
     def capture(self, precision=None):
         """Capture the whole chain in one HIP graph (intermediates live in the
-        graph's private pool).  Returns the output tensor the graph writes."""
-        self.apply(precision=precision)  # build plans / warm up
+        graph's private pool).  Returns the output tensor the graph writes.  In the
+        3×BF16 form the constant environments are split once, before the capture, and the
+        graph splits only the operands that change."""
+        if precision == T._lib.PREC_BF16X3:
+            self.mark_constant_operands(precision)
+        self.apply(precision=precision)  # build plans / warm up (splits the constants)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()

@@ -297,3 +297,66 @@ def test_dot_groups(lib, ta, tb):
     assert np.array_equal(o1, o2)
     ref = _ref(tab, Af, Bf, Cm, 1.0, 0.0)
     assert np.linalg.norm(o1 - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_bf16x3_constant_operands(lib):
+    """tkmi_gemm_plan_set_constant: a constant operand is split into bf16 planes once and
+    reused while later calls pass the same pointer — a change made in place without
+    tkmi_gemm_plan_invalidate is not seen (the reuse is real), invalidate or another
+    pointer splits again — and the other operand is split at every call."""
+    rng = np.random.default_rng(31)
+    shapes = [(300, 280, 290), (130, 70, 33), (64, 64, 64)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0], [0, 0, 1])
+    arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+    cols = [lib.i64(arr[:, i]) for i in range(11)]
+    ta, tb = lib.i32(arr[:, 5]), lib.i32(arr[:, 8])
+    L = lib.load()
+    h = C.c_void_p()
+    lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                      cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1], cols[10][1]),
+              "plan")
+    s = torch.cuda.current_stream().cuda_stream
+    Z = np.zeros(nc)
+
+    def run(dA, dB):
+        dC = torch.zeros(nc, dtype=torch.float64, device="cuda")
+        lib.check(L.tkmi_gemm_exec(h, 2, 1.0, 0.0, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s), "exec")
+        return dC.cpu().numpy()
+
+    def close(out, A, B):
+        ref = _ref(tab, A, B, Z, 1.0, 0.0)
+        return np.linalg.norm(out - ref) <= 1e-6 * np.linalg.norm(ref)
+
+    try:
+        for mask, which in ((1, "A"), (2, "B")):
+            A0, B0 = rng.standard_normal(na), rng.standard_normal(nb)
+            dA, dB = torch.from_numpy(A0).cuda(), torch.from_numpy(B0).cuda()
+            lib.check(L.tkmi_gemm_plan_set_constant(h, mask), "constant")
+            first = run(dA, dB)
+            assert close(first, A0, B0)
+            # change the constant operand in place: the old split is reused, bit for bit
+            X1 = rng.standard_normal(na if which == "A" else nb)
+            (dA if which == "A" else dB).copy_(torch.from_numpy(X1))
+            stale = run(dA, dB)
+            np.testing.assert_array_equal(stale, first)
+            A1, B1 = (X1, B0) if which == "A" else (A0, X1)
+            lib.check(L.tkmi_gemm_plan_invalidate(h), "invalidate")
+            assert close(run(dA, dB), A1, B1)
+            # the other operand is split at every call
+            Y2 = rng.standard_normal(nb if which == "A" else na)
+            (dB if which == "A" else dA).copy_(torch.from_numpy(Y2))
+            A2, B2 = (A1, Y2) if which == "A" else (Y2, B1)
+            assert close(run(dA, dB), A2, B2)
+            # another pointer for the constant operand: split again
+            X3 = rng.standard_normal(na if which == "A" else nb)
+            d3 = torch.from_numpy(X3).cuda()
+            out3 = run(d3, dB) if which == "A" else run(dA, d3)
+            assert close(out3, *((X3, B2) if which == "A" else (A2, X3)))
+        # nothing constant: an in-place change is seen at once
+        lib.check(L.tkmi_gemm_plan_set_constant(h, 0), "constant")
+        A4 = rng.standard_normal(na)
+        dA.copy_(torch.from_numpy(A4))
+        assert close(run(dA, dB), A4, dB.cpu().numpy())
+        assert L.tkmi_gemm_plan_set_constant(h, 4) == 1
+    finally:
+        L.tkmi_gemm_plan_destroy(h)
