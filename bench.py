@@ -7,8 +7,10 @@ captured in one HIP graph), FP64 — the drop-in arithmetic.
 Reported:
   value        contract GFLOP/s = Σ_c 2·m·n·k of the chain's grouped GEMMs / step time
   permute      aggregate GB/s of the chain's permutes (16 B/element, 24 with β≠0)
-  precision    FP32-MFMA and 3×BF16 split forms: step time, speed-up over FP64 and
-               relative Frobenius error of y against the FP64 result
+  precision    reduced-precision forms: step time, speed-up over FP64 and relative
+               Frobenius error of y against the FP64 result — 3×BF16 split GEMMs on the
+               FP64 chain, and the float32 chain (every intermediate FP32) with FP32-MFMA
+               GEMMs (a Float32 mul!) or 3×BF16 split GEMMs
   roofline     dominant kernel (FP64 grouped GEMM) achieved TFLOP/s vs 78.6 peak,
                from HIP events around each of its launches on the launch stream
   cpu_baseline the oracle's CPU restatement (table-driven C permute loop + numpy/
@@ -401,28 +403,38 @@ def main():
     log(f"FP64 step {ms64:.4f} ms (wall {wall/args.steps*1e3:.4f} ms), {flops/ms64/1e6:.1f} GFLOP/s")
     eager_err = float(np.abs(y_ref - y_eager.numpy()).max())
 
-    # reduced-precision forms (separate graphs)
+    # reduced-precision forms (separate graphs): the 3×BF16 GEMMs on the FP64 chain, then the
+    # float32 chain (operands and x rounded to FP32 once, every intermediate FP32) with FP32
+    # MFMA GEMMs (a Float32 mul!) and with 3×BF16 GEMMs, its GEMMs tuned on FP32 data
     prec_report = {}
-    for name, prec in (("fp32", _lib.PREC_F32), ("bf16x3", _lib.PREC_BF16X3)):
-        H2 = TwoSiteHeff.__new__(TwoSiteHeff)
-        H2.__dict__.update({k: v for k, v in H.__dict__.items() if k not in ("graph", "outs")})
-        H2.outs = [None] * 4
-        H2.graph = None
-        H2.capture(precision=prec)
+
+    def time_form(name, Hx, prec, data):
+        Hx.capture(precision=prec)
         for _ in range(args.warmup):
-            H2.replay()
+            Hx.replay()
         torch.cuda.synchronize()
         e0.record(st)
         for _ in range(args.steps):
-            H2.replay()
+            Hx.replay()
         e1.record(st)
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.steps
-        y = H2.graph_y.numpy()
+        y = Hx.graph_y.numpy().astype(np.float64)
         err = float(np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref))
-        prec_report[name] = {"ms_per_step": round(ms, 5), "gflops": round(flops / ms / 1e6, 1),
+        prec_report[name] = {"data": data, "ms_per_step": round(ms, 5), "gflops": round(flops / ms / 1e6, 1),
                              "speedup_vs_fp64": round(ms64 / ms, 3), "rel_frobenius_err_vs_fp64": err}
         log(f"{name}: {ms:.4f} ms, x{ms64/ms:.2f} vs FP64, rel err {err:.2e}")
+        return Hx
+
+    H2 = H.astype(torch.float64)
+    time_form("bf16x3_f64data", H2, _lib.PREC_BF16X3, "float64")
+    H32 = H.astype(torch.float32)
+    if not (args.counter_workload and args.gemm_configs):
+        T.start_trace()
+        H32.apply(precision=_lib.PREC_BF16X3)
+        T.autotune_trace(T.stop_trace(), precisions=(_lib.PREC_F32, _lib.PREC_BF16X3))
+    time_form("fp32", H32, _lib.PREC_F32, "float32")
+    H32b = time_form("bf16x3", H32, _lib.PREC_BF16X3, "float32")
 
     # svd_trunc of the two-site result (truncrank 1024 ∩ trunctol 1e-10), FP64 on the device,
     # for the FP64 result and for the 3×BF16 result: discarded-weight difference
@@ -437,7 +449,7 @@ def main():
         torch.cuda.synchronize()
         svd_ms = (time.perf_counter() - t0) * 1e3
         kept = sum(int(v.numel()) for v in Sk.values())
-        ybf = T.permute(H2.graph_y, (0, 1), (2, 3), copy=True)
+        ybf = T.permute(T.TensorMap(H32b.graph_y.data.double(), H32b.graph_y.space), (0, 1), (2, 3), copy=True)
         _, Skb, _, epsbf = svd_trunc(ybf, 1024, 1e-10)
         svd_report = {"ms": round(svd_ms, 2), "algorithm": DEFAULT_SVD_ALGORITHM,
                       "block_shapes": sorted({(d1, d2) for _, (d1, d2), _ in y64t.space.degeneracystructure()[1]},

@@ -191,6 +191,15 @@ int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk,
  * the split groups of C without their full sums.                              */
 int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alpha,
                          double beta, const double *A, const double *B, double *C, void *stream);
+/* tkmi_gemm_exec_parts for the element type `dtype` of A, B and C (all three):
+ * TKMI_F64, or TKMI_F32 for the blocks of Float32 TensorMaps.  FP32 data runs in
+ * TKMI_PREC_F32 (FP32 MFMA: the sgemm of a Float32 mul!, src/tensors/linalg.jl:330-370)
+ * or TKMI_PREC_BF16X3 (a float splits exactly into its three bf16 terms); each element of
+ * C is formed in FP64, α·(A·B) + β·C, and rounded once to FP32.  Split-K partials and dot
+ * partials stay FP64.  TKMI_PREC_F64 with TKMI_F32 returns TKMI_ERR_UNSUPPORTED.      */
+int tkmi_gemm_exec_dtype(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, int32_t parts,
+                         double alpha, double beta, const void *A, const void *B, void *C,
+                         void *stream);
 /* Algorithmic flops routed to the MFMA kernel and to the skinny kernel.       */
 int tkmi_gemm_plan_split(tkmi_gemm_plan plan, double *flops_mfma, double *flops_skinny,
                          int64_t *skinny_groups);
@@ -205,6 +214,12 @@ int tkmi_gemm_plan_splitk(tkmi_gemm_plan plan, int32_t config, int64_t *splitk, 
  * `stream`; meant for warm-up, not for capture.                               */
 int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double *A, const double *B,
                             double *C, int32_t reps, void *stream);
+/* The autotuner on operands of element type `dtype` (see tkmi_gemm_exec_dtype).  The
+ * choice is kept per precision form, for every element type: tune with the data the
+ * plan runs on.                                                                 */
+int tkmi_gemm_plan_autotune_dtype(tkmi_gemm_plan plan, int32_t precision, int32_t dtype,
+                                  const void *A, const void *B, void *C, int32_t reps,
+                                  void *stream);
 /* 3×BF16 operands that do not change between calls (an environment tensor of a sweep,
  * constant across a Lanczos or Davidson run): operands = 1 (A), 2 (B), 3 (both), 0 (none).
  * The split of a constant operand into its three bf16 planes is made by the first

@@ -97,6 +97,12 @@ __device__ inline int find_group(const GGroup *__restrict__ g, int64_t ng, int64
     return (int)lo;
 }
 
+// One element of C: r (the FP64 result α·(A·B)) plus β·C, rounded once to C's element type.
+template <typename TD>
+__device__ __forceinline__ void put_c(TD *p, double r, double beta, bool has_beta) {
+    *p = (TD)(has_beta ? r + beta * (double)*p : r);
+}
+
 // --------------------------------------------------------------------------------
 // FP64 / FP32 kernel (same data flow; T_LDS = compute type of the MFMA)
 // --------------------------------------------------------------------------------
@@ -122,11 +128,12 @@ struct MfmaK4<float> {
 // NBUF = 2: double-buffered LDS, one barrier per slab.  NBUF = 1: one LDS buffer (half
 // the LDS, so more workgroups per CU), the next slab waits in registers while the
 // current one is multiplied; two barriers per slab.  NBUF = 3: one LDS buffer and two
-// register stages, loads two slabs ahead.
-template <typename TC, int BM, int BN, int BK, int NBUF>
+// register stages, loads two slabs ahead.  TD is the element type of A, B and C in
+// memory (double, or float for FP32 tensors); split-K partial products stay FP64.
+template <typename TC, typename TD, int BM, int BN, int BK, int NBUF>
 __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, int64_t ngroups, int64_t ntiles,
-                                             const double *__restrict__ A, const double *__restrict__ Bm,
-                                             double *__restrict__ C, double *__restrict__ ws, double alpha,
+                                             const TD *__restrict__ A, const TD *__restrict__ Bm,
+                                             TD *__restrict__ C, double *__restrict__ ws, double alpha,
                                              double beta) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -186,26 +193,26 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
     for (int e = 0; e < AE; ++e) {
         int i, k;
         a_el(e, i, k);
-        aoff[e] = (uint32_t)(sizeof(double) * (ta ? (k + i * lda) : (i + k * lda)));
+        aoff[e] = (uint32_t)(sizeof(TD) * (ta ? (k + i * lda) : (i + k * lda)));
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
         int j, k;
         b_el(e, j, k);
-        boff[e] = (uint32_t)(sizeof(double) * (tb ? (j + k * ldb) : (k + j * ldb)));
+        boff[e] = (uint32_t)(sizeof(TD) * (tb ? (j + k * ldb) : (k + j * ldb)));
     }
-    auto ld_at = [](const double *base, uint32_t off) {
-        return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    auto ld_at = [](const TD *base, uint32_t off) {
+        return *reinterpret_cast<const TD *>(reinterpret_cast<const char *>(base) + off);
     };
     const int64_t astep = ta ? BK : BK * lda;  // pointer advance per K slab
     const int64_t bstep = tb ? BK * ldb : BK;
-    const double *__restrict__ Aslab = A + G.a_off + (ta ? row0 * lda + kbeg : row0 + kbeg * lda);
-    const double *__restrict__ Bslab = Bm + G.b_off + (tb ? col0 + kbeg * ldb : col0 * ldb + kbeg);
+    const TD *__restrict__ Aslab = A + G.a_off + (ta ? row0 * lda + kbeg : row0 + kbeg * lda);
+    const TD *__restrict__ Bslab = Bm + G.b_off + (tb ? col0 + kbeg * ldb : col0 * ldb + kbeg);
     const bool tile_full = (row0 + BM <= m) && (col0 + BN <= n);
 
     // slabs are loaded strictly in K order: every load advances the slab pointers
-    double ra[AE], rb[BE];
-    auto load_into = [&](double (&xa)[AE], double (&xb)[BE], int64_t k0) {
+    TD ra[AE], rb[BE];
+    auto load_into = [&](TD (&xa)[AE], TD (&xb)[BE], int64_t k0) {
         if (tile_full && k0 + BK <= K) {
 #pragma unroll
             for (int e = 0; e < AE; ++e) xa[e] = ld_at(Aslab, aoff[e]);
@@ -219,19 +226,19 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
             for (int e = 0; e < AE; ++e) {
                 int i, k;
                 a_el(e, i, k);
-                xa[e] = (i < ilim && k < klim) ? ld_at(Aslab, aoff[e]) : 0.0;
+                xa[e] = (i < ilim && k < klim) ? ld_at(Aslab, aoff[e]) : TD(0);
             }
 #pragma unroll
             for (int e = 0; e < BE; ++e) {
                 int j, k;
                 b_el(e, j, k);
-                xb[e] = (j < jlim && k < klim) ? ld_at(Bslab, boff[e]) : 0.0;
+                xb[e] = (j < jlim && k < klim) ? ld_at(Bslab, boff[e]) : TD(0);
             }
         }
         Aslab += astep;
         Bslab += bstep;
     };
-    auto store_from = [&](int buf, const double (&xa)[AE], const double (&xb)[BE]) {
+    auto store_from = [&](int buf, const TD (&xa)[AE], const TD (&xb)[BE]) {
 #pragma unroll
         for (int e = 0; e < AE; ++e) {
             int i, k;
@@ -312,7 +319,7 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
         }
     } else {
         // one LDS buffer, two register stages: loads run two slabs ahead of the MFMAs
-        double ra1[AE], rb1[BE];
+        TD ra1[AE], rb1[BE];
         if (nk > 0) load_into(ra, rb, kbeg);
         if (nk > 1) load_into(ra1, rb1, kbeg + BK);
         for (int64_t kt = 0; kt < nk; kt += 2) {
@@ -339,52 +346,53 @@ __device__ __forceinline__ void gemm_k4_body(const GGroup *__restrict__ groups, 
         run(std::false_type{});
 
     // epilogue: C = α·acc + β·C, or this split's raw partial product into the workspace
-    const bool partial = G.splits > 1;
-    double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
-    const int64_t ldc = partial ? m : G.ldc;
-    const bool has_beta = !partial && (beta != 0.0);
-    const double a_ = partial ? 1.0 : alpha;
+    auto emit = [&](auto *Cg, int64_t ldc, double a_, bool has_beta) {
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
-            if (col >= n) continue;
+            for (int j = 0; j < FN; ++j) {
+                const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
+                if (col >= n) continue;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t row = row0 + wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
-                if (row < m) {
-                    double r = a_ * (double)acc[i][j][v];
-                    double *p = Cg + row + col * ldc;
-                    *p = has_beta ? r + beta * *p : r;
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t row = row0 + wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
+                    if (row < m) put_c(Cg + row + col * ldc, a_ * (double)acc[i][j][v], beta, has_beta);
                 }
             }
-        }
+    };
+    const bool partial = G.splits > 1;
+    if constexpr (std::is_same<TD, double>::value)
+        emit(partial ? ws + G.ws_off + z * m * n : C + G.c_off, partial ? m : G.ldc, partial ? 1.0 : alpha,
+             !partial && beta != 0.0);
+    else if (partial)
+        emit(ws + G.ws_off + z * m * n, m, 1.0, false);
+    else
+        emit(C + G.c_off, G.ldc, alpha, beta != 0.0);
 }
 
-template <typename TC, int BM, int BN, int BK>
+template <typename TC, typename TD, int BM, int BN, int BK>
 __global__ __launch_bounds__(NT) void gemm_k4_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                     int64_t ntiles, const double *__restrict__ A,
-                                                     const double *__restrict__ Bm, double *__restrict__ C,
+                                                     int64_t ntiles, const TD *__restrict__ A,
+                                                     const TD *__restrict__ Bm, TD *__restrict__ C,
                                                      double *__restrict__ ws, double alpha, double beta) {
-    gemm_k4_body<TC, BM, BN, BK, 2>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
+    gemm_k4_body<TC, TD, BM, BN, BK, 2>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
 }
 
 // 64×64×16 tiles fit 4 waves per SIMD (≤ 128 VGPR + AGPR) without spills when asked to
-template <typename TC, int BM, int BN, int BK>
+template <typename TC, typename TD, int BM, int BN, int BK>
 __global__ __launch_bounds__(NT, (BM * BN <= 64 * 64 && BK <= 16) ? 4 : 1) void gemm_k4s_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                      int64_t ntiles, const double *__restrict__ A,
-                                                      const double *__restrict__ Bm, double *__restrict__ C,
+                                                      int64_t ntiles, const TD *__restrict__ A,
+                                                      const TD *__restrict__ Bm, TD *__restrict__ C,
                                                       double *__restrict__ ws, double alpha, double beta) {
-    gemm_k4_body<TC, BM, BN, BK, 1>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
+    gemm_k4_body<TC, TD, BM, BN, BK, 1>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
 }
 
-template <typename TC, int BM, int BN, int BK>
+template <typename TC, typename TD, int BM, int BN, int BK>
 __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                      int64_t ntiles, const double *__restrict__ A,
-                                                      const double *__restrict__ Bm, double *__restrict__ C,
+                                                      int64_t ntiles, const TD *__restrict__ A,
+                                                      const TD *__restrict__ Bm, TD *__restrict__ C,
                                                       double *__restrict__ ws, double alpha, double beta) {
-    gemm_k4_body<TC, BM, BN, BK, 3>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
+    gemm_k4_body<TC, TD, BM, BN, BK, 3>(groups, ngroups, ntiles, A, Bm, C, ws, alpha, beta);
 }
 
 // --------------------------------------------------------------------------------
@@ -402,11 +410,11 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
 // L·x launch: the host schedule removes the tail and the prefetch the cold start of each
 // tile, but neither is what bounds these launches, so the autotuner keeps gemm_k4s_kernel.
 // --------------------------------------------------------------------------------
-template <typename TC, int BM, int BN, int BK>
+template <typename TC, typename TD, int BM, int BN, int BK>
 __device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, int64_t ngroups,
                                              const int32_t *__restrict__ sched,
-                                             const int32_t *__restrict__ slot_start, const double *__restrict__ A,
-                                             const double *__restrict__ Bm, double *__restrict__ C,
+                                             const int32_t *__restrict__ slot_start, const TD *__restrict__ A,
+                                             const TD *__restrict__ Bm, TD *__restrict__ C,
                                              double *__restrict__ ws, double alpha, double beta) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
@@ -492,23 +500,23 @@ __device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, 
         for (int e = 0; e < AE; ++e) {
             int i, k;
             a_el(L.ta, e, i, k);
-            aoff[e] = (uint32_t)(sizeof(double) * (L.ta ? (k + i * L.lda) : (i + k * L.lda)));
+            aoff[e] = (uint32_t)(sizeof(TD) * (L.ta ? (k + i * L.lda) : (i + k * L.lda)));
         }
 #pragma unroll
         for (int e = 0; e < BE; ++e) {
             int j, k;
             b_el(L.tb, e, j, k);
-            boff[e] = (uint32_t)(sizeof(double) * (L.tb ? (j + k * L.ldb) : (k + j * L.ldb)));
+            boff[e] = (uint32_t)(sizeof(TD) * (L.tb ? (j + k * L.ldb) : (k + j * L.ldb)));
         }
     };
-    auto ld_at = [](const double *base, uint32_t off) {
-        return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+    auto ld_at = [](const TD *base, uint32_t off) {
+        return *reinterpret_cast<const TD *>(reinterpret_cast<const char *>(base) + off);
     };
-    double ra[AE], rb[BE];
+    TD ra[AE], rb[BE];
     auto load_step = [&]() {
         const int64_t k0 = L.kbeg + (int64_t)lkt * BK;
-        const double *Aslab = A + L.a_off + (L.ta ? L.row0 * L.lda + k0 : L.row0 + k0 * L.lda);
-        const double *Bslab = Bm + L.b_off + (L.tb ? L.col0 + k0 * L.ldb : L.col0 * L.ldb + k0);
+        const TD *Aslab = A + L.a_off + (L.ta ? L.row0 * L.lda + k0 : L.row0 + k0 * L.lda);
+        const TD *Bslab = Bm + L.b_off + (L.tb ? L.col0 + k0 * L.ldb : L.col0 * L.ldb + k0);
         if (L.full && k0 + BK <= L.K) {
 #pragma unroll
             for (int e = 0; e < AE; ++e) ra[e] = ld_at(Aslab, aoff[e]);
@@ -521,13 +529,13 @@ __device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, 
             for (int e = 0; e < AE; ++e) {
                 int i, k;
                 a_el(L.ta, e, i, k);
-                ra[e] = (i < ilim && k < klim) ? ld_at(Aslab, aoff[e]) : 0.0;
+                ra[e] = (i < ilim && k < klim) ? ld_at(Aslab, aoff[e]) : TD(0);
             }
 #pragma unroll
             for (int e = 0; e < BE; ++e) {
                 int j, k;
                 b_el(L.tb, e, j, k);
-                rb[e] = (j < jlim && k < klim) ? ld_at(Bslab, boff[e]) : 0.0;
+                rb[e] = (j < jlim && k < klim) ? ld_at(Bslab, boff[e]) : TD(0);
             }
         }
         ++lkt;
@@ -581,16 +589,10 @@ __device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, 
                     for (int j = 0; j < FN; ++j) acc[i][j] = MfmaK4<TC>::mma(af[i][s], bfr[j][s], acc[i][j]);
         }
     };
-    auto epilogue = [&](const Tile &M) {
-        const int64_t m = M.m, n = M.n;
-        const bool partial = M.partial;
-        double *Cg = partial ? ws + M.ws_off + M.z * m * n : C + M.c_off;
-        const int64_t ldc = partial ? m : M.ldc;
-        const bool has_beta = !partial && (beta != 0.0);
-        const double a_ = partial ? 1.0 : alpha;
+    auto emit = [&](const Tile &M, auto *Cg, int64_t ldc, double a_, bool has_beta) {
         // the tile's corner on the scalar side, rows and columns inside it 32-bit
-        double *Ct = Cg + M.row0 + M.col0 * ldc;
-        const int rlim = (int)min((int64_t)BM, m - M.row0), clim = (int)min((int64_t)BN, n - M.col0);
+        auto *Ct = Cg + M.row0 + M.col0 * ldc;
+        const int rlim = (int)min((int64_t)BM, M.m - M.row0), clim = (int)min((int64_t)BN, M.n - M.col0);
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -600,15 +602,22 @@ __device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, 
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int row = wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
-                        if (row < rlim) {
-                            const double r = a_ * (double)acc[i][j][v];
-                            double *p = Ct + row + (int64_t)col * ldc;
-                            *p = has_beta ? r + beta * *p : r;
-                        }
+                        if (row < rlim)
+                            put_c(Ct + row + (int64_t)col * ldc, a_ * (double)acc[i][j][v], beta, has_beta);
                     }
                 }
                 acc[i][j] = acc_t{0, 0, 0, 0};
             }
+    };
+    auto epilogue = [&](const Tile &M) {
+        const bool partial = M.partial;
+        if constexpr (std::is_same<TD, double>::value)
+            emit(M, partial ? ws + M.ws_off + M.z * M.m * M.n : C + M.c_off, partial ? M.m : M.ldc,
+                 partial ? 1.0 : alpha, !partial && beta != 0.0);
+        else if (partial)
+            emit(M, ws + M.ws_off + M.z * M.m * M.n, M.m, 1.0, false);
+        else
+            emit(M, C + M.c_off, M.ldc, alpha, beta != 0.0);
     };
 
     L = decode(s0);
@@ -646,18 +655,19 @@ __device__ __forceinline__ void gemm_pk_body(const GGroup *__restrict__ groups, 
     }
 }
 
-template <typename TC, int BM, int BN, int BK>
+template <typename TC, typename TD, int BM, int BN, int BK>
 __global__ __launch_bounds__(NT, 4) void gemm_pks_kernel(
     const GGroup *__restrict__ groups, int64_t ngroups, const int32_t *__restrict__ sched,
-    const int32_t *__restrict__ slot_start, const double *__restrict__ A, const double *__restrict__ Bm,
-    double *__restrict__ C, double *__restrict__ ws, double alpha, double beta) {
+    const int32_t *__restrict__ slot_start, const TD *__restrict__ A, const TD *__restrict__ Bm,
+    TD *__restrict__ C, double *__restrict__ ws, double alpha, double beta) {
     static_assert(BM * BN <= 64 * 64, "four workgroups per CU: 64×64 tiles");
-    gemm_pk_body<TC, BM, BN, BK>(groups, ngroups, sched, slot_start, A, Bm, C, ws, alpha, beta);
+    gemm_pk_body<TC, TD, BM, BN, BK>(groups, ngroups, sched, slot_start, A, Bm, C, ws, alpha, beta);
 }
 
 // Split-K reduction: one thread per output element of the split groups.
+template <typename TD>
 __global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ rg, int64_t ng, int64_t total,
-                                                    const double *__restrict__ ws, double *__restrict__ C,
+                                                    const double *__restrict__ ws, TD *__restrict__ C,
                                                     double alpha, double beta) {
     const bool has_beta = (beta != 0.0);
     for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < total; e += (int64_t)gridDim.x * NT) {
@@ -672,9 +682,7 @@ __global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ r
         double acc = 0.0;
         for (int64_t zz = 0; zz < R.splits; ++zz) acc += ws[R.ws_off + zz * mn + le];
         const int64_t row = le % R.m, col = le / R.m;
-        double *p = C + R.c_off + row + col * R.ldc;
-        const double r = alpha * acc;
-        *p = has_beta ? r + beta * *p : r;
+        put_c(C + R.c_off + row + col * R.ldc, alpha * acc, beta, has_beta);
     }
 }
 
@@ -698,9 +706,12 @@ typedef uint32_t u4_t __attribute__((ext_vector_type(4)));  // 16 bytes = 8 bf16
 
 constexpr int PACK_T = 64;
 
-// which: 0 = the A and the B tiles of every group, 1 = the A tiles only, 2 = the B tiles only
+// which: 0 = the A and the B tiles of every group, 1 = the A tiles only, 2 = the B tiles only.
+// TD: element type of A and B (a float splits exactly: its 24-bit significand is three
+// 8-bit bf16 significands)
+template <typename TD>
 __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restrict__ pg, int64_t ng,
-                                                        const double *__restrict__ A, const double *__restrict__ Bm,
+                                                        const TD *__restrict__ A, const TD *__restrict__ Bm,
                                                         __bf16 *__restrict__ P, int which) {
     __shared__ double tile[PACK_T][PACK_T + 1];
     const int64_t q = blockIdx.x;
@@ -720,7 +731,7 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
     const int64_t tk = (G.kp + PACK_T - 1) / PACK_T;
     const int64_t i0 = (r / tk) * PACK_T, k0 = (r % tk) * PACK_T;
     const int64_t lim = isA ? G.m : G.n;
-    const double *src = isA ? A + G.a_off : Bm + G.b_off;
+    const TD *src = isA ? A + G.a_off : Bm + G.b_off;
     const int64_t ld = isA ? G.lda : G.ldb;
     // element (i, k) of op(A) / of op(B)ᵀ lies at i + k·ld (rows contiguous) or k + i·ld
     const bool rows_contiguous = isA ? (G.ta == 0) : (G.tb != 0);
@@ -730,14 +741,14 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
         const int64_t gi = i0 + ii;
         for (int kk = t / PACK_T; kk < PACK_T; kk += NT / PACK_T) {
             const int64_t gk = k0 + kk;
-            tile[ii][kk] = (gi < lim && gk < G.k) ? src[gi + gk * ld] : 0.0;
+            tile[ii][kk] = (gi < lim && gk < G.k) ? (double)src[gi + gk * ld] : 0.0;
         }
     } else {
         const int kk = t % PACK_T;
         const int64_t gk = k0 + kk;
         for (int ii = t / PACK_T; ii < PACK_T; ii += NT / PACK_T) {
             const int64_t gi = i0 + ii;
-            tile[ii][kk] = (gi < lim && gk < G.k) ? src[gk + gi * ld] : 0.0;
+            tile[ii][kk] = (gi < lim && gk < G.k) ? (double)src[gk + gi * ld] : 0.0;
         }
     }
     __syncthreads();
@@ -772,10 +783,10 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
 
 // STAGES: register stages of prefetched slabs (1: the next slab loads while the current
 // one is multiplied; 2: loads run two slabs ahead, for the short 32-deep bf16 slabs)
-template <int BM, int BN, int STAGES>
+template <typename TD, int BM, int BN, int STAGES>
 __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
                                                          int64_t ntiles, const __bf16 *__restrict__ P,
-                                                         double *__restrict__ C, double *__restrict__ ws,
+                                                         TD *__restrict__ C, double *__restrict__ ws,
                                                          double alpha, double beta) {
     constexpr int BK = 32;
     constexpr int WM = BM / 2, WN = BN / 2;
@@ -896,27 +907,29 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
     }
     }
 
-    const bool partial = G.splits > 1;
-    double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
-    const int64_t ldc = partial ? m : G.ldc;
-    const bool has_beta = !partial && (beta != 0.0);
-    const double a_ = partial ? 1.0 : alpha;
+    auto emit = [&](auto *Cg, int64_t ldc, double a_, bool has_beta) {
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
-            if (col >= n) continue;
+            for (int j = 0; j < FN; ++j) {
+                const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
+                if (col >= n) continue;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t row = row0 + wm * WM + i * 16 + 4 * (lane >> 4) + v;
-                if (row < m) {
-                    double r = a_ * (acc64[i][j][v] + (double)lo[i][j][v]);
-                    double *p = Cg + row + col * ldc;
-                    *p = has_beta ? r + beta * *p : r;
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t row = row0 + wm * WM + i * 16 + 4 * (lane >> 4) + v;
+                    if (row < m)
+                        put_c(Cg + row + col * ldc, a_ * (acc64[i][j][v] + (double)lo[i][j][v]), beta, has_beta);
                 }
             }
-        }
+    };
+    const bool partial = G.splits > 1;
+    if constexpr (std::is_same<TD, double>::value)
+        emit(partial ? ws + G.ws_off + z * m * n : C + G.c_off, partial ? m : G.ldc, partial ? 1.0 : alpha,
+             !partial && beta != 0.0);
+    else if (partial)
+        emit(ws + G.ws_off + z * m * n, m, 1.0, false);
+    else
+        emit(C + G.c_off, G.ldc, alpha, beta != 0.0);
 }
 
 
@@ -927,10 +940,10 @@ __global__ __launch_bounds__(NT) void gemm_bf16x3_kernel(const GGroup *__restric
 // O(long·(k + short)) bytes for O(long·k·short) flops, i.e. they are HBM-bound, so
 // FMA in FP64 is used for every precision form.
 // --------------------------------------------------------------------------------
-template <int SK>
+template <int SK, typename TD>
 __global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                         const double *__restrict__ A,
-                                                         const double *__restrict__ Bm, double *__restrict__ C,
+                                                         const TD *__restrict__ A,
+                                                         const TD *__restrict__ Bm, TD *__restrict__ C,
                                                          double alpha, double beta) {
     constexpr int KC = 32;
     __shared__ double Sm[KC][SK + 1];
@@ -938,8 +951,8 @@ __global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restric
     const GGroup &G = groups[find_group(groups, ngroups, tile)];
     const int64_t lt = tile - G.tile_start;
     const int64_t m = G.m, n = G.n, K = G.k;
-    const double *Ag = A + G.a_off;
-    const double *Bg = Bm + G.b_off;
+    const TD *Ag = A + G.a_off;
+    const TD *Bg = Bm + G.b_off;
     const int64_t lda = G.lda, ldb = G.ldb, ldc = G.ldc;
     const int ta = G.ta, tb = G.tb;
     const bool longm = (G.skinny_mode == 0);
@@ -982,15 +995,10 @@ __global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restric
     }
     if (x >= L) return;
     const bool has_beta = (beta != 0.0);
-    double *Cg = C + G.c_off;
+    TD *Cg = C + G.c_off;
 #pragma unroll
-    for (int q = 0; q < SK; ++q) {
-        if (q < S) {
-            double *pc = longm ? (Cg + x + (int64_t)q * ldc) : (Cg + q + x * ldc);
-            const double r = alpha * acc[q];
-            *pc = has_beta ? r + beta * *pc : r;
-        }
-    }
+    for (int q = 0; q < SK; ++q)
+        if (q < S) put_c(longm ? (Cg + x + (int64_t)q * ldc) : (Cg + q + x * ldc), alpha * acc[q], beta, has_beta);
 }
 
 // --------------------------------------------------------------------------------
@@ -1005,8 +1013,9 @@ struct DGroup {
     int64_t part_start, nparts, chunk;
 };
 
+template <typename TD>
 __global__ __launch_bounds__(NT) void gemm_dot_partial(const DGroup *__restrict__ dg, int64_t ng,
-                                                       const double *__restrict__ A, const double *__restrict__ B,
+                                                       const TD *__restrict__ A, const TD *__restrict__ B,
                                                        double *__restrict__ parts) {
     __shared__ double sh[NT];
     const int64_t part = (int64_t)blockIdx.x;
@@ -1017,17 +1026,17 @@ __global__ __launch_bounds__(NT) void gemm_dot_partial(const DGroup *__restrict_
     }
     const DGroup &D = dg[lo];
     const int64_t k0 = (part - D.part_start) * D.chunk, k1 = min(D.k, k0 + D.chunk);
-    const double *a = A + D.a_off, *b = B + D.b_off;
+    const TD *a = A + D.a_off, *b = B + D.b_off;
     const int64_t ai = D.a_inc, bi = D.b_inc;
     // two independent sums per thread keep two load pairs in flight; the order of every
     // addition is fixed by (k, chunk), so results are reproducible
     double s0 = 0.0, s1 = 0.0;
     int64_t i = k0 + threadIdx.x;
     for (; i + NT < k1; i += 2 * NT) {
-        s0 += a[i * ai] * b[i * bi];
-        s1 += a[(i + NT) * ai] * b[(i + NT) * bi];
+        s0 += (double)a[i * ai] * (double)b[i * bi];
+        s1 += (double)a[(i + NT) * ai] * (double)b[(i + NT) * bi];
     }
-    for (; i < k1; i += NT) s0 += a[i * ai] * b[i * bi];
+    for (; i < k1; i += NT) s0 += (double)a[i * ai] * (double)b[i * bi];
     sh[threadIdx.x] = s0 + s1;
     __syncthreads();
     for (int w = NT / 2; w > 0; w >>= 1) {
@@ -1037,8 +1046,9 @@ __global__ __launch_bounds__(NT) void gemm_dot_partial(const DGroup *__restrict_
     if (threadIdx.x == 0) parts[part] = sh[0];
 }
 
+template <typename TD>
 __global__ __launch_bounds__(NT) void gemm_dot_final(const DGroup *__restrict__ dg, const double *__restrict__ parts,
-                                                     double *__restrict__ C, double alpha, double beta) {
+                                                     TD *__restrict__ C, double alpha, double beta) {
     __shared__ double sh[NT];
     const DGroup &D = dg[blockIdx.x];
     double s = 0.0;
@@ -1049,11 +1059,7 @@ __global__ __launch_bounds__(NT) void gemm_dot_final(const DGroup *__restrict__ 
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double *c = C + D.c_off;
-        const double r = alpha * sh[0];
-        *c = beta != 0.0 ? r + beta * *c : r;
-    }
+    if (threadIdx.x == 0) put_c(C + D.c_off, alpha * sh[0], beta, beta != 0.0);
 }
 
 struct DotSet {
@@ -1112,6 +1118,7 @@ struct tkmi_gemm_plan_s {
     // comes from another pointer or after tkmi_gemm_plan_invalidate
     int32_t const_ops = 0;
     const void *split_src[2] = {nullptr, nullptr};
+    int32_t split_dt[2] = {TKMI_F64, TKMI_F64};
     __bf16 *panels = nullptr;
     SkinnySet skinny;
     DotSet dots;
@@ -1137,16 +1144,16 @@ int kernel_occupancy() {
     hipError_t e;
     if constexpr (NBUF == 5)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&gemm_pks_kernel<double, BM, BN, BK>), NT, 0);
+            &nb, reinterpret_cast<const void *>(&gemm_pks_kernel<double, double, BM, BN, BK>), NT, 0);
     else if constexpr (NBUF == 3)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&gemm_k4p_kernel<double, BM, BN, BK>), NT, 0);
+            &nb, reinterpret_cast<const void *>(&gemm_k4p_kernel<double, double, BM, BN, BK>), NT, 0);
     else if constexpr (NBUF == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&gemm_k4_kernel<double, BM, BN, BK>), NT, 0);
+            &nb, reinterpret_cast<const void *>(&gemm_k4_kernel<double, double, BM, BN, BK>), NT, 0);
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&gemm_k4s_kernel<double, BM, BN, BK>), NT, 0);
+            &nb, reinterpret_cast<const void *>(&gemm_k4s_kernel<double, double, BM, BN, BK>), NT, 0);
     return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
@@ -1333,72 +1340,69 @@ int auto_cfg(const tkmi_gemm_plan_s *) {
 
 int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
 
-template <int BM, int BN, int BK, int NBUF>
-void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const double *A, const double *B, double *C,
-              double alpha, double beta, hipStream_t s, bool tiles, bool reduce) {
+// One launch of an FP64 / FP32 MFMA kernel over the tiles of a configuration.
+template <typename TC, typename TD, int BM, int BN, int BK, int NBUF>
+void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha, double beta, hipStream_t s) {
+    const dim3 grid((unsigned)ts.ntiles), block(NT);
+    const int64_t ng = (int64_t)ts.groups.size();
+    if constexpr (NBUF == 5)
+        hipLaunchKernelGGL((gemm_pks_kernel<TC, TD, BM, BN, BK>), dim3((unsigned)ts.nslots), block, 0, s,
+                           ts.d_groups, ng, ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
+    else if constexpr (NBUF == 3)
+        hipLaunchKernelGGL((gemm_k4p_kernel<TC, TD, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, ts.ws, alpha, beta);
+    else if constexpr (NBUF == 2)
+        hipLaunchKernelGGL((gemm_k4_kernel<TC, TD, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, ts.ws, alpha, beta);
+    else
+        hipLaunchKernelGGL((gemm_k4s_kernel<TC, TD, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
+                           B, C, ts.ws, alpha, beta);
+}
+
+constexpr int dtype_of(double *) { return TKMI_F64; }
+constexpr int dtype_of(float *) { return TKMI_F32; }
+
+template <typename TD, int BM, int BN, int BK, int NBUF>
+void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const TD *A, const TD *B, TD *C, double alpha,
+              double beta, hipStream_t s, bool tiles, bool reduce) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
     if (!tiles) {
-    } else if (prec != TKMI_PREC_BF16X3) {
-        if constexpr (NBUF == 5) {
-            const dim3 pgrid((unsigned)ts.nslots);
-            if (prec == TKMI_PREC_F64)
-                hipLaunchKernelGGL((gemm_pks_kernel<double, BM, BN, BK>), pgrid, block, 0, s, ts.d_groups, ng,
-                                   ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
-            else
-                hipLaunchKernelGGL((gemm_pks_kernel<float, BM, BN, BK>), pgrid, block, 0, s, ts.d_groups, ng,
-                                   ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
-        } else if constexpr (NBUF == 3) {
-            if (prec == TKMI_PREC_F64)
-                hipLaunchKernelGGL((gemm_k4p_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
-                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
-            else
-                hipLaunchKernelGGL((gemm_k4p_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
-                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
-        } else if constexpr (NBUF == 2) {
-            if (prec == TKMI_PREC_F64)
-                hipLaunchKernelGGL((gemm_k4_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
-                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
-            else
-                hipLaunchKernelGGL((gemm_k4_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
-                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
-        } else {
-            if (prec == TKMI_PREC_F64)
-                hipLaunchKernelGGL((gemm_k4s_kernel<double, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
-                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
-            else
-                hipLaunchKernelGGL((gemm_k4s_kernel<float, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng,
-                                   ts.ntiles, A, B, C, ts.ws, alpha, beta);
-        }
-    }
-    else {
+    } else if (prec == TKMI_PREC_F64) {
+        if constexpr (std::is_same<TD, double>::value) launch_k4<double, TD, BM, BN, BK, NBUF>(ts, A, B, C, alpha, beta, s);
+    } else if (prec == TKMI_PREC_F32) {
+        launch_k4<float, TD, BM, BN, BK, NBUF>(ts, A, B, C, alpha, beta, s);
+    } else {
         // split the operands whose panels do not already hold them (constant operands from
-        // the same pointer as the last split keep theirs)
-        const bool keepA = (p->const_ops & 1) && p->split_src[0] == A;
-        const bool keepB = (p->const_ops & 2) && p->split_src[1] == B;
+        // the same pointer and element type as the last split keep theirs)
+        const int dt = dtype_of((TD *)nullptr);
+        const bool keepA = (p->const_ops & 1) && p->split_src[0] == A && p->split_dt[0] == dt;
+        const bool keepB = (p->const_ops & 2) && p->split_src[1] == B && p->split_dt[1] == dt;
         const int which = !keepA && !keepB ? 0 : (!keepA ? 1 : (!keepB ? 2 : -1));
         const int64_t npack = which == 0 ? p->pack_total : (which == 1 ? p->pack_total_a : p->pack_total_b);
         if (which >= 0 && npack > 0)
-            hipLaunchKernelGGL(split_pack_kernel, dim3((unsigned)npack), block, 0, s, p->d_pack,
+            hipLaunchKernelGGL(split_pack_kernel<TD>, dim3((unsigned)npack), block, 0, s, p->d_pack,
                                (int64_t)p->pack.size(), A, B, p->panels, which);
         p->split_src[0] = (p->const_ops & 1) ? (const void *)A : nullptr;
         p->split_src[1] = (p->const_ops & 2) ? (const void *)B : nullptr;
+        p->split_dt[0] = p->split_dt[1] = dt;
         if constexpr (NBUF == 3)
-            hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 2>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
+            hipLaunchKernelGGL((gemm_bf16x3_kernel<TD, BM, BN, 2>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
                                (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
         else
-            hipLaunchKernelGGL((gemm_bf16x3_kernel<BM, BN, 1>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
+            hipLaunchKernelGGL((gemm_bf16x3_kernel<TD, BM, BN, 1>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
                                (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
     }
     if (reduce && ts.relems > 0) {
         const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
-        hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
+        hipLaunchKernelGGL(splitk_reduce<TD>, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
                            (int64_t)ts.rgroups.size(), ts.relems, ts.ws, C, alpha, beta);
     }
 }
 
-int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const double *B, double *C,
-               double alpha, double beta, hipStream_t s, bool tiles = true, bool reduce = true) {
+template <typename TD>
+int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const TD *A, const TD *B, TD *C, double alpha, double beta,
+               hipStream_t s, bool tiles = true, bool reduce = true) {
     const TileSet &ts = p->ts[c];
     if (ts.ntiles == 0) return TKMI_OK;
     if (prec == TKMI_PREC_BF16X3 && !p->panels) {
@@ -1414,45 +1418,45 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const double *A, const doub
     if (ts.ntiles * NT >= ((int64_t)1 << 31))
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more output tiles than one launch can hold (2^31 work-items)");
     switch (c) {
-        case 0: launch_k<64, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 1: launch_k<128, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 2: launch_k<128, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 3: launch_k<64, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 4: launch_k<64, 64, 32, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 5: launch_k<64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 6: launch_k<64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 7: launch_k<128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        case 8: launch_k<64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        default: launch_k<64, 64, 16, 5>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 0: launch_k<TD, 64, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 1: launch_k<TD, 128, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 2: launch_k<TD, 128, 64, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 3: launch_k<TD, 64, 128, 16, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 4: launch_k<TD, 64, 64, 32, 2>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 5: launch_k<TD, 64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 6: launch_k<TD, 64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 7: launch_k<TD, 128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 8: launch_k<TD, 64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        default: launch_k<TD, 64, 64, 16, 5>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
 
-int launch_skinny(const SkinnySet &ss, const double *A, const double *B, double *C, double alpha, double beta,
-                  hipStream_t s) {
+template <typename TD>
+int launch_skinny(const SkinnySet &ss, const TD *A, const TD *B, TD *C, double alpha, double beta, hipStream_t s) {
     if (ss.ntiles == 0) return TKMI_OK;
     if (ss.ntiles * NT >= ((int64_t)1 << 31))
         TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more skinny tiles than one launch can hold (2^31 work-items)");
     dim3 grid((unsigned)ss.ntiles), block(NT);
     const int64_t ng = (int64_t)ss.groups.size();
     if (ss.sk <= 8)
-        hipLaunchKernelGGL(gemm_skinny_kernel<8>, grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
+        hipLaunchKernelGGL((gemm_skinny_kernel<8, TD>), grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
     else if (ss.sk <= 16)
-        hipLaunchKernelGGL(gemm_skinny_kernel<16>, grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
+        hipLaunchKernelGGL((gemm_skinny_kernel<16, TD>), grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
     else
-        hipLaunchKernelGGL(gemm_skinny_kernel<32>, grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
+        hipLaunchKernelGGL((gemm_skinny_kernel<32, TD>), grid, block, 0, s, ss.d_groups, ng, A, B, C, alpha, beta);
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
 
-int launch_dots(const DotSet &ds, const double *A, const double *B, double *C, double alpha, double beta,
-                hipStream_t s) {
+template <typename TD>
+int launch_dots(const DotSet &ds, const TD *A, const TD *B, TD *C, double alpha, double beta, hipStream_t s) {
     if (ds.groups.empty()) return TKMI_OK;
-    hipLaunchKernelGGL(gemm_dot_partial, dim3((unsigned)ds.nparts), dim3(NT), 0, s, ds.d_groups,
+    hipLaunchKernelGGL(gemm_dot_partial<TD>, dim3((unsigned)ds.nparts), dim3(NT), 0, s, ds.d_groups,
                        (int64_t)ds.groups.size(), A, B, ds.parts);
-    hipLaunchKernelGGL(gemm_dot_final, dim3((unsigned)ds.groups.size()), dim3(NT), 0, s, ds.d_groups, ds.parts, C,
-                       alpha, beta);
+    hipLaunchKernelGGL(gemm_dot_final<TD>, dim3((unsigned)ds.groups.size()), dim3(NT), 0, s, ds.d_groups, ds.parts,
+                       C, alpha, beta);
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
@@ -1477,6 +1481,81 @@ void free_gemm(tkmi_gemm_plan_s *p) {
 
 int check_prec(int32_t precision) {
     if (precision < 0 || precision > 2) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown precision");
+    return TKMI_OK;
+}
+
+int check_dtype(int32_t precision, int32_t dtype) {
+    if (int rc = check_prec(precision)) return rc;
+    if (dtype != TKMI_F64 && dtype != TKMI_F32) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: unknown element type");
+    if (dtype == TKMI_F32 && precision == TKMI_PREC_F64)
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: FP32 data runs in TKMI_PREC_F32 or TKMI_PREC_BF16X3");
+    return TKMI_OK;
+}
+
+template <typename TD>
+int exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alpha, double beta, const TD *A,
+               const TD *B, TD *C, hipStream_t s) {
+    int rc = TKMI_OK;
+    if (alpha == 0.0) {
+        // BLAS convention: A and B are not read, C = β·C (β = 0 writes zeros)
+        for (int q = 0; q < 2; ++q) {
+            if (!(parts & (1 << q)) || plan->scale_total[q] == 0) continue;
+            const int64_t blocks = std::min<int64_t>((plan->scale_total[q] + NT - 1) / NT, 8192);
+            hipLaunchKernelGGL(splitk_reduce<TD>, dim3((unsigned)blocks), dim3(NT), 0, s, plan->d_scale[q],
+                               (int64_t)plan->scale[q].size(), plan->scale_total[q], (const double *)nullptr, C,
+                               0.0, beta);
+        }
+        TKMI_HIP(hipGetLastError());
+        return TKMI_OK;
+    }
+    // bit 2: the MFMA launch without its split-K reduction; bit 3: that reduction alone
+    // (timing only: either leaves the split groups of C without their sums)
+    const bool tiles = (parts & 1) != 0, reduce = (parts & 1) ? !(parts & 4) : (parts & 8) != 0;
+    if (tiles || reduce)
+        rc = launch_cfg<TD>(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s, tiles, reduce);
+    if (rc) return rc;
+    if (parts & 2) rc = launch_skinny<TD>(plan->skinny, A, B, C, alpha, beta, s);
+    if (!rc && (parts & 2)) rc = launch_dots<TD>(plan->dots, A, B, C, alpha, beta, s);
+    return rc;
+}
+
+template <typename TD>
+int autotune(tkmi_gemm_plan plan, int32_t precision, const TD *A, const TD *B, TD *C, int32_t reps,
+             hipStream_t s) {
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    TKMI_HIP(hipEventCreate(&e0));
+    TKMI_HIP(hipEventCreate(&e1));
+    // Interleaved rounds, best time per configuration (drift and clock ramps hit every
+    // configuration alike); another configuration replaces the default only when it
+    // is faster by more than 3%, so near-ties resolve the same way on every run.
+    constexpr int ROUNDS = 3;
+    float tbest[NCFG];
+    for (int c = 0; c < NCFG; ++c) tbest[c] = 1e30f;
+    int rc = TKMI_OK;
+    for (int round = 0; round < ROUNDS && rc == TKMI_OK; ++round) {
+        for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
+            if (plan->ts[c].ntiles == 0) continue;
+            if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
+            if ((rc = launch_cfg<TD>(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
+            for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg<TD>(plan, c, precision, A, B, C, 1.0, 0.0, s);
+            if (rc) break;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            tbest[c] = std::min(tbest[c], ms);
+        }
+    }
+    const int d = auto_cfg(plan);
+    int best_c = d;
+    for (int c = 0; c < NCFG; ++c)
+        if (tbest[c] < tbest[best_c]) best_c = c;
+    if (tbest[d] <= 1.03f * tbest[best_c]) best_c = d;  // within 3% of the fastest: keep the default
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    plan->choice[precision] = best_c;
     return TKMI_OK;
 }
 
@@ -1661,29 +1740,7 @@ int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, 
                          const double *A, const double *B, double *C, void *stream) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
     if (int rc = check_prec(precision)) return rc;
-    hipStream_t s = tkmi::as_stream(stream);
-    int rc = TKMI_OK;
-    if (alpha == 0.0) {
-        // BLAS convention: A and B are not read, C = β·C (β = 0 writes zeros)
-        for (int q = 0; q < 2; ++q) {
-            if (!(parts & (1 << q)) || plan->scale_total[q] == 0) continue;
-            const int64_t blocks = std::min<int64_t>((plan->scale_total[q] + NT - 1) / NT, 8192);
-            hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(NT), 0, s, plan->d_scale[q],
-                               (int64_t)plan->scale[q].size(), plan->scale_total[q], (const double *)nullptr, C,
-                               0.0, beta);
-        }
-        TKMI_HIP(hipGetLastError());
-        return TKMI_OK;
-    }
-    // bit 2: the MFMA launch without its split-K reduction; bit 3: that reduction alone
-    // (timing only: either leaves the split groups of C without their sums)
-    const bool tiles = (parts & 1) != 0, reduce = (parts & 1) ? !(parts & 4) : (parts & 8) != 0;
-    if (tiles || reduce)
-        rc = launch_cfg(plan, cfg_of(plan, precision), precision, A, B, C, alpha, beta, s, tiles, reduce);
-    if (rc) return rc;
-    if (parts & 2) rc = launch_skinny(plan->skinny, A, B, C, alpha, beta, s);
-    if (!rc && (parts & 2)) rc = launch_dots(plan->dots, A, B, C, alpha, beta, s);
-    return rc;
+    return exec_parts<double>(plan, precision, parts, alpha, beta, A, B, C, tkmi::as_stream(stream));
 }
 
 int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double beta, const double *A,
@@ -1691,46 +1748,31 @@ int tkmi_gemm_exec(tkmi_gemm_plan plan, int32_t precision, double alpha, double 
     return tkmi_gemm_exec_parts(plan, precision, 3, alpha, beta, A, B, C, stream);
 }
 
+int tkmi_gemm_exec_dtype(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, int32_t parts, double alpha,
+                         double beta, const void *A, const void *B, void *C, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (int rc = check_dtype(precision, dtype)) return rc;
+    hipStream_t s = tkmi::as_stream(stream);
+    if (dtype == TKMI_F32)
+        return exec_parts<float>(plan, precision, parts, alpha, beta, (const float *)A, (const float *)B, (float *)C, s);
+    return exec_parts<double>(plan, precision, parts, alpha, beta, (const double *)A, (const double *)B, (double *)C, s);
+}
+
 int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double *A, const double *B, double *C,
                             int32_t reps, void *stream) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
     if (int rc = check_prec(precision)) return rc;
-    if (reps < 1) reps = 1;
+    return autotune<double>(plan, precision, A, B, C, reps, tkmi::as_stream(stream));
+}
+
+int tkmi_gemm_plan_autotune_dtype(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, const void *A,
+                                  const void *B, void *C, int32_t reps, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (int rc = check_dtype(precision, dtype)) return rc;
     hipStream_t s = tkmi::as_stream(stream);
-    hipEvent_t e0, e1;
-    TKMI_HIP(hipEventCreate(&e0));
-    TKMI_HIP(hipEventCreate(&e1));
-    // Interleaved rounds, best time per configuration (drift and clock ramps hit every
-    // configuration alike); another configuration replaces the default only when it
-    // is faster by more than 3%, so near-ties resolve the same way on every run.
-    constexpr int ROUNDS = 3;
-    float tbest[NCFG];
-    for (int c = 0; c < NCFG; ++c) tbest[c] = 1e30f;
-    int rc = TKMI_OK;
-    for (int round = 0; round < ROUNDS && rc == TKMI_OK; ++round) {
-        for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
-            if (plan->ts[c].ntiles == 0) continue;
-            if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
-            if ((rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
-            if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
-            for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg(plan, c, precision, A, B, C, 1.0, 0.0, s);
-            if (rc) break;
-            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            tbest[c] = std::min(tbest[c], ms);
-        }
-    }
-    const int d = auto_cfg(plan);
-    int best_c = d;
-    for (int c = 0; c < NCFG; ++c)
-        if (tbest[c] < tbest[best_c]) best_c = c;
-    if (tbest[d] <= 1.03f * tbest[best_c]) best_c = d;  // within 3% of the fastest: keep the default
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    plan->choice[precision] = best_c;
-    return TKMI_OK;
+    if (dtype == TKMI_F32)
+        return autotune<float>(plan, precision, (const float *)A, (const float *)B, (float *)C, reps, s);
+    return autotune<double>(plan, precision, (const double *)A, (const double *)B, (double *)C, reps, s);
 }
 
 int tkmi_gemm_plan_set_constant(tkmi_gemm_plan plan, int32_t operands) {

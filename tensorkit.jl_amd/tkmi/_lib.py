@@ -50,11 +50,14 @@ SIGNATURES = [
     ("tkmi_gemm_config_info", C.c_int, [C.c_int32, c_i32p, c_i32p, c_i32p, c_i32p]),
     ("tkmi_gemm_exec_parts", C.c_int, [c_vp, C.c_int32, C.c_int32, C.c_double, C.c_double, c_vp, c_vp, c_vp,
                                        c_vp]),
+    ("tkmi_gemm_exec_dtype", C.c_int, [c_vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, c_vp, c_vp,
+                                       c_vp, c_vp]),
     ("tkmi_gemm_plan_split", C.c_int, [c_vp, c_f64p, c_f64p, c_i64p]),
     ("tkmi_gemm_plan_info", C.c_int, [c_vp, c_i64p, c_f64p]),
     ("tkmi_gemm_plan_splitk", C.c_int, [c_vp, C.c_int32, c_i64p, c_i64p]),
     ("tkmi_gemm_plan_query", C.c_int, [c_vp, C.c_int32, c_i32p, c_i64p]),
     ("tkmi_gemm_plan_autotune", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, c_vp, C.c_int32, c_vp]),
+    ("tkmi_gemm_plan_autotune_dtype", C.c_int, [c_vp, C.c_int32, C.c_int32, c_vp, c_vp, c_vp, C.c_int32, c_vp]),
     ("tkmi_gemm_plan_set_constant", C.c_int, [c_vp, C.c_int32]),
     ("tkmi_gemm_plan_invalidate", C.c_int, [c_vp]),
     ("tkmi_gemm_plan_destroy", C.c_int, [c_vp]),

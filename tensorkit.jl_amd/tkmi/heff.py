@@ -76,6 +76,20 @@ class TwoSiteHeff:
     def operands(self):
         return [self.L, self.M1, self.M2, self.R]
 
+    def astype(self, dtype):
+        """The same operator with its operands (and x) converted to `dtype`, each element
+        rounded once.  A float32 operator keeps every intermediate of the chain in FP32 (the
+        permutes move half the bytes); its grouped GEMMs run in FP32 or 3×BF16."""
+        H = TwoSiteHeff.__new__(TwoSiteHeff)
+        H.V, H.P, H.W, H.spaces = self.V, self.P, self.W, self.spaces
+        for k in ("L", "M1", "M2", "R", "x"):
+            t = getattr(self, k)
+            setattr(H, k, T.TensorMap(t.data.to(dtype), t.space))
+        H.precision = self.precision
+        H.outs = [None] * len(STEPS)
+        H.graph = None
+        return H
+
     def apply(self, x=None, precision=None):
         """y = H_eff(x); intermediates are reused across calls."""
         x = self.x if x is None else x

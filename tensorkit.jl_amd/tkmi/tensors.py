@@ -610,19 +610,32 @@ def gemm_plan(tC, tA, tB, with_beta_groups=True):
     return plan
 
 
+def gemm_precision(dtype, precision=None):
+    """The arithmetic of a grouped GEMM on tensors of element type `dtype`: float64 runs
+    in DEFAULT_PRECISION unless told otherwise; float32 runs in FP32 (the sgemm of a
+    Float32 mul!) or, when asked, in 3×BF16 — FP64 arithmetic on float32 data is refused."""
+    if dtype == torch.float32:
+        prec = _lib.PREC_F32 if precision is None else precision
+        if prec == _lib.PREC_F64:
+            raise TypeError("mul!: float32 tensors multiply in FP32 or 3×BF16, not in FP64")
+        return prec
+    return DEFAULT_PRECISION if precision is None else precision
+
+
 def mul_(tC, tA, tB, alpha=1.0, beta=0.0, precision=None):
     """`mul!(tC, tA, tB, α, β)` (`linalg.jl:330-370`): one grouped launch."""
     if tA.space.compose(tB.space) != tC.space:
         raise SpaceMismatch(f"{tC.space} ≠ {tA.space} * {tB.space}")
-    if not (tA.dtype == tB.dtype == tC.dtype == torch.float64):
-        raise TypeError("mul!: the grouped GEMM takes float64 tensors")
+    if not (tA.dtype == tB.dtype == tC.dtype):
+        raise TypeError(f"mul!: element types differ (A {tA.dtype}, B {tB.dtype}, C {tC.dtype})")
     plan = gemm_plan(tC, tA, tB)
-    prec = DEFAULT_PRECISION if precision is None else precision
+    prec = gemm_precision(tC.dtype, precision)
     lib = _lib.load()
     if _TRACE is not None:
         _TRACE.append(("gemm", plan, int(prec), _buf(tA), _buf(tB), _buf(tC), float(alpha), float(beta)))
-    _lib.check(lib.tkmi_gemm_exec(plan.handle, int(prec), float(alpha), float(beta), tA.data.data_ptr(),
-                                  tB.data.data_ptr(), tC.data.data_ptr(), _stream()), "mul!")
+    _lib.check(lib.tkmi_gemm_exec_dtype(plan.handle, int(prec), _DT[tC.dtype], 3, float(alpha), float(beta),
+                                        tA.data.data_ptr(), tB.data.data_ptr(), tC.data.data_ptr(), _stream()),
+               "mul!")
     return tC
 
 
@@ -869,5 +882,8 @@ def autotune_trace(trace, precisions=(DEFAULT_PRECISION,), reps=5):
             if (id(plan), p) in done:
                 continue
             done.add((id(plan), p))
-            _lib.check(lib.tkmi_gemm_plan_autotune(plan.handle, int(p), tA.data_ptr(), tB.data_ptr(),
-                                                   tC.data_ptr(), int(reps), _stream()), "autotune")
+            if tC.dtype == torch.float32 and p == _lib.PREC_F64:
+                continue
+            _lib.check(lib.tkmi_gemm_plan_autotune_dtype(plan.handle, int(p), _DT[tC.dtype], tA.data_ptr(),
+                                                         tB.data_ptr(), tC.data_ptr(), int(reps), _stream()),
+                       "autotune")

@@ -128,3 +128,16 @@ def test_mixed_element_types_are_refused():
         assert not FC.eligible(C, A, pA, B, pB)
         with pytest.raises(TypeError):
             T.blas_contract_(C, A, pA, B, pB, ((0, 2), (1,)), 1.0, 0.0)
+
+
+def test_float32_gemm_arithmetic():
+    """Float32 tensors multiply in FP32 (the sgemm of a Float32 mul!) unless 3×BF16 is asked
+    for; FP64 arithmetic on float32 data is refused.  Float64 keeps the default form."""
+    import torch
+    from tkmi import tensors as T, _lib
+    assert T.gemm_precision(torch.float32) == _lib.PREC_F32
+    assert T.gemm_precision(torch.float32, _lib.PREC_BF16X3) == _lib.PREC_BF16X3
+    with pytest.raises(TypeError):
+        T.gemm_precision(torch.float32, _lib.PREC_F64)
+    assert T.gemm_precision(torch.float64) == T.DEFAULT_PRECISION
+    assert T.gemm_precision(torch.float64, _lib.PREC_BF16X3) == _lib.PREC_BF16X3

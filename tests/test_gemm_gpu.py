@@ -51,7 +51,13 @@ def _run(lib, tab, A, B, Cm, prec, alpha, beta, tile=0):
     dB = torch.from_numpy(B).cuda()
     dC = torch.from_numpy(Cm).cuda()
     s = torch.cuda.current_stream().cuda_stream
-    lib.check(L.tkmi_gemm_exec(h, prec, alpha, beta, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s), "exec")
+    if A.dtype == np.float32:
+        # the blocks of Float32 tensors: A, B and C all float32
+        assert B.dtype == Cm.dtype == np.float32
+        lib.check(L.tkmi_gemm_exec_dtype(h, prec, lib.TKMI_F32, 3, alpha, beta, dA.data_ptr(), dB.data_ptr(),
+                                         dC.data_ptr(), s), "exec f32")
+    else:
+        lib.check(L.tkmi_gemm_exec(h, prec, alpha, beta, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s), "exec")
     out = dC.cpu().numpy()
     L.tkmi_gemm_plan_destroy(h)
     return out
@@ -297,6 +303,84 @@ def test_dot_groups(lib, ta, tb):
     assert np.array_equal(o1, o2)
     ref = _ref(tab, Af, Bf, Cm, 1.0, 0.0)
     assert np.linalg.norm(o1 - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+# ---------------------------------------------------------------------------
+# float32 tensors (tkmi_gemm_exec_dtype with TKMI_F32): A, B, C float32, results formed
+# in FP64 and rounded once.  Integer data keeps every partial sum below 2^24, so the FP32
+# MFMA, the exact bf16 split of a float and the final rounding are all exact.
+# ---------------------------------------------------------------------------
+def _f32(*xs):
+    return [x.astype(np.float32) for x in xs]
+
+
+@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_f32_data_exact_integer(lib, ta, tb, tile):
+    rng = np.random.default_rng(50 + 10 * ta + tb)
+    tab, na, nb, nc = _make_groups(rng, SHAPES, [ta] * len(SHAPES), [tb] * len(SHAPES))
+    A = rng.integers(-8, 9, na).astype(np.float64)
+    B = rng.integers(-8, 9, nb).astype(np.float64)
+    Cm = rng.integers(-8, 9, nc).astype(np.float64)
+    for prec in (1, 2):
+        for alpha, beta in ((1.0, 0.0), (2.0, -1.0), (1.0, 1.0)):
+            out = _run(lib, tab, *_f32(A, B, Cm), prec, alpha, beta, tile)
+            assert out.dtype == np.float32
+            np.testing.assert_array_equal(out, _ref(tab, A, B, Cm, alpha, beta).astype(np.float32))
+
+
+@pytest.mark.parametrize("tile", [1002, 1005, 1008, 1009])
+def test_f32_data_split_k_and_dots_exact(lib, tile):
+    """Split-K groups (FP64 partials, reduced into float32 C) and dot groups (FP64 sums
+    rounded once: a sum above 2^24 rounds like the exact sum cast to float32)."""
+    rng = np.random.default_rng(61)
+    shapes = [(100, 90, 3000), (64, 64, 1000), (130, 33, 4097), (40, 700, 513), (1, 1, 8193), (1, 1, 2_500_000),
+              (40, 8, 30)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0, 1, 0, 1, 0], [1, 0, 0, 1, 1, 0, 0])
+    A = rng.integers(-4, 5, na).astype(np.float64)
+    B = rng.integers(-4, 5, nb).astype(np.float64)
+    Cm = rng.integers(-4, 5, nc).astype(np.float64)
+    for prec in (1, 2):
+        for alpha, beta in ((1.0, 0.0), (0.5, 2.0), (0.0, 3.0)):
+            out = _run(lib, tab, *_f32(A, B, Cm), prec, alpha, beta, tile)
+            np.testing.assert_array_equal(out, _ref(tab, A, B, Cm, alpha, beta).astype(np.float32))
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 2e-5), (2, 1e-6)])
+def test_f32_data_random(lib, prec, tol):
+    """Random float32 data: relative Frobenius error against the FP64 product of the same
+    float32 inputs (FP32 MFMA: FP32 accumulation; 3×BF16: an exact split, the three
+    smallest of the nine products dropped)."""
+    rng = np.random.default_rng(62)
+    shapes = [(300, 280, 290), (64, 64, 64), (33, 17, 1000), (128, 129, 5), (12, 300, 20)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0, 1, 0], [0, 0, 1, 1, 1])
+    A, B, Cm = _f32(rng.standard_normal(na), rng.standard_normal(nb), rng.standard_normal(nc))
+    ref = _ref(tab, A.astype(np.float64), B.astype(np.float64), Cm.astype(np.float64), 1.5, -0.5)
+    for tile in (1005, 1002):
+        out = _run(lib, tab, A, B, Cm, prec, 1.5, -0.5, tile)
+        err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+        assert err <= tol, (prec, tile, err)
+
+
+def test_f32_data_refuses_fp64_arithmetic(lib):
+    rng = np.random.default_rng(63)
+    tab, na, nb, nc = _make_groups(rng, [(8, 8, 8)], [0], [0])
+    arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+    cols = [lib.i64(arr[:, i]) for i in range(11)]
+    ta, tb = lib.i32(arr[:, 5]), lib.i32(arr[:, 8])
+    L = lib.load()
+    h = C.c_void_p()
+    lib.check(L.tkmi_gemm_plan_create(C.byref(h), 1, cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                      cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1], cols[10][1]),
+              "plan")
+    try:
+        d = torch.zeros(max(na, nb, nc), dtype=torch.float32, device="cuda")
+        s = torch.cuda.current_stream().cuda_stream
+        assert L.tkmi_gemm_exec_dtype(h, 0, lib.TKMI_F32, 3, 1.0, 0.0, d.data_ptr(), d.data_ptr(), d.data_ptr(),
+                                      s) == 6  # TKMI_ERR_UNSUPPORTED
+        assert L.tkmi_gemm_exec_dtype(h, 1, 7, 3, 1.0, 0.0, d.data_ptr(), d.data_ptr(), d.data_ptr(), s) == 1
+    finally:
+        L.tkmi_gemm_plan_destroy(h)
 
 
 def test_bf16x3_constant_operands(lib):

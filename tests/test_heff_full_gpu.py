@@ -6,8 +6,9 @@
     tables, one OpenBLAS GEMM per coupled sector, fused contractions replayed unfused):
     relative Frobenius error ≤ 1e-13; the captured graph replays to the eager result bit
     for bit.
-  * FP32 and 3×BF16 forms: against the dense einsum at D = 96, and against the FP64 chain at
-    D = 2052, relative Frobenius error ≤ 1e-6.
+  * FP32 and 3×BF16 GEMMs, on the FP64 chain and on the float32 chain (operands and x
+    rounded to FP32 once, every intermediate FP32): against the dense einsum at D = 96, and
+    against the FP64 chain at D = 2052, relative Frobenius error ≤ 1e-6.
   * svd_trunc(truncrank 1024 ∩ trunctol 1e-10) of the two-site result as (V⊗P) ← (P⊗V)′
     (src/factorizations/truncation.jl:171-322): singular values per block against numpy's
     LAPACK, the kept set and ε against a sequential restatement of the truncation; the
@@ -53,15 +54,30 @@ def heff(T):
     torch.cuda.synchronize()
     out = {"H": H, "trace": trace, "y64": y64.numpy().copy(), "space": y64.space}
     for name, prec in (("fp32", _lib.PREC_F32), ("bf16x3", _lib.PREC_BF16X3)):
-        H2 = TwoSiteHeff.__new__(TwoSiteHeff)
-        H2.__dict__.update({k: v for k, v in H.__dict__.items() if k not in ("graph", "outs")})
-        H2.outs = [None] * 4
-        H2.graph = None
+        H2 = H.astype(torch.float64)
         H2.capture(precision=prec)
         H2.replay()
         torch.cuda.synchronize()
         out[name] = H2.graph_y.numpy().copy()
         out[name + "_t"] = H2.graph_y
+    # the float32 chain, its GEMMs tuned on FP32 data
+    H32 = H.astype(torch.float32)
+    T.start_trace()
+    H32.apply(precision=_lib.PREC_F32)
+    T.autotune_trace(T.stop_trace(), precisions=(_lib.PREC_F32, _lib.PREC_BF16X3))
+    for name, prec in (("fp32_f32data", _lib.PREC_F32), ("bf16x3_f32data", _lib.PREC_BF16X3)):
+        H32.capture(precision=prec)
+        for _ in range(2):
+            H32.replay()
+            torch.cuda.synchronize()
+            y = H32.graph_y
+            assert y.dtype == torch.float32
+            if name in out:
+                np.testing.assert_array_equal(y.numpy(), out[name])   # replays agree bit for bit
+            out[name] = y.numpy().copy()
+        # and the eager float32 chain gives the graph's bits
+        H32.outs = [None] * 4
+        np.testing.assert_array_equal(H32.apply(precision=prec).numpy(), out[name])
     return out
 
 
@@ -92,7 +108,7 @@ def test_full_fp64_chain_vs_host_replay(T, heff):
 
 
 def test_full_reduced_precision_vs_fp64(heff):
-    for name in ("fp32", "bf16x3"):
+    for name in ("fp32", "bf16x3", "fp32_f32data", "bf16x3_f32data"):
         err = relerr(heff[name], heff["y64"])
         assert err <= 1e-6, (name, err)
 
@@ -112,6 +128,10 @@ def test_reduced_precision_vs_dense_d96(T):
         y = H.apply(precision=prec)
         err = relerr(dense(y), ref)
         assert err <= 1e-6, (prec, err)
+        y32 = H.astype(torch.float32).apply(precision=prec)
+        assert y32.dtype == torch.float32
+        err = relerr(O.to_dense(y32.numpy().astype(np.float64), *oracle_legs(y32.space), O.U1Kind), ref)
+        assert err <= 1e-6, (prec, "float32 chain", err)
 
 
 def _sequential_truncation(sv, howmany, atol):
