@@ -226,6 +226,25 @@ def permute(data, cod, dom, p1, p2, kind, alpha=1.0):
     return out, ncod, ndom
 
 
+def transpose(data, cod, dom, p1, p2, kind, alpha=1.0):
+    """Oracle planar `transpose(t, (p1, p2))` for U1 and fZ2⊠U1.  A planar transpose is a
+    product of bends and folds (src/fusiontrees/duality_manipulations.jl:193-569), whose
+    coefficients are built from F-, A- and B-symbols, quantum dimensions and
+    Frobenius–Schur phases only — no R-symbol, so no Koszul sign.  For these sectors every
+    one of them is 1 (dimensions 1, F = A = B = 1, χ = 1), so each sub-block moves with
+    coefficient 1 and the result is the dense permutation of axes, for either braiding.
+    The permutation must be cyclic in the linearised leg order."""
+    n1, n2 = len(cod), len(dom)
+    lin = linearize(p1, p2, n1, n2)
+    n = n1 + n2
+    if n and any(lin[(i + 1) % n] != (lin[i] + 1) % n for i in range(n)):
+        raise ValueError(f"{(p1, p2)} is not a cyclic permutation")
+    ncod, ndom = permuted_legs(cod, dom, p1, p2)
+    dense = to_dense(data, cod, dom, kind)
+    out = from_dense(np.ascontiguousarray(dense.transpose(list(p1) + list(p2))), ncod, ndom, kind)
+    return alpha * out if alpha != 1.0 else out, ncod, ndom
+
+
 def blocks(data, cod, dom, kind):
     total, bl, _ = layout(cod, dom, kind)
     return {c: data[o:o + d1 * d2].reshape((d1, d2), order="F") for c, d1, d2, o in bl}

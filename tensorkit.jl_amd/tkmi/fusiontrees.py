@@ -13,6 +13,11 @@ the coefficients that the device permute kernels apply:
 - braid of a pair / block (fsbraid)   `src/fusiontrees/braiding_manipulations.jl:283-340`
 - linearizepermutation / permutation2swaps `src/fusiontrees/duality_manipulations.jl:571-582`,
   `src/auxiliary/auxiliary.jl:13-28`
+- multi_Fmove / multi_Fmove_inv / multi_associator (UniqueFusion)
+                                      `src/fusiontrees/basic_manipulations.jl:99-470`
+- foldright / foldleft (A-moves), cycleclockwise / cycleanticlockwise, transpose (fstranspose),
+  planar_trace / elementary_trace (tree pairs, UniqueFusion)
+                                      `src/fusiontrees/duality_manipulations.jl:193-744`
 
 Only multiplicity-free fusion (UniqueFusion, SimpleFusion) is supported:
 all vertex labels are 1.
@@ -426,3 +431,240 @@ def split_tree(f, M):
     f2 = FusionTree((coupled1,) + f.uncoupled[M:], f.coupled, (False,) + f.isdual[M:],
                     inner_ext[M:M + max(0, N - M - 1)], vert_ext[M:N])
     return f1, f2
+
+
+# --------------------------------------------------------------------------
+# planar moves on tree pairs (UniqueFusion): A-moves, cyclic rotations, transpose and
+# planar traces.  No braiding enters: the coefficients are products of F-, A- and
+# B-symbols, quantum dimensions and Frobenius–Schur phases only.
+# --------------------------------------------------------------------------
+def _conj(x):
+    return x.conjugate() if isinstance(x, complex) else x
+
+
+def _vertex_info(f, k):
+    """`vertex_info(f, k)` (`basic_manipulations.jl:103-111`), 1-based k ≥ 2: the sectors
+    left and right of uncoupled leg k (the incoming line and the outgoing line of its
+    vertex)."""
+    N = len(f.uncoupled)
+    left = f.uncoupled[0] if k == 2 else f.innerlines[k - 3]
+    right = f.coupled if k == N else f.innerlines[k - 2]
+    return left, right
+
+
+def _multi_associator(long, short):
+    """`multi_associator(long, short)` (`basic_manipulations.jl:137-167`), multiplicity-free:
+    the product of F-symbols that recouples `long` into its first leg times `short`."""
+    if long.uncoupled[1:] != short.uncoupled or long.isdual[1:] != short.isdual:
+        return 0
+    a = long.uncoupled[0]
+    coeff = 1
+    for k in range(2, len(long.uncoupled)):
+        c = long.uncoupled[k]
+        e, d = _vertex_info(long, k + 1)
+        b, e2 = _vertex_info(short, k)
+        coeff *= a.Fsymbol(b, c, d, e, e2)
+    return coeff
+
+
+def multi_Fmove_pair(f):
+    """`multi_Fmove(f)` for UniqueFusion (`basic_manipulations.jl:191-211`): the one tree
+    of the legs after the first, coupling to dual(a) ⊗ c, and its coefficient."""
+    if not f.uncoupled:
+        raise ValueError("multi_Fmove requires at least one uncoupled sector")
+    coupled = f.uncoupled[0].dual().otimes(f.coupled)[0]
+    g = FusionTree(f.uncoupled[1:], coupled, f.isdual[1:])
+    return g, _multi_associator(f, g)
+
+
+def multi_Fmove_inv_pair(a, c, f, isduala=False):
+    """`multi_Fmove_inv(a, c, f, isduala)` for UniqueFusion (`basic_manipulations.jl:
+    353-470`): a fused onto the left of f to coupled sector c, as the one standard tree,
+    with the product of conjugated F-symbols."""
+    b = f.coupled
+    if not a.Nsymbol(b, c):
+        raise ValueError(f"cannot fuse sectors {a} and {b} to {c}")
+    N = len(f.uncoupled)
+    g = FusionTree((a,) + f.uncoupled, c, (bool(isduala),) + f.isdual)
+    if N <= 1:
+        return g, 1
+    coeff = 1
+    for k in range(2, N + 1):
+        cc = f.uncoupled[k - 1]
+        bb, e2 = (f.uncoupled[0], _vertex_info(f, 2)[1]) if k == 2 else _vertex_info(f, k)
+        e, d = _vertex_info(g, k + 1)
+        coeff *= _conj(a.Fsymbol(bb, cc, d, e, e2))
+    return g, coeff
+
+
+def foldright_pair(f1, f2):
+    """`foldright((f₁, f₂))` (`duality_manipulations.jl:217-232`): the first splitting
+    vertex a ⊗ b ← c becomes the fusion vertex b ← dual(a) ⊗ c."""
+    if not f1.uncoupled:
+        raise ValueError("foldright needs a splitting leg")
+    a = f1.uncoupled[0]
+    isduala = f1.isdual[0]
+    g1, c1 = multi_Fmove_pair(f1)
+    b, c = g1.coupled, f1.coupled
+    g2, c2 = multi_Fmove_inv_pair(a.dual(), b, f2, not isduala)
+    coeff = c.sqrtdim() / b.sqrtdim() * c1 * a.Asymbol(b, c) * _conj(c2)
+    if isduala:
+        coeff *= a.frobenius_schur_phase()
+    return (g1, g2), coeff
+
+
+def foldleft_pair(f1, f2):
+    """`foldleft((f₁, f₂))` (`duality_manipulations.jl:313-317`): conj of the foldright of
+    the swapped pair."""
+    (g2, g1), coeff = foldright_pair(f2, f1)
+    return (g1, g2), _conj(coeff)
+
+
+def cycleclockwise_pair(f1, f2):
+    """`cycleclockwise` (`duality_manipulations.jl:401-410`): the last fusion leg bent to
+    the splitting side, the first splitting leg folded to the fusion side."""
+    if f1.uncoupled:
+        (t1, t2), u1 = foldright_pair(f1, f2)
+        (g1, g2), u2 = bendleft_pair(t1, t2)
+    else:
+        (t1, t2), u1 = bendleft_pair(f1, f2)
+        (g1, g2), u2 = foldright_pair(t1, t2)
+    return (g1, g2), u2 * u1
+
+
+def cycleanticlockwise_pair(f1, f2):
+    """`cycleanticlockwise` (`duality_manipulations.jl:430-440`)."""
+    if f2.uncoupled:
+        (t1, t2), u1 = foldleft_pair(f1, f2)
+        (g1, g2), u2 = bendright_pair(t1, t2)
+    else:
+        (t1, t2), u1 = bendright_pair(f1, f2)
+        (g1, g2), u2 = foldleft_pair(t1, t2)
+    return (g1, g2), u2 * u1
+
+
+def transpose_pair(f1, f2, p1, p2):
+    """`transpose((f₁, f₂), (p₁, p₂))` = `fstranspose` (`duality_manipulations.jl:518-569`):
+    repartition to len(p₁) splitting legs, then rotate the linearised legs one step at a
+    time (anticlockwise while leg 1 sits in the first half, clockwise otherwise) until leg
+    1 is where the cyclic permutation puts it.  0-based (p₁, p₂)."""
+    N = len(p1) + len(p2)
+    p = linearizepermutation(p1, p2, len(f1.uncoupled), len(f2.uncoupled))
+    (g1, g2), U = repartition_pair(f1, f2, len(p1))
+    if N == 0:
+        return (g1, g2), U
+    i1 = p.index(0) + 1
+    if i1 == 1:
+        return (g1, g2), U
+    nhalf = N >> 1
+    while 1 < i1 <= nhalf:
+        (g1, g2), u = cycleanticlockwise_pair(g1, g2)
+        U = u * U
+        i1 -= 1
+    while nhalf < i1:
+        (g1, g2), u = cycleclockwise_pair(g1, g2)
+        U = u * U
+        i1 = i1 % N + 1
+    return (g1, g2), U
+
+
+def join_trees(f1, f2):
+    """`join(f₁, f₂)` (`basic_manipulations.jl:57-97`) for UniqueFusion: f₂'s first leg is
+    f₁'s coupled sector; the joined tree has f₁'s legs followed by f₂'s others."""
+    if f2.uncoupled[0] != f1.coupled:
+        raise ValueError("cannot connect the two fusion trees")
+    return FusionTree(f1.uncoupled + f2.uncoupled[1:], f2.coupled, f1.isdual + f2.isdual[1:])
+
+
+def elementary_trace_tree(f, i):
+    """`elementary_trace(f, i)` (`duality_manipulations.jl:697-744`), 0-based i: the trace
+    of legs i and (i+1) mod N of one splitting tree → {tree: coefficient}."""
+    N = len(f.uncoupled)
+    if not (N > 1 and 0 <= i < N):
+        raise ValueError(f"cannot trace outputs {i} and {i + 1} out of only {N} outputs")
+    if not (i < N - 1 or f.coupled.isunit()):
+        raise ValueError("cannot trace the last and first output of a tree with a non-trivial coupled sector")
+    j = (i + 1) % N
+    b, b2 = f.uncoupled[i], f.uncoupled[j]
+    if not (b == b2.dual() and f.isdual[i] != f.isdual[j]):
+        return {}
+    if i < N - 1:
+        fleft, frem = split_tree(f, i)
+        ftrace, fright = split_tree(frem, 3)
+        a, d = ftrace.uncoupled[0], ftrace.coupled
+        if a != d:
+            return {}
+        coeff = b.sqrtdim()
+        if i > 0:
+            coeff *= a.Fsymbol(b, b.dual(), a, ftrace.innerlines[0], a.unit())
+        if ftrace.isdual[1]:
+            coeff *= b.frobenius_schur_phase()
+        return {join_trees(fleft, fright): coeff}
+    fleft, _ = split_tree(f, N - 1)
+    g, c0 = multi_Fmove_pair(fleft)
+    if not g.coupled.isunit():
+        return {}
+    coeff = c0 * b.sqrtdim()
+    if not f.isdual[N - 1]:
+        coeff *= _conj(b.frobenius_schur_phase())
+    return {g: coeff}
+
+
+def planar_trace_tree(f, q1, q2):
+    """`planar_trace(f, (q₁, q₂))` (`duality_manipulations.jl:631-688`), 0-based: trace the
+    pairs (q₁[k], q₂[k]), innermost neighbouring pair first."""
+    if not q1:
+        return {f: 1}
+    N = len(f.uncoupled)
+    for i, j in zip(q1, q2):
+        if not (f.uncoupled[i] == f.uncoupled[j].dual() and f.isdual[i] != f.isdual[j]):
+            return {}
+    k = None
+    for kk in range(len(q1)):
+        if (q1[kk] + 1) % N == q2[kk]:
+            i, j, k = q1[kk], q2[kk], kk
+            break
+        if (q2[kk] + 1) % N == q1[kk]:
+            i, j, k = q2[kk], q1[kk], kk
+            break
+    if k is None:
+        raise ValueError(f"indices {q1} and {q2} do not form a valid planar trace on a fusion tree with {N} legs")
+    q1r = tuple(l - (l > i) - (l > j) for n, l in enumerate(q1) if n != k)
+    q2r = tuple(l - (l > i) - (l > j) for n, l in enumerate(q2) if n != k)
+    out = {}
+    for g, c1 in elementary_trace_tree(f, i).items():
+        for h, c2 in planar_trace_tree(g, q1r, q2r).items():
+            c = c1 * c2
+            if c != 0:
+                out[h] = out.get(h, 0) + c
+    return out
+
+
+def planar_trace_pair(f1, f2, p1, p2, q1, q2):
+    """`planar_trace((f₁, f₂), (p₁, p₂), (q₁, q₂))` (`duality_manipulations.jl:575-629`),
+    UniqueFusion branch, 0-based: repartition to one splitting tree, trace it planarly,
+    transpose the rest to (p₁, p₂) → {(f₁′, f₂′): coefficient}."""
+    n1, n2 = len(f1.uncoupled), len(f2.uncoupled)
+    N = len(p1) + len(p2) + 2 * len(q1)
+    if n1 + n2 != N:
+        raise ValueError(f"fusion tree pair has {n1 + n2} indices, but the permutation expects {N}")
+    if not q1:
+        g, c = transpose_pair(f1, f2, p1, p2)
+        return {g: c}
+
+    def lin(x):
+        return x if x < n1 else 2 * n1 + n2 - 1 - x
+
+    q1l = tuple(lin(x) for x in q1)
+    q2l = tuple(lin(x) for x in q2)
+    qall = q1l + q2l
+    p1l = tuple(lin(x) - sum(lin(x) > y for y in qall) for x in p1)
+    p2l = tuple(lin(x) - sum(lin(x) > y for y in qall) for x in p2)
+    (g1, g2), c1 = repartition_pair(f1, f2, N)
+    out = {}
+    for h, c2 in planar_trace_tree(g1, q1l, q2l).items():
+        t, c3 = transpose_pair(h, g2, p1l, p2l)
+        c = c1 * c2 * c3
+        if c != 0:
+            out[t] = out.get(t, 0) + c
+    return out

@@ -23,6 +23,7 @@ UNIQUE = "unique"        # FusionStyle UniqueFusion (abelian)
 SIMPLE = "simple"        # FusionStyle SimpleFusion (multiplicity free, non-abelian)
 BOSONIC = "bosonic"
 FERMIONIC = "fermionic"
+NOBRAIDING = "nobraiding"  # BraidingStyle NoBraiding: planar operations only
 
 
 def _half(two):
@@ -146,6 +147,51 @@ class U1Irrep(Sector):
     def __repr__(self):
         q = self.charge
         return f"U1Irrep({q.numerator}/{q.denominator})" if q.denominator != 1 else f"U1Irrep({q.numerator})"
+
+
+class PlanarTrivial(Sector):
+    """The trivial sector without a braiding (TensorKit's `PlanarTrivial`, the sector of
+    `ℙ^n` spaces, `src/spaces/planarspace.jl`): one sector, every symbol 1, and only
+    planar operations (transpose, planar traces and contractions) are defined."""
+
+    __slots__ = ()
+    braiding_style = NOBRAIDING
+
+    @property
+    def _key(self):
+        return ()
+
+    @property
+    def _order(self):
+        return ()
+
+    @staticmethod
+    def unit():
+        return PlanarTrivial()
+
+    def dual(self):
+        return self
+
+    def otimes(self, b):
+        return (PlanarTrivial(),)
+
+    def Nsymbol(a, b, c):
+        return 1
+
+    def Fsymbol(a, b, c, d, e, f):
+        return 1
+
+    def Rsymbol(a, b, c):
+        raise ValueError("PlanarTrivial sectors have no braiding")
+
+    def Bsymbol(a, b, c):
+        return 1
+
+    def Asymbol(a, b, c):
+        return 1
+
+    def __repr__(self):
+        return "PlanarTrivial()"
 
 
 class FermionParity(Sector):

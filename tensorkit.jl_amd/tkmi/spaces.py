@@ -117,6 +117,13 @@ def SU2Space(*pairs, dual=False):
     return GradedSpace({SU2Irrep(j) if not isinstance(j, SU2Irrep) else j: n for j, n in pairs}, dual)
 
 
+def PlanarSpace(n, dual=False):
+    """`ℙ^n` (`src/spaces/planarspace.jl`): n dimensions of the braiding-free trivial
+    sector; its tensors admit the planar operations only."""
+    from .sectors import PlanarTrivial
+    return GradedSpace({PlanarTrivial(): int(n)}, dual)
+
+
 def Vect(pairs, dual=False):
     if isinstance(pairs, dict):
         pairs = tuple(pairs.items())

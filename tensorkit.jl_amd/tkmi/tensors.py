@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from .spaces import HomSpace, SpaceMismatch, IndexError_
-from .sectors import BOSONIC, FERMIONIC
+from .sectors import BOSONIC, FERMIONIC, NOBRAIDING
 from .spaces import ProductSpace, SectorMismatch
 from . import treetransformers as TT
 from .treetransformers import (treebraider, flat_plan, twist_factor, tracer, _make_trace_plan, _TRANSFORMER_CACHE,
@@ -231,6 +231,8 @@ def spacecheck_transform(tdst, tsrc, p1, p2):
 def add_transform_(tdst, tsrc, p1, p2, alpha, beta, levels=None, dst_twist=()):
     """`add_transform!` (`indexmanipulations.jl:557-580`) with the device kernel."""
     spacecheck_transform(tdst, tsrc, p1, p2)
+    if tsrc.space.unit.braiding_style == NOBRAIDING and not _is_identity(tsrc, p1, p2):
+        raise SectorMismatch("permute and braid need a braiding; use the planar operations (transpose)")
     if tdst.dtype != tsrc.dtype:
         raise TypeError("permute!: dtype mismatch")
     is_adj = isinstance(tsrc, AdjointTensorMap)
@@ -289,27 +291,26 @@ def _iscyclic(p):
 
 
 def transpose(t, p1=None, p2=None):
-    """`transpose(t, (p₁, p₂))` (`indexmanipulations.jl:380-392`): a planar
-    (cyclic, in the linearised leg order) repartition.  With bosonic braiding
-    the planar coefficients coincide with `permute`'s, so the same device
-    transform is used; fermionic transposes (fold moves with twists) are not
-    provided."""
+    """`transpose(t, (p₁, p₂))` (`indexmanipulations.jl:380-420`): a planar
+    (cyclic, in the linearised leg order) repartition, into a new tensor.  Default:
+    (reverse(domain), reverse(codomain))."""
     if p1 is None:
         p1 = tuple(reversed(t.domainind()))
         p2 = tuple(reversed(t.codomainind()))
     p1, p2 = _check_transpose(t, p1, p2)
-    return permute(t, p1, p2, copy=True)
+    tdst = TensorMap.empty(t.space.permute(p1, p2), t.dtype)
+    return transpose_(tdst, t, p1, p2, 1.0, 0.0)
 
 
 def _check_transpose(t, p1, p2):
     p1, p2 = tuple(p1), tuple(p2)
     n1, n2 = t.numout, t.numin
+    if sorted(p1 + p2) != list(range(t.numind)):
+        raise IndexError_(f"{(p1, p2)} is not a valid permutation for {t.space}")
     lin = [x if x < n1 else n2 + 2 * n1 - 1 - x for x in p1] + \
           [x if x < n1 else n2 + 2 * n1 - 1 - x for x in reversed(p2)]
     if not _iscyclic(lin):
         raise ValueError(f"{(p1, p2)} is not a cyclic permutation for {t.space}")
-    if t.space.unit.braiding_style == FERMIONIC:
-        raise NotImplementedError("planar transpose of fermionic tensors")
     return p1, p2
 
 
@@ -322,13 +323,12 @@ def _repartition_perm(tsrc, n1):
 
 def repartition_(tdst, tsrc, alpha=1.0, beta=0.0):
     """`repartition!(tdst, tsrc, α, β)` (`indexmanipulations.jl:445-457`): move legs
-    between codomain and domain without changing their cyclic order.  The linearised
-    leg order is unchanged, so no leg crosses another: the transform is bends only and
-    coincides with `permute!` for every braiding style."""
+    between codomain and domain without changing their cyclic order — a `transpose!`
+    (bends only: the linearised leg order is unchanged)."""
     if tsrc.numind != tdst.numind:
         raise ValueError("tsrc and tdst should have an equal amount of indices")
     p1, p2 = _repartition_perm(tsrc, tdst.numout)
-    return permute_(tdst, tsrc, p1, p2, alpha, beta)
+    return transpose_(tdst, tsrc, p1, p2, alpha, beta)
 
 
 def repartition(tsrc, n1, n2=None, copy=False):
@@ -337,7 +337,10 @@ def repartition(tsrc, n1, n2=None, copy=False):
     if n1 + n2 != tsrc.numind:
         raise ValueError(f"({n1}, {n2}) is not a partition of the {tsrc.numind} indices of {tsrc.space}")
     p1, p2 = _repartition_perm(tsrc, n1)
-    return permute(tsrc, p1, p2, copy=copy)
+    if not copy and _is_identity(tsrc, p1, p2):
+        return tsrc
+    tdst = TensorMap.empty(tsrc.space.permute(p1, p2), tsrc.dtype)
+    return transpose_(tdst, tsrc, p1, p2, 1.0, 0.0)
 
 
 def _deprecated(old, new):
@@ -364,10 +367,27 @@ def add_transpose_(tdst, tsrc, p, alpha, beta):
 
 
 def transpose_(tdst, tsrc, p1, p2, alpha=1.0, beta=0.0):
-    """`transpose!(tdst, tsrc, (p₁, p₂), α, β)` (`indexmanipulations.jl:380-392`):
-    the in-place planar transpose (bosonic sectors: a cyclic `permute!`)."""
+    """`transpose!(tdst, tsrc, (p₁, p₂), α, β)` (`indexmanipulations.jl:380-392`): the
+    planar transpose.  Bosonic sectors: the planar coefficients are the braided ones, a
+    cyclic `permute!`.  Otherwise (fermions, sectors without a braiding) the transform
+    applies the bend/fold coefficients of `transpose((f₁, f₂), p)`; the data moves as in
+    a permute, with the same rounding (tensoradd! of α·coeff·src onto β·dst)."""
     p1, p2 = _check_transpose(tsrc, p1, p2)
-    return permute_(tdst, tsrc, p1, p2, alpha, beta)
+    if tsrc.space.unit.braiding_style == BOSONIC:
+        return permute_(tdst, tsrc, p1, p2, alpha, beta)
+    spacecheck_transform(tdst, tsrc, p1, p2)
+    if tdst.dtype != tsrc.dtype:
+        raise TypeError("transpose!: dtype mismatch")
+    is_adj = isinstance(tsrc, AdjointTensorMap)
+    if not is_adj and _is_identity(tsrc, p1, p2):
+        return add_(tdst, tsrc, alpha, beta)
+    plan = TT.treetransposer(tdst.space, tsrc.space, p1, p2, adjoint_src=is_adj)
+    if _TRACE is not None:
+        _TRACE.append(("transform", plan, p1 + p2, _buf(tsrc), _buf(tdst), float(alpha), float(beta)))
+    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(),
+                                               tdst.data.data_ptr(), float(alpha), float(beta), _stream()),
+               "transpose!")
+    return tdst
 
 
 def planaradd_(C, A, p, alpha=1.0, beta=0.0):
@@ -376,12 +396,79 @@ def planaradd_(C, A, p, alpha=1.0, beta=0.0):
     return transpose_(C, A, p[0], p[1], alpha, beta)
 
 
+def _cyclicpermute(t):
+    return tuple(t[1:]) + tuple(t[:1])
+
+
+def reorder_indices(codA, domA, codB, domB, oindA, cindA, oindB, cindB, p1, p2):
+    """`reorder_indices` (`src/planar/planaroperations.jl:196-251`), 0-based: the open and
+    contracted legs of A and B in the cyclic orders a planar contraction needs —
+    A as (oindA2 ← cindA2), B as (cindB2 ← oindB2), the contracted legs paired as before
+    and C's legs (oindA2..., oindB2...) in the order (p₁, p₂) asks for."""
+    n1, n2 = len(oindA), len(oindB)
+    if len(p1) != n1 or sorted(p1) != list(range(n1)) or len(p2) != n2 or \
+            sorted(p2) != list(range(n1, n1 + n2)):
+        raise IndexError_(f"{(p1, p2)} does not place A's open legs in the codomain and B's in the domain")
+    oindA2 = tuple(oindA[i] for i in p1)
+    oindB2 = tuple(oindB[i - n1] for i in p2)
+    indA = tuple(codA) + tuple(reversed(domA))
+    indB = tuple(codB) + tuple(reversed(domB))
+    # cycle indA to (oindA2..., reverse(cindA2)...) and indB to (cindB2..., reverse(oindB2)...)
+    for _ in range(len(indA)):
+        if not oindA2 or indA[0] == oindA2[0]:
+            break
+        indA = _cyclicpermute(indA)
+    for _ in range(len(indB)):
+        if not oindB2 or indB[-1] == oindB2[0]:
+            break
+        indB = _cyclicpermute(indB)
+    if tuple(indA[:n1]) != oindA2 or any(indB[len(indB) - 1 - j] != oindB2[j] for j in range(n2)):
+        raise ValueError("the contraction is not planar: open legs are not in cyclic order")
+    nc = len(indA) - n1
+    if nc != len(indB) - n2:
+        raise IndexError_("numbers of contracted legs of A and B differ")
+    cindA2 = tuple(reversed(indA[n1:n1 + nc]))
+    cindB2 = tuple(indB[:nc])
+    # with no open legs on one side the cyclic order fixes the contracted legs only up to
+    # a rotation: align them with the pairing given by (cindA, cindB)
+    if not oindA2 and cindA:
+        hit = cindA[list(cindB).index(cindB2[0])]
+        for _ in range(len(cindA2)):
+            if hit == cindA2[0]:
+                break
+            cindA2 = _cyclicpermute(cindA2)
+    if not oindB2 and cindB:
+        hit = cindB[list(cindA).index(cindA2[0])]
+        for _ in range(len(cindB2)):
+            if hit == cindB2[0]:
+                break
+            cindB2 = _cyclicpermute(cindB2)
+    if sorted(cindA2) != sorted(cindA) or sorted(zip(cindA2, cindB2)) != sorted(zip(cindA, cindB)):
+        raise ValueError("the contraction is not planar: contracted legs cannot be paired in cyclic order")
+    return oindA2, cindA2, oindB2, cindB2
+
+
 def planarcontract_(C, A, pA, B, pB, pAB, alpha=1.0, beta=0.0, precision=None):
-    """`planarcontract!` (`src/planar/planaroperations.jl:150-190`): bosonic
-    sectors forward to `contract!`."""
-    if C.space.unit.braiding_style != "bosonic":
-        raise NotImplementedError("planarcontract! for non-bosonic sectors")
-    return contract_(C, A, pA, B, pB, pAB, alpha, beta, precision)
+    """`planarcontract!` (`src/planar/planaroperations.jl:150-190`).  Bosonic sectors
+    forward to `contract!`.  Otherwise: `reorder_indices`, the planar transposes of A into
+    (oindA ← cindA) and of B into (cindB ← oindB) where they are not already in that
+    form, and one grouped `mul!` into C."""
+    if C.space.unit.braiding_style == BOSONIC:
+        return contract_(C, A, pA, B, pB, pAB, alpha, beta, precision)
+    codA, domA = A.codomainind(), A.domainind()
+    codB, domB = B.codomainind(), B.domainind()
+    (oindA, cindA), (cindB, oindB) = (tuple(pA[0]), tuple(pA[1])), (tuple(pB[0]), tuple(pB[1]))
+    oindA, cindA, oindB, cindB = reorder_indices(codA, domA, codB, domB, oindA, cindA, oindB, cindB,
+                                                 tuple(pAB[0]), tuple(pAB[1]))
+    if oindA == codA and cindA == domA and not isinstance(A, AdjointTensorMap):
+        A2 = A
+    else:
+        A2 = transpose(A, oindA, cindA)
+    if cindB == codB and oindB == domB and not isinstance(B, AdjointTensorMap):
+        B2 = B
+    else:
+        B2 = transpose(B, cindB, oindB)
+    return mul_(C, A2, B2, alpha, beta, precision)
 
 
 def _legs(t, inds):
@@ -490,11 +577,30 @@ def tensortrace(A, p, q, conjA=False, alpha=1.0):
 
 
 def planartrace_(C, A, p, q, alpha=1.0, beta=0.0):
-    """`planartrace!` (`src/planar/planaroperations.jl:66-111`): bosonic sectors
-    forward to `trace_permute!`."""
-    if A.space.unit.braiding_style != BOSONIC:
-        raise NotImplementedError("planartrace! for non-bosonic sectors")
-    return trace_permute_(C, A, p, q, alpha, beta)
+    """`planartrace!` (`src/planar/planaroperations.jl:66-111`).  Bosonic sectors forward
+    to `trace_permute!`.  Otherwise every source tree pair contributes through the
+    coefficients of `planar_trace((f₁, f₂), p, q)` — one device transform summing the
+    traced diagonal slices, C = β·C + α·Σ coeff·(trace of A[f₁, f₂])."""
+    if A.space.unit.braiding_style == BOSONIC:
+        return trace_permute_(C, A, p, q, alpha, beta)
+    (p1, p2), (q1, q2) = (tuple(p[0]), tuple(p[1])), (tuple(q[0]), tuple(q[1]))
+    if len(q1) != len(q2):
+        raise IndexError_("number of trace indices does not match")
+    if sorted(p1 + p2 + q1 + q2) != list(range(A.numind)):
+        raise IndexError_(f"{(p1, p2)}, {(q1, q2)} do not cover the indices of {A.space}")
+    W = A.space
+    if C.space != W.select(p1, p2):
+        raise SpaceMismatch(f"trace: A = {W}, C = {C.space}, p₁ = {p1}, p₂ = {p2}")
+    if any(W[a] != W[b].dual() for a, b in zip(q1, q2)):
+        raise SpaceMismatch(f"trace: A = {W}, q₁ = {q1}, q₂ = {q2}")
+    if C.dtype != A.dtype:
+        raise TypeError("planartrace!: dtype mismatch")
+    if isinstance(A, AdjointTensorMap):
+        A = transpose(A, A.codomainind(), A.domainind())
+    plan = tracer(C.space, W, p1, p2, q1, q2, planar=True)
+    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[C.dtype], A.data.data_ptr(), C.data.data_ptr(),
+                                               float(alpha), float(beta), _stream()), "planartrace!")
+    return C
 
 
 def tr(t):

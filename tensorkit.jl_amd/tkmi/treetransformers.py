@@ -8,7 +8,9 @@ Mirrors `src/tensors/treetransformers.jl`:
   recoupling matrix U (dst trees × src trees), the common sub-block size and
   the (strides, offset) of every tree; sorted by descending
   `length(U)*prod(sz)` (`:186-205`).
-- `treebraider`/`treetransposer` LRU caches (`:156-180`, `src/auxiliary/caches.jl:135-173`).
+- `treebraider`/`treetransposer` LRU caches (`:156-180`, `src/auxiliary/caches.jl:135-173`);
+  the transposer's coefficients come from the planar moves (bends and folds), with the
+  same data movement as a permute.
 
 The serialized table is handed to `tkmi_abelian_plan_create` /
 `tkmi_generic_plan_create`, which return an immutable device plan.
@@ -22,7 +24,8 @@ from contextlib import contextmanager
 import numpy as np
 
 from . import _lib
-from .fusiontrees import FusionTree, braid_pair, braid_block, permute_block, split_tree
+from .fusiontrees import (FusionTree, braid_pair, braid_block, permute_block, split_tree, transpose_pair,
+                          planar_trace_pair)
 from .sectors import UNIQUE
 from .spaces import clear_space_caches
 
@@ -139,20 +142,21 @@ def twist_factor(tree_pair, legs, numout):
     return th
 
 
-def abelian_table(Vdst, Vsrc, p1, p2, src_structure=None, dst_twist=()):
+def abelian_table(Vdst, Vsrc, p1, p2, src_structure=None, dst_twist=(), move=braid_pair):
     """Rows of `AbelianTreeTransformer(braid, p, Vdst, Vsrc)`.
 
     `src_structure` overrides the source's sub-block structure (used for the
     lazy adjoint of a tensor); `dst_twist` lists destination legs whose twist
     is folded into the coefficient (the fermionic `twist!` after the copy in
-    `blas_contract!`, `src/tensors/tensoroperations.jl:425-441`)."""
+    `blas_contract!`, `src/tensors/tensoroperations.jl:425-441`); `move` is the tree
+    pair map (braid_pair for braid/permute, transpose_pair for the planar transpose)."""
     fts_src = src_structure if src_structure is not None else Vsrc.subblockstructure()
     fts_dst = Vdst.subblockstructure()
     n = Vsrc.numind
     coeffs, dsts, srcs, pairs = [], [], [], []
     for pair in Vsrc.fusiontrees():
         f1, f2 = pair
-        (g1, g2), coeff = braid_pair(f1, f2, p1, p2)
+        (g1, g2), coeff = move(f1, f2, p1, p2)
         if dst_twist:
             coeff *= twist_factor((g1, g2), dst_twist, Vdst.numout)
         sd = fts_dst[(g1, g2)]
@@ -285,6 +289,27 @@ def treebraider(Vdst, Vsrc, p1, p2, levels=None, *, adjoint_src=False, dst_twist
     return plan
 
 
+def treetransposer(Vdst, Vsrc, p1, p2, *, adjoint_src=False):
+    """Cached device plan for the planar `transpose!` from `Vsrc` to `Vdst` (`treetransposer`,
+    `treetransformers.jl:169-180`): the sub-blocks move as in a permute, each scaled by the
+    coefficient of `transpose((f₁, f₂), (p₁, p₂))`.  Multiplicity-free abelian sectors
+    (UniqueFusion) of any braiding style."""
+    key = ("transpose", Vdst, Vsrc, tuple(p1), tuple(p2), adjoint_src)
+    plan = _TRANSFORMER_CACHE.get(key)
+    if plan is not None:
+        return plan
+    if Vsrc.permute(p1, p2) != Vdst:
+        from .spaces import SpaceMismatch
+        raise SpaceMismatch("Incompatible spaces for transposing.")
+    if Vsrc.unit.fusion_style != UNIQUE:
+        raise NotImplementedError("planar transpose of non-abelian sectors without a symmetric braiding")
+    src_structure = adjoint_structure(Vsrc) if adjoint_src else None
+    tab = abelian_table(Vdst, Vsrc, tuple(p1), tuple(p2), src_structure, move=transpose_pair)
+    plan = _make_abelian_plan(tab, _perm0(p1, p2), Vdst.dim(), Vsrc.dim())
+    _TRANSFORMER_CACHE.put(key, plan)
+    return plan
+
+
 def adjoint_structure(Vadj):
     """Sub-block structure of the lazy adjoint t′ of a tensor t with space
     `Vadj.adjoint()`: (f₁, f₂) of t′ is t's (f₂, f₁) sub-block viewed with the
@@ -376,7 +401,7 @@ def flat_plan(length):
 # partial trace (trace_permute!) as a transform: every destination sub-block sums
 # the traced diagonal slices of the source sub-blocks that reach it
 # ---------------------------------------------------------------------------
-def trace_table(Vdst, Vsrc, p1, p2, q1, q2):
+def trace_table(Vdst, Vsrc, p1, p2, q1, q2, planar=False):
     """Rows of a trace transform, one per destination sub-block (in `fusiontrees(Vdst)`
     order): (coeffs, dst sizes, [dst (strides..., offset)], [source slice (strides...,
     offset)]).  Restates `_trace_permute!` (`src/tensors/tensoroperations.jl:245-300`):
@@ -388,7 +413,7 @@ def trace_table(Vdst, Vsrc, p1, p2, q1, q2):
     destination order."""
     fts_src = Vsrc.subblockstructure()
     fts_dst = Vdst.subblockstructure()
-    contrib = _trace_contrib(Vsrc, p1, p2, q1, q2)
+    contrib = _trace_contrib(Vsrc, p1, p2, q1, q2, planar)
     keep = tuple(p1) + tuple(p2)
     rows = []
     for dpair in Vdst.fusiontrees():
@@ -415,9 +440,19 @@ def _diag_slices(szs, sts, offs, keep, q1, q2):
             for t in itertools.product(*[range(szs[a]) for a in q1])]
 
 
-def _trace_contrib(Vsrc, p1, p2, q1, q2):
-    """Destination tree pair → [(coefficient, source tree pair)] of a trace."""
+def _trace_contrib(Vsrc, p1, p2, q1, q2, planar=False):
+    """Destination tree pair → [(coefficient, source tree pair)] of a trace; with `planar`,
+    the coefficients of the planar trace (`planar_trace`, `duality_manipulations.jl:575-629`,
+    as `planartrace!` uses them for sectors without a symmetric braiding)."""
     n1, n2 = len(p1), len(p2)
+    if planar:
+        if Vsrc.unit.fusion_style != UNIQUE:
+            raise NotImplementedError("planar trace of non-abelian sectors without a symmetric braiding")
+        contrib = {}
+        for spair in Vsrc.fusiontrees():
+            for dpair, c in planar_trace_pair(spair[0], spair[1], p1, p2, q1, q2).items():
+                contrib.setdefault(dpair, []).append((float(c), spair))
+        return contrib
     r1, r2 = tuple(p1) + tuple(q1), tuple(p2) + tuple(q2)
     contrib = {}
 
@@ -530,12 +565,13 @@ def _make_trace_plan(rows, n, dst_len, src_len):
     return plan_from_packed("trace", tuple(range(n)), pack_trace(rows), dst_len, src_len, table=rows)
 
 
-def tracer(Vdst, Vsrc, p1, p2, q1, q2):
-    """Cached device plan for `trace_permute!` from `Vsrc` to `Vdst`."""
-    key = ("trace", Vdst, Vsrc, tuple(p1), tuple(p2), tuple(q1), tuple(q2))
+def tracer(Vdst, Vsrc, p1, p2, q1, q2, planar=False):
+    """Cached device plan for `trace_permute!` (with `planar`: `planartrace!`) from `Vsrc`
+    to `Vdst`."""
+    key = ("trace", Vdst, Vsrc, tuple(p1), tuple(p2), tuple(q1), tuple(q2), planar)
     plan = _TRANSFORMER_CACHE.get(key)
     if plan is None:
-        rows = trace_table(Vdst, Vsrc, p1, p2, q1, q2)
+        rows = trace_table(Vdst, Vsrc, p1, p2, q1, q2, planar)
         plan = _make_trace_plan(rows, len(p1) + len(p2), Vdst.dim(), Vsrc.dim())
         _TRANSFORMER_CACHE.put(key, plan)
     return plan
