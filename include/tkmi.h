@@ -48,7 +48,9 @@ enum tkmi_status {
     TKMI_ERR_UNSUPPORTED = 6     /* dtype / rank not supported              */
 };
 
-enum tkmi_dtype { TKMI_F64 = 0, TKMI_F32 = 1 };
+/* Element types: Float64, Float32, ComplexF64 and ComplexF32 (interleaved (re, im) pairs,
+ * Julia's layout of Complex{T} arrays). */
+enum tkmi_dtype { TKMI_F64 = 0, TKMI_F32 = 1, TKMI_C128 = 2, TKMI_C64 = 3 };
 
 /* Arithmetic forms of the grouped GEMM. */
 enum tkmi_precision {
@@ -123,6 +125,20 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
  * (1, 2, 4, 8 trees per block).                                               */
 int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst,
                         double alpha, double beta, void *stream);
+
+/* tkmi_transform_exec for every element type, with complex α = alpha_re + i·alpha_im and
+ * β = beta_re + i·beta_im, and the source read conjugated when conj_src ≠ 0 (the data of
+ * an AdjointTensorMap of complex tensors, src/tensors/adjoint.jl:30-48, as
+ * add_transform! reads it).  The recoupling coefficients are real, so a complex element
+ * is transformed as (re, im) with the same coefficients; a complex α·coeff multiplies a
+ * one-tree block's source as Julia's complex product (four rounded products, then their
+ * difference and sum), a complex β multiplies dst likewise.  Real dtypes need zero
+ * imaginary parts (TKMI_ERR_ARGUMENT) and ignore conj_src.  Plans with gathered
+ * coefficients take real data only (TKMI_ERR_UNSUPPORTED).  Accumulation is FP64 for
+ * every element type; TKMI_C64 results are rounded to FP32 pairs when stored.      */
+int tkmi_transform_exec_ex(tkmi_transform_plan plan, int32_t dtype, int32_t conj_src, const void *src,
+                           void *dst, double alpha_re, double alpha_im, double beta_re,
+                           double beta_im, void *stream);
 
 /* Adjoint transform: dsrc = β·dsrc + α·transform†(ddst), the adjoint with
  * respect to Σ_c dim(c)⟨·,·⟩, i.e. the pullback of tkmi_transform_exec and
@@ -200,6 +216,19 @@ int tkmi_gemm_exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, 
 int tkmi_gemm_exec_dtype(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, int32_t parts,
                          double alpha, double beta, const void *A, const void *B, void *C,
                          void *stream);
+/* C_g = α·op(A_g)·op(B_g) + β·C_g for complex data (dtype TKMI_C128 or TKMI_C64, A, B and
+ * C alike), complex α = alpha_re + i·alpha_im, β likewise: the zgemm/cgemm of mul! on
+ * ComplexF64 / ComplexF32 blocks (src/tensors/linalg.jl:330-370).  conj_a / conj_b
+ * conjugate op(A) / op(B) — with transa / transb set, op is the conjugate transpose,
+ * the adjoint block of an AdjointTensorMap.  precision TKMI_PREC_F64 multiplies on FP64
+ * MFMA, TKMI_PREC_F32 on FP32 MFMA (inputs rounded to FP32), four real products per
+ * complex one; results are formed in FP64 and rounded once to C's type.  Every group of
+ * the plan runs on one 64×64-tile complex kernel (split-K partials in FP64 pairs).  The
+ * first complex call on a plan builds its complex tiles (device allocation: not during
+ * graph capture).  TKMI_PREC_BF16X3 returns TKMI_ERR_UNSUPPORTED.                 */
+int tkmi_gemm_exec_z(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, int32_t conj_a,
+                     int32_t conj_b, double alpha_re, double alpha_im, double beta_re,
+                     double beta_im, const void *A, const void *B, void *C, void *stream);
 /* Algorithmic flops routed to the MFMA kernel and to the skinny kernel.       */
 int tkmi_gemm_plan_split(tkmi_gemm_plan plan, double *flops_mfma, double *flops_skinny,
                          int64_t *skinny_groups);

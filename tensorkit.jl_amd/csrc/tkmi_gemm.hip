@@ -1002,6 +1002,212 @@ __global__ __launch_bounds__(NT) void gemm_skinny_kernel(const GGroup *__restric
 }
 
 // --------------------------------------------------------------------------------
+// Complex groups (ComplexF64 / ComplexF32 data, FP64 or FP32 MFMA): every group of the
+// plan, tiled BM×BN with BK-deep K slabs.  The real and imaginary parts of a slab are
+// staged in separate LDS arrays, each 16×16 fragment takes four real products,
+//   Re += a_r·b_r − a_i·b_i,   Im += a_r·b_i + a_i·b_r,
+// and op(A) / op(B) of an adjoint operand is the conjugate transpose (its imaginary parts
+// negated as they are staged).  One LDS buffer, the next slab waits in registers; 64-bit
+// addresses and a bounds test on every load.  Split groups write their partial products
+// as (re, im) pairs, reduced by zreduce.
+// --------------------------------------------------------------------------------
+typedef double z2_t __attribute__((ext_vector_type(2)));
+typedef float c2_t __attribute__((ext_vector_type(2)));
+
+// α·v + β·c in FP64 (complex α, β), rounded once to C's element type
+template <typename TD>
+__device__ __forceinline__ void put_z(TD *p, double vr, double vi, double ar, double ai, double br, double bi,
+                                      bool has_beta) {
+    double rr = ar * vr - ai * vi, ri = ar * vi + ai * vr;
+    if (has_beta) {
+        const double cr = (double)(*p).x, ci = (double)(*p).y;
+        rr += br * cr - bi * ci;
+        ri += br * ci + bi * cr;
+    }
+    TD out;
+    out.x = rr;
+    out.y = ri;
+    *p = out;
+}
+
+template <typename TC, typename TD, int BM, int BN, int BK>
+__global__ __launch_bounds__(NT) void gemm_z_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                    int64_t ntiles, const TD *__restrict__ A,
+                                                    const TD *__restrict__ Bm, TD *__restrict__ C,
+                                                    double *__restrict__ ws, double ar, double ai, double br,
+                                                    double bi, int conja, int conjb) {
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int AE = BM * BK / NT, BE = BN * BK / NT;
+    constexpr int KP = BK + 16 / (int)sizeof(TC);
+    __shared__ __attribute__((aligned(16))) TC Ars[BM][KP];
+    __shared__ __attribute__((aligned(16))) TC Ais[BM][KP];
+    __shared__ __attribute__((aligned(16))) TC Brs[BN][KP];
+    __shared__ __attribute__((aligned(16))) TC Bis[BN][KP];
+    using acc_t = typename MfmaK4<TC>::acc_t;
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t row0 = (lmn / G.tiles_n) * BM, col0 = (lmn % G.tiles_n) * BN;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);
+    const int64_t lda = G.lda, ldb = G.ldb;
+    const int ta = G.ta, tb = G.tb;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    acc_t cr[FM][FN], ci[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            cr[i][j] = acc_t{0, 0, 0, 0};
+            ci[i][j] = acc_t{0, 0, 0, 0};
+        }
+    auto a_el = [&](int e, int &i, int &k) {
+        const int idx = e * NT + tid;
+        if (!ta) { i = idx % BM; k = idx / BM; }
+        else     { k = idx % BK; i = idx / BK; }
+    };
+    auto b_el = [&](int e, int &j, int &k) {
+        const int idx = e * NT + tid;
+        if (!tb) { k = idx % BK; j = idx / BK; }
+        else     { j = idx % BN; k = idx / BN; }
+    };
+    TD ra[AE], rb[BE];
+    const TD zero = TD{0, 0};
+    auto load = [&](int64_t k0) {
+        const int64_t ilim = min((int64_t)BM, m - row0), jlim = min((int64_t)BN, n - col0);
+        const int64_t klim = min((int64_t)BK, K - k0);
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, k;
+            a_el(e, i, k);
+            const int64_t gi = row0 + i, gk = k0 + k;
+            ra[e] = (i < ilim && k < klim) ? A[G.a_off + (ta ? gk + gi * lda : gi + gk * lda)] : zero;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int j, k;
+            b_el(e, j, k);
+            const int64_t gj = col0 + j, gk = k0 + k;
+            rb[e] = (j < jlim && k < klim) ? Bm[G.b_off + (tb ? gj + gk * ldb : gk + gj * ldb)] : zero;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            int i, k;
+            a_el(e, i, k);
+            Ars[i][k] = (TC)ra[e].x;
+            Ais[i][k] = conja ? -(TC)ra[e].y : (TC)ra[e].y;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            int j, k;
+            b_el(e, j, k);
+            Brs[j][k] = (TC)rb[e].x;
+            Bis[j][k] = conjb ? -(TC)rb[e].y : (TC)rb[e].y;
+        }
+    };
+    constexpr int KV = 16 / (int)sizeof(TC), KC = 4 * KV;
+    static_assert(BK % KC == 0, "K slab must hold whole fragment rounds");
+    const int kq = KV * (lane >> 4);
+    auto mma_slab = [&]() {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += KC) {
+            typedef TC vk_t __attribute__((ext_vector_type(KV)));
+            vk_t far_[FM], fai[FM], fbr[FN], fbi[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                far_[i] = *reinterpret_cast<const vk_t *>(&Ars[wm * WM + i * 16 + (lane & 15)][kk + kq]);
+                fai[i] = *reinterpret_cast<const vk_t *>(&Ais[wm * WM + i * 16 + (lane & 15)][kk + kq]);
+            }
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                fbr[j] = *reinterpret_cast<const vk_t *>(&Brs[wn * WN + j * 16 + (lane & 15)][kk + kq]);
+                fbi[j] = *reinterpret_cast<const vk_t *>(&Bis[wn * WN + j * 16 + (lane & 15)][kk + kq]);
+            }
+#pragma unroll
+            for (int q = 0; q < KV; ++q)
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        cr[i][j] = MfmaK4<TC>::mma(far_[i][q], fbr[j][q], cr[i][j]);
+                        cr[i][j] = MfmaK4<TC>::mma(-fai[i][q], fbi[j][q], cr[i][j]);
+                        ci[i][j] = MfmaK4<TC>::mma(far_[i][q], fbi[j][q], ci[i][j]);
+                        ci[i][j] = MfmaK4<TC>::mma(fai[i][q], fbr[j][q], ci[i][j]);
+                    }
+        }
+    };
+    const int64_t nk = K > kbeg ? (K - kbeg + BK - 1) / BK : 0;
+    if (nk > 0) load(kbeg);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        if (kt > 0) __syncthreads();
+        store();
+        __syncthreads();
+        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+        mma_slab();
+    }
+
+    const bool partial = G.splits > 1;
+    const bool has_beta = !partial && (br != 0.0 || bi != 0.0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int64_t col = col0 + wn * WN + j * 16 + (lane & 15);
+            if (col >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = row0 + wm * WM + i * 16 + MfmaK4<TC>::row(lane, v);
+                if (row >= m) continue;
+                const double vr = (double)cr[i][j][v], vi = (double)ci[i][j][v];
+                if (partial) {
+                    double *w = ws + 2 * (G.ws_off + z * m * n + row + col * m);
+                    w[0] = vr;
+                    w[1] = vi;
+                } else {
+                    put_z(C + G.c_off + row + col * G.ldc, vr, vi, ar, ai, br, bi, has_beta);
+                }
+            }
+        }
+}
+
+// Complex split-K reduction (and, with ws = nullptr, C = β·C for α = 0): one thread per
+// output element, partial (re, im) pairs summed in fixed z order.
+template <typename TD>
+__global__ __launch_bounds__(NT) void zreduce(const RGroup *__restrict__ rg, int64_t ng, int64_t total,
+                                              const double *__restrict__ ws, TD *__restrict__ C, double ar,
+                                              double ai, double br, double bi) {
+    const bool has_beta = (br != 0.0 || bi != 0.0);
+    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < total; e += (int64_t)gridDim.x * NT) {
+        int64_t lo = 0, hi = ng - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (rg[mid].elem_start <= e) lo = mid; else hi = mid - 1;
+        }
+        const RGroup &R = rg[lo];
+        const int64_t le = e - R.elem_start;
+        const int64_t mn = R.m * R.n;
+        double vr = 0.0, vi = 0.0;
+        if (ws)
+            for (int64_t zz = 0; zz < R.splits; ++zz) {
+                vr += ws[2 * (R.ws_off + zz * mn + le)];
+                vi += ws[2 * (R.ws_off + zz * mn + le) + 1];
+            }
+        const int64_t row = le % R.m, col = le / R.m;
+        put_z(C + R.c_off + row + col * R.ldc, vr, vi, ar, ai, br, bi, has_beta);
+    }
+}
+
+// --------------------------------------------------------------------------------
 // Dot groups (m = n = 1 with K > SKINNY_K: a contraction down to a scalar, whose only
 // coupled sector is the unit): HBM-bound reductions in two passes of fixed order, FP64
 // for every precision form.  Pass 1: one workgroup per K chunk of a group writes its
@@ -1132,6 +1338,14 @@ struct tkmi_gemm_plan_s {
     std::vector<RGroup> scale[2];
     int64_t scale_total[2] = {0, 0};
     RGroup *d_scale[2] = {nullptr, nullptr};
+    // complex data: every group with m, n > 0 on the complex kernel (tiles made at the
+    // first complex call), and the C blocks of all groups for α = 0
+    std::vector<GGroup> zall;
+    TileSet zts;
+    bool zready = false;
+    std::vector<RGroup> zscale;
+    int64_t zscale_total = 0;
+    RGroup *d_zscale = nullptr;
 };
 
 namespace {
@@ -1332,6 +1546,60 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     return rc;
 }
 
+// Tiles of the complex kernel (64×64×16) over every group of the plan, with the split-K
+// factor of the same schedule model; partial products take two doubles per element.
+constexpr int ZBM = 64, ZBN = 64, ZBK = 16;
+
+int build_ztiles(tkmi_gemm_plan_s *p) {
+    TileSet &ts = p->zts;
+    const int64_t slots = cfg_slots(5);
+    int64_t S = 1;
+    double best = split_model(p->zall, 1, ZBM, ZBN, ZBK, slots);
+    for (int64_t cand = 2; cand <= MAX_SPLIT; ++cand) {
+        const double t = split_model(p->zall, cand, ZBM, ZBN, ZBK, slots);
+        if (t < best * 0.98) {
+            best = t;
+            S = cand;
+        }
+    }
+    ts.split = S;
+    int64_t ws_len = 0;
+    std::vector<GGroup> order;
+    for (auto g : p->zall) {
+        g.tiles_m = (g.m + ZBM - 1) / ZBM;
+        g.tiles_n = (g.n + ZBN - 1) / ZBN;
+        split_group(g, S, ZBK);
+        g.ws_off = 0;
+        if (g.splits > 1) {
+            g.ws_off = ws_len;
+            ws_len += g.splits * g.m * g.n;
+            ts.rgroups.push_back(RGroup{g.m, g.n, g.c_off, g.ldc, g.ws_off, g.splits, ts.relems});
+            ts.relems += g.m * g.n;
+        }
+        order.push_back(g);
+        p->zscale.push_back(RGroup{g.m, g.n, g.c_off, g.ldc, 0, 0, p->zscale_total});
+        p->zscale_total += g.m * g.n;
+    }
+    std::stable_sort(order.begin(), order.end(), [](const GGroup &a, const GGroup &b) {
+        return std::min(a.k, a.kchunk) > std::min(b.k, b.kchunk);
+    });
+    int64_t start = 0;
+    for (auto &g : order) {
+        g.tile_start = start;
+        start += g.tiles_m * g.tiles_n * g.splits;
+        ts.groups.push_back(g);
+    }
+    ts.ntiles = start;
+    int rc = TKMI_OK;
+    if (!ts.groups.empty()) rc = tkmi::upload((void **)&ts.d_groups, ts.groups.data(), ts.groups.size() * sizeof(GGroup));
+    if (!rc && !ts.rgroups.empty())
+        rc = tkmi::upload((void **)&ts.d_rgroups, ts.rgroups.data(), ts.rgroups.size() * sizeof(RGroup));
+    if (!rc && ws_len > 0) rc = tkmi::upload((void **)&ts.ws, nullptr, 2 * ws_len * sizeof(double));
+    if (!rc && !p->zscale.empty())
+        rc = tkmi::upload((void **)&p->d_zscale, p->zscale.data(), p->zscale.size() * sizeof(RGroup));
+    return rc;
+}
+
 int auto_cfg(const tkmi_gemm_plan_s *) {
     // 64×64×16 with one LDS buffer: the fastest on the sector GEMMs measured (K or N of
     // a few hundred); the autotuner moves a plan elsewhere when another is > 3% faster
@@ -1463,6 +1731,10 @@ int launch_dots(const DotSet &ds, const TD *A, const TD *B, TD *C, double alpha,
 
 void free_gemm(tkmi_gemm_plan_s *p) {
     if (!p) return;
+    (void)hipFree(p->zts.d_groups);
+    (void)hipFree(p->zts.d_rgroups);
+    (void)hipFree(p->zts.ws);
+    (void)hipFree(p->d_zscale);
     (void)hipFree(p->panels);
     (void)hipFree(p->d_pack);
     (void)hipFree(p->skinny.d_groups);
@@ -1517,6 +1789,41 @@ int exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alp
     if (parts & 2) rc = launch_skinny<TD>(plan->skinny, A, B, C, alpha, beta, s);
     if (!rc && (parts & 2)) rc = launch_dots<TD>(plan->dots, A, B, C, alpha, beta, s);
     return rc;
+}
+
+template <typename TC, typename TD>
+int zexec(tkmi_gemm_plan p, double ar, double ai, double br, double bi, int conja, int conjb, const TD *A,
+          const TD *B, TD *C, hipStream_t s) {
+    if (!p->zready) {
+        // first complex call on this plan: its tiles (not during graph capture)
+        if (int rc = build_ztiles(p)) return rc;
+        p->zready = true;
+    }
+    if (ar == 0.0 && ai == 0.0) {
+        // BLAS convention: A and B are not read, C = β·C (β = 0 writes zeros)
+        if (p->zscale_total > 0) {
+            const int64_t blocks = std::min<int64_t>((p->zscale_total + NT - 1) / NT, 8192);
+            hipLaunchKernelGGL(zreduce<TD>, dim3((unsigned)blocks), dim3(NT), 0, s, p->d_zscale,
+                               (int64_t)p->zscale.size(), p->zscale_total, (const double *)nullptr, C, 0.0, 0.0,
+                               br, bi);
+        }
+        TKMI_HIP(hipGetLastError());
+        return TKMI_OK;
+    }
+    const TileSet &ts = p->zts;
+    if (ts.ntiles == 0) return TKMI_OK;
+    if (ts.ntiles * NT >= ((int64_t)1 << 31))
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: more output tiles than one launch can hold (2^31 work-items)");
+    hipLaunchKernelGGL((gemm_z_kernel<TC, TD, ZBM, ZBN, ZBK>), dim3((unsigned)ts.ntiles), dim3(NT), 0, s,
+                       ts.d_groups, (int64_t)ts.groups.size(), ts.ntiles, A, B, C, ts.ws, ar, ai, br, bi, conja,
+                       conjb);
+    if (ts.relems > 0) {
+        const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
+        hipLaunchKernelGGL(zreduce<TD>, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
+                           (int64_t)ts.rgroups.size(), ts.relems, (const double *)ts.ws, C, ar, ai, br, bi);
+    }
+    TKMI_HIP(hipGetLastError());
+    return TKMI_OK;
 }
 
 template <typename TD>
@@ -1616,6 +1923,7 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
         const double f = 2.0 * (double)G.m * (double)G.n * (double)G.k;
         p->flops += f;
         if (G.m == 0 || G.n == 0) continue;
+        p->zall.push_back(G);
         if (dot_shape(G.m, G.n, G.k)) {
             // a dot product: op(A) is 1×k, op(B) is k×1
             constexpr int64_t DOT_CHUNK = 8192, DOT_MAXPARTS = 1024;
@@ -1756,6 +2064,30 @@ int tkmi_gemm_exec_dtype(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, 
     if (dtype == TKMI_F32)
         return exec_parts<float>(plan, precision, parts, alpha, beta, (const float *)A, (const float *)B, (float *)C, s);
     return exec_parts<double>(plan, precision, parts, alpha, beta, (const double *)A, (const double *)B, (double *)C, s);
+}
+
+int tkmi_gemm_exec_z(tkmi_gemm_plan plan, int32_t precision, int32_t dtype, int32_t conj_a, int32_t conj_b,
+                     double alpha_re, double alpha_im, double beta_re, double beta_im, const void *A, const void *B,
+                     void *C, void *stream) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (int rc = check_prec(precision)) return rc;
+    if (precision == TKMI_PREC_BF16X3)
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "gemm: complex data runs in TKMI_PREC_F64 or TKMI_PREC_F32");
+    if (dtype != TKMI_C128 && dtype != TKMI_C64) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: dtype must be TKMI_C128 or TKMI_C64");
+    hipStream_t s = tkmi::as_stream(stream);
+    const int ca = conj_a != 0, cb = conj_b != 0;
+    if (dtype == TKMI_C128) {
+        auto a = (const z2_t *)A, b = (const z2_t *)B;
+        auto c = (z2_t *)C;
+        return precision == TKMI_PREC_F64
+                   ? zexec<double, z2_t>(plan, alpha_re, alpha_im, beta_re, beta_im, ca, cb, a, b, c, s)
+                   : zexec<float, z2_t>(plan, alpha_re, alpha_im, beta_re, beta_im, ca, cb, a, b, c, s);
+    }
+    auto a = (const c2_t *)A, b = (const c2_t *)B;
+    auto c = (c2_t *)C;
+    return precision == TKMI_PREC_F64
+               ? zexec<double, c2_t>(plan, alpha_re, alpha_im, beta_re, beta_im, ca, cb, a, b, c, s)
+               : zexec<float, c2_t>(plan, alpha_re, alpha_im, beta_re, beta_im, ca, cb, a, b, c, s);
 }
 
 int tkmi_gemm_plan_autotune(tkmi_gemm_plan plan, int32_t precision, const double *A, const double *B, double *C,

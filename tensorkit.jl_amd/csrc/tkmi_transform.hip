@@ -37,6 +37,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -91,14 +92,69 @@ __device__ __forceinline__ double scaled_sum(double beta, double c, double v) {
     return bc + v;
 }
 
+// Complex elements: (re, im) pairs, accumulated in FP64 pairs.  The recoupling
+// coefficients are real; α, β and the conjugation of the source may be complex / set.
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+
+template <typename T>
+struct Acc {
+    typedef double type;
+    static constexpr bool cplx = false;
+};
+template <>
+struct Acc<f2_t> {
+    typedef d2_t type;
+    static constexpr bool cplx = true;
+};
+template <>
+struct Acc<d2_t> {
+    typedef d2_t type;
+    static constexpr bool cplx = true;
+};
+
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ double widen(float v) { return (double)v; }
+__device__ __forceinline__ d2_t widen(d2_t v) { return v; }
+__device__ __forceinline__ d2_t widen(f2_t v) { return __builtin_convertvector(v, d2_t); }
+template <typename T>
+__device__ __forceinline__ T narrow(typename Acc<T>::type v) {
+    if constexpr (std::is_same<T, f2_t>::value)
+        return __builtin_convertvector(v, f2_t);
+    else
+        return (T)v;
+}
+template <typename T>
+__device__ __forceinline__ T conj_of(T v) {
+    if constexpr (Acc<T>::cplx) v.y = -v.y;
+    return v;
+}
+
+// (a.x + i a.y)·(b.x + i b.y) with the four products rounded before they are combined,
+// as Julia's complex multiply (no fused multiply-add)
+__device__ __forceinline__ d2_t cmul(d2_t a, d2_t b) {
+    double p1 = a.x * b.x, p2 = a.y * b.y, p3 = a.x * b.y, p4 = a.y * b.x;
+    asm volatile("" : "+v"(p1), "+v"(p2), "+v"(p3), "+v"(p4));
+    return d2_t{p1 - p2, p3 + p4};
+}
+
+// The scalars of one transform call: α = ar + i·ai, β = br + i·bi (imaginary parts zero for
+// real data) and whether the source is read conjugated (complex data only).
+struct Scal {
+    double ar, ai, br, bi;
+    int conj;
+};
+
 // One tile of one work item.  COPY: the tile is nr runs of nw elements along dW
 // (contiguous on both sides), numbered run after run; otherwise a tW × tR tile
 // read with lanes along dR and written with lanes along dW through LDS.
 template <typename T, int L, bool COPY, bool SLICED>
 __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__restrict__ trees,
                                                const double *__restrict__ Umat, const T *__restrict__ src,
-                                               T *__restrict__ dst, double alpha, double beta, bool keep,
+                                               T *__restrict__ dst, const Scal &sc, bool keep,
                                                int64_t t, T *__restrict__ lds, int dB_in, int lgB) {
+    using A = typename Acc<T>::type;
+    const double alpha = sc.ar, beta = sc.br;
     constexpr int E = class_elems(L);
     const int nread = B.nread, nwrite = B.nwrite;
     const int dW = B.dW, dR = B.dR;
@@ -219,20 +275,21 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
         return w < nw && r < nr && b < nb;
     };
 
-    const bool has_alpha = (alpha != 0.0);  // α = 0: the source is not read (BLAS convention)
+    const bool has_alpha = (alpha != 0.0 || sc.ai != 0.0);  // α = 0: the source is not read (BLAS convention)
     // One source tree (every abelian entry, and generic blocks of one tree): the
     // reference scales by α·coeff (tensoradd! with α*coeff, indexmanipulations.jl:658-669,
     // 684-690), so α is folded into the coefficient.  Several source trees: the reference
     // forms Σ_i U[j,i]·src_i (the recoupling GEMM, :710-711) and scales that by α when it
     // inserts the column (:715-720), so α is applied to the sum at the store.
-    // Sums are FP64 for both element types: FP32 data with the Float64 scalars of the
-    // reference's tensoradd! is promoted to Float64 and rounded once when stored.
+    // Sums are FP64 for every element type: FP32 data with the Float64 scalars of the
+    // reference's tensoradd! is promoted to Float64 and rounded once when stored.  Complex
+    // data: a complex α·coeff multiplies the source as a complex product.
     const bool single = (nread == 1);
-    double o[L][E];
+    A o[L][E];
 #pragma unroll
     for (int j = 0; j < L; ++j)
 #pragma unroll
-        for (int k = 0; k < E; ++k) o[j][k] = 0.0;
+        for (int k = 0; k < E; ++k) o[j][k] = (A)0.0;
 
     for (int ic = 0; ic < nread && has_alpha; ic += L) {
         const int nc = min(L, nread - ic);
@@ -250,6 +307,8 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                     x[i][k] = pos_read(k, w, r, b)
                                   ? __builtin_nontemporal_load(&sp[(int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB])
                                   : (T)0;
+                    if constexpr (Acc<T>::cplx)
+                        if (sc.conj) x[i][k] = conj_of(x[i][k]);
                 }
             }
         }
@@ -282,9 +341,17 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 for (int i = 0; i < L; ++i) {
                     if (i < nc) {
                         const double uc = Umat[B.u_start + j + (int64_t)(ic + i) * B.u_ld];
-                        const double u = single ? alpha * uc : uc;
+                        if (Acc<T>::cplx && single && sc.ai != 0.0) {
+                            if constexpr (Acc<T>::cplx) {
+                                const d2_t u = {alpha * uc, sc.ai * uc};
 #pragma unroll
-                        for (int k = 0; k < E; ++k) o[j][k] += u * (double)x[i][k];
+                                for (int k = 0; k < E; ++k) o[j][k] += cmul(u, widen(x[i][k]));
+                            }
+                        } else {
+                            const double u = single ? alpha * uc : uc;
+#pragma unroll
+                            for (int k = 0; k < E; ++k) o[j][k] += u * widen(x[i][k]);
+                        }
                     }
                 }
             }
@@ -292,14 +359,27 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     }
 
     // several source trees: α scales the recoupling sum (α = 1 leaves it as it is)
-    if (!single && alpha != 1.0) {
+    if (!single && (alpha != 1.0 || sc.ai != 0.0)) {
 #pragma unroll
         for (int j = 0; j < L; ++j)
 #pragma unroll
-            for (int k = 0; k < E; ++k) o[j][k] = alpha * o[j][k];
+            for (int k = 0; k < E; ++k) {
+                if constexpr (Acc<T>::cplx)
+                    o[j][k] = sc.ai != 0.0 ? cmul(d2_t{alpha, sc.ai}, o[j][k]) : alpha * o[j][k];
+                else
+                    o[j][k] = alpha * o[j][k];
+            }
     }
-    const bool has_beta = (beta != 0.0);
-    auto with_beta = [&](double v, double c) { return scaled_sum(beta, c, v); };
+    const bool has_beta = (beta != 0.0 || sc.bi != 0.0);
+    auto with_beta = [&](A v, T c) -> A {
+        if constexpr (Acc<T>::cplx) {
+            const d2_t cw = widen(c);
+            if (sc.bi == 0.0) return d2_t{scaled_sum(beta, cw.x, v.x), scaled_sum(beta, cw.y, v.y)};
+            return cmul(d2_t{beta, sc.bi}, cw) + v;
+        } else {
+            return scaled_sum(beta, (double)c, v);
+        }
+    };
     auto store_all = [&](auto store) {
 #pragma unroll
         for (int j = 0; j < L; ++j) {
@@ -314,7 +394,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                         int w, r, b;
                         if (pos_write(k, w, r, b)) {
                             T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
-                            store((T)with_beta(o[j][k], (double)*a), a);
+                            store(narrow<T>(with_beta(o[j][k], *a)), a);
                         }
                     }
                 } else {
@@ -323,7 +403,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                         int w, r, b;
                         if (pos_write(k, w, r, b)) {
                             T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
-                            store((T)o[j][k], a);
+                            store(narrow<T>(o[j][k]), a);
                         }
                     }
                 }
@@ -344,8 +424,8 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 template <typename T, int L, bool STAGED, bool SLICED>
 __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     const TBlock *__restrict__ blocks, int64_t nblocks, const TTree *__restrict__ trees,
-    const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, double alpha,
-    double beta, int keep, int64_t tile0, const int32_t *__restrict__ slices) {
+    const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, Scal sc, int keep,
+    int64_t tile0, const int32_t *__restrict__ slices) {
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
     // binary search: last item with tile_start <= gt
     int64_t lo = 0, hi = nblocks - 1;
@@ -361,18 +441,18 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
         constexpr int LDSN = L * (class_area(L) + class_twmax(L));  // tW·(tR+1) ≤ area + tW per tree
         __shared__ T lds[LDSN];
         if (B.mode == 0) {
-            transform_tile<T, L, true, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds,
+            transform_tile<T, L, true, false>(B, trees, Umat, src, dst, sc, keep, gt - B.tile_start, lds,
                                               -1, 0);
         } else if (SLICED) {
             const int32_t sl = slices[lo];
-            transform_tile<T, L, false, true>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds,
+            transform_tile<T, L, false, true>(B, trees, Umat, src, dst, sc, keep, gt - B.tile_start, lds,
                                               sl < 0 ? -1 : (sl & 0xff), sl < 0 ? 0 : (sl >> 8));
         } else {
-            transform_tile<T, L, false, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, lds,
+            transform_tile<T, L, false, false>(B, trees, Umat, src, dst, sc, keep, gt - B.tile_start, lds,
                                                -1, 0);
         }
     } else {
-        transform_tile<T, L, true, false>(B, trees, Umat, src, dst, alpha, beta, keep, gt - B.tile_start, nullptr,
+        transform_tile<T, L, true, false>(B, trees, Umat, src, dst, sc, keep, gt - B.tile_start, nullptr,
                                           -1, 0);
     }
 }
@@ -763,7 +843,7 @@ int add_item(tkmi_transform_plan_s *p, std::vector<HostItem> &items, const std::
 
 template <typename T, int L>
 void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const double *dU, const T *src, T *dst,
-                  double alpha, double beta, hipStream_t stream) {
+                  const Scal &sc, hipStream_t stream) {
     const int keep = side.written * (int64_t)sizeof(T) <= KEEP_BYTES;
     // the dispatch packet holds the grid size in work-items as 32 bits: keep
     // blocks × NTHREADS below 2^31 per launch and continue at tile0 = t0
@@ -772,26 +852,26 @@ void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const do
         const unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
         if (side.staged && side.sliced)
             hipLaunchKernelGGL((transform_kernel<T, L, true, true>), dim3(grid), dim3(NTHREADS), 0, stream,
-                               side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta,
+                               side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, sc,
                                keep, t0, (const int32_t *)side.d_slices);
         else if (side.staged)
             hipLaunchKernelGGL((transform_kernel<T, L, true, false>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
-                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta, keep, t0,
+                               (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, sc, keep, t0,
                                (const int32_t *)nullptr);
         else
             hipLaunchKernelGGL((transform_kernel<T, L, false, false>), dim3(grid), dim3(NTHREADS), 0, stream,
-                               side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, alpha, beta,
+                               side.d_blocks, (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, sc,
                                keep, t0, (const int32_t *)nullptr);
     }
 }
 
 template <typename T>
 int launch_side(const tkmi_transform_plan_s *p, const PlanSide *sides, const double *dU, const T *src, T *dst,
-                double alpha, double beta, hipStream_t stream) {
-    if (sides[0].ntiles) launch_class<T, 1>(p, sides[0], dU, src, dst, alpha, beta, stream);
-    if (sides[1].ntiles) launch_class<T, 2>(p, sides[1], dU, src, dst, alpha, beta, stream);
-    if (sides[2].ntiles) launch_class<T, 4>(p, sides[2], dU, src, dst, alpha, beta, stream);
-    if (sides[3].ntiles) launch_class<T, 8>(p, sides[3], dU, src, dst, alpha, beta, stream);
+                const Scal &sc, hipStream_t stream) {
+    if (sides[0].ntiles) launch_class<T, 1>(p, sides[0], dU, src, dst, sc, stream);
+    if (sides[1].ntiles) launch_class<T, 2>(p, sides[1], dU, src, dst, sc, stream);
+    if (sides[2].ntiles) launch_class<T, 4>(p, sides[2], dU, src, dst, sc, stream);
+    if (sides[3].ntiles) launch_class<T, 8>(p, sides[3], dU, src, dst, sc, stream);
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
@@ -802,23 +882,40 @@ int64_t total_tiles(const PlanSide *sides) {
     return n;
 }
 
-int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst, double alpha, double beta,
-                void *stream, bool adjoint, const void *gsrc = nullptr) {
+size_t elem_size(int32_t dtype) {
+    switch (dtype) {
+        case TKMI_F64: return 8;
+        case TKMI_F32: return 4;
+        case TKMI_C128: return 16;
+        case TKMI_C64: return 8;
+        default: return 0;
+    }
+}
+
+int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst, const Scal &sc, void *stream,
+                bool adjoint, const void *gsrc = nullptr) {
     if (!p) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
+    const double alpha = sc.ar;
+    const bool has_alpha = sc.ar != 0.0 || sc.ai != 0.0;
     const PlanSide *sides = adjoint ? p->adj : p->fwd;
     const double *dU = adjoint ? p->d_Ut : p->d_U;
     // a source may be empty (null) when no block reads it: e.g. a contraction with an
     // empty operand into a non-empty C only scales C by β
     bool reads = false;
     for (int c = 0; c < NCLASS; ++c) reads |= sides[c].ntiles > 0 && sides[c].reads;
-    if ((total_tiles(sides) > 0 && !dst) || (reads && alpha != 0.0 && !src))
+    (void)alpha;
+    if ((total_tiles(sides) > 0 && !dst) || (reads && has_alpha && !src))
         TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");
-    if (dtype != TKMI_F64 && dtype != TKMI_F32)
-        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64 or TKMI_F32");
+    const size_t es0 = elem_size(dtype);
+    if (!es0) TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: dtype must be TKMI_F64, TKMI_F32, TKMI_C128 or TKMI_C64");
+    if ((dtype == TKMI_F64 || dtype == TKMI_F32) && (sc.ai != 0.0 || sc.bi != 0.0))
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: complex scalars need complex data");
+    if (p->gather_len > 0 && dtype != TKMI_F64 && dtype != TKMI_F32)
+        TKMI_FAIL(TKMI_ERR_UNSUPPORTED, "transform: gathered coefficients need real data");
     // the source and destination vectors must not overlap (tiles read and write them in
     // no particular order), except an elementwise plan run in place, or α = 0 (no reads)
-    if (reads && alpha != 0.0 && src && dst) {
-        const uintptr_t es = dtype == TKMI_F64 ? 8 : 4;
+    if (reads && has_alpha && src && dst) {
+        const uintptr_t es = es0;
         const uintptr_t s0 = (uintptr_t)src, d0 = (uintptr_t)dst;
         const uintptr_t s1 = s0 + es * (uintptr_t)(adjoint ? p->dst_len : p->src_len);
         const uintptr_t d1 = d0 + es * (uintptr_t)(adjoint ? p->src_len : p->dst_len);
@@ -839,9 +936,12 @@ int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst
             hipLaunchKernelGGL(gather_coeffs<float>, dim3(grid), dim3(NTHREADS), 0, s, p->d_gcoef, p->d_gidx,
                                (const float *)gsrc, p->gather_len, p->d_U);
     }
-    if (dtype == TKMI_F64)
-        return launch_side<double>(p, sides, dU, (const double *)src, (double *)dst, alpha, beta, s);
-    return launch_side<float>(p, sides, dU, (const float *)src, (float *)dst, alpha, beta, s);
+    switch (dtype) {
+        case TKMI_F64: return launch_side<double>(p, sides, dU, (const double *)src, (double *)dst, sc, s);
+        case TKMI_F32: return launch_side<float>(p, sides, dU, (const float *)src, (float *)dst, sc, s);
+        case TKMI_C128: return launch_side<d2_t>(p, sides, dU, (const d2_t *)src, (d2_t *)dst, sc, s);
+        default: return launch_side<f2_t>(p, sides, dU, (const f2_t *)src, (f2_t *)dst, sc, s);
+    }
 }
 
 bool in_range(const int64_t *sz, const int64_t *st, int64_t off, int N, int64_t len) {
@@ -988,12 +1088,18 @@ int tkmi_generic_plan_create(tkmi_transform_plan *plan, int32_t numind, const in
 
 int tkmi_transform_exec(tkmi_transform_plan plan, int32_t dtype, const void *src, void *dst, double alpha,
                         double beta, void *stream) {
-    return exec_common(plan, dtype, src, dst, alpha, beta, stream, false);
+    return exec_common(plan, dtype, src, dst, Scal{alpha, 0.0, beta, 0.0, 0}, stream, false);
+}
+
+int tkmi_transform_exec_ex(tkmi_transform_plan plan, int32_t dtype, int32_t conj_src, const void *src, void *dst,
+                           double alpha_re, double alpha_im, double beta_re, double beta_im, void *stream) {
+    const int conj = (conj_src != 0) && (dtype == TKMI_C128 || dtype == TKMI_C64);
+    return exec_common(plan, dtype, src, dst, Scal{alpha_re, alpha_im, beta_re, beta_im, conj}, stream, false);
 }
 
 int tkmi_transform_adjoint_exec(tkmi_transform_plan plan, int32_t dtype, const void *ddst, void *dsrc,
                                 double alpha, double beta, void *stream) {
-    return exec_common(plan, dtype, ddst, dsrc, alpha, beta, stream, true);
+    return exec_common(plan, dtype, ddst, dsrc, Scal{alpha, 0.0, beta, 0.0, 0}, stream, true);
 }
 
 int tkmi_transform_plan_set_gather(tkmi_transform_plan plan, int64_t n, const double *coef, const int64_t *idx) {
@@ -1013,7 +1119,7 @@ int tkmi_transform_exec_gather(tkmi_transform_plan plan, int32_t dtype, const vo
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null plan");
     if (!plan->d_gcoef) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_exec_gather: plan has no gather");
     if (plan->gather_len > 0 && !gsrc) TKMI_FAIL(TKMI_ERR_ARGUMENT, "tkmi_transform_exec_gather: null gather source");
-    return exec_common(plan, dtype, src, dst, alpha, beta, stream, false, gsrc);
+    return exec_common(plan, dtype, src, dst, Scal{alpha, 0.0, beta, 0.0, 0}, stream, false, gsrc);
 }
 
 int tkmi_transform_plan_info(tkmi_transform_plan plan, int64_t *ntiles, int64_t *nentries, int64_t *elements) {

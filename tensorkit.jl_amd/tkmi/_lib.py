@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtkmi.so")
 
 TKMI_OK = 0
-TKMI_F64, TKMI_F32 = 0, 1
+TKMI_F64, TKMI_F32, TKMI_C128, TKMI_C64 = 0, 1, 2, 3
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 SVD_QRITERATION, SVD_DIVIDEANDCONQUER, SVD_JACOBI = 0, 1, 2
 
@@ -39,6 +39,8 @@ SIGNATURES = [
      [C.POINTER(c_vp), C.c_int32, c_i32p, C.c_int64, c_i64p, c_i64p, c_f64p, c_i64p, c_i64p,
       c_i64p, c_i64p, c_f64p, c_f64p, C.c_int64, C.c_int64]),
     ("tkmi_transform_exec", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, C.c_double, C.c_double, c_vp]),
+    ("tkmi_transform_exec_ex", C.c_int, [c_vp, C.c_int32, C.c_int32, c_vp, c_vp, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, c_vp]),
     ("tkmi_transform_adjoint_exec", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, C.c_double, C.c_double, c_vp]),
     ("tkmi_transform_plan_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
     ("tkmi_transform_plan_destroy", C.c_int, [c_vp]),
@@ -52,6 +54,8 @@ SIGNATURES = [
                                        c_vp]),
     ("tkmi_gemm_exec_dtype", C.c_int, [c_vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, c_vp, c_vp,
                                        c_vp, c_vp]),
+    ("tkmi_gemm_exec_z", C.c_int, [c_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, c_vp, c_vp, c_vp, c_vp]),
     ("tkmi_gemm_plan_split", C.c_int, [c_vp, c_f64p, c_f64p, c_i64p]),
     ("tkmi_gemm_plan_info", C.c_int, [c_vp, c_i64p, c_f64p]),
     ("tkmi_gemm_plan_splitk", C.c_int, [c_vp, C.c_int32, c_i64p, c_i64p]),

@@ -46,15 +46,15 @@ def _pair_map(Wdst, Wsrc, p1, p2, dst_twist):
 
 
 def eligible(C, A, pA, B, pB):
-    """The fused form applies to abelian sectors, plain (non-adjoint) operands of one
-    element type (the transform reads A, B and C with C's), a tiny B and skinny GEMMs
-    in every coupled sector.  Mixed element types fall through to the unfused path,
+    """The fused form applies to abelian sectors, plain (non-adjoint) operands of one real
+    element type (the transform reads A, B and C with C's, and its coefficients, which
+    hold B's elements, are real), a tiny B and skinny GEMMs in every coupled sector.  Mixed element types fall through to the unfused path,
     which raises for them."""
     if C.space.unit.fusion_style != UNIQUE:
         return False
     if any(type(t).__name__ != "TensorMap" for t in (A, B, C)):
         return False
-    if not (A.dtype == B.dtype == C.dtype):
+    if not (A.dtype == B.dtype == C.dtype) or A.dtype.is_complex:
         return False
     if B.space.dim() > MAX_B_ELEMENTS or len(pA[0]) == 0:
         return False

@@ -31,7 +31,11 @@ from . import fusedcontract as _FC
 # transform (see fusedcontract); False restores permute + mul! + permute
 FUSE_SMALL_OPERAND = True
 
-_DT = {torch.float64: _lib.TKMI_F64, torch.float32: _lib.TKMI_F32}
+_DT = {torch.float64: _lib.TKMI_F64, torch.float32: _lib.TKMI_F32,
+       torch.complex128: _lib.TKMI_C128, torch.complex64: _lib.TKMI_C64}
+_COMPLEX = (torch.complex128, torch.complex64)
+_NP = {torch.float64: np.float64, torch.float32: np.float32, torch.complex128: np.complex128,
+       torch.complex64: np.complex64}
 
 DEFAULT_PRECISION = _lib.PREC_F64
 
@@ -64,6 +68,27 @@ def stop_trace():
     return tr
 
 
+def _texec(plan, dtype, src_ptr, dst_ptr, alpha, beta, conj=False, what="transform"):
+    """One transform launch, dst = β·dst + α·transform(src); complex element types take
+    complex α and β and read the source conjugated when `conj` (the data of the adjoint of
+    a complex tensor)."""
+    lib = _lib.load()
+    a, b = complex(alpha), complex(beta)
+    if dtype in _COMPLEX:
+        _lib.check(lib.tkmi_transform_exec_ex(plan.handle, _DT[dtype], int(bool(conj)), src_ptr, dst_ptr, a.real,
+                                              a.imag, b.real, b.imag, _stream()), what)
+        return
+    if a.imag or b.imag:
+        raise TypeError(f"{what}: complex scalars need complex tensors")
+    _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[dtype], src_ptr, dst_ptr, a.real, b.real, _stream()), what)
+
+
+def _scalar(x):
+    """A scalar as the trace records it (complex only when it has an imaginary part)."""
+    z = complex(x)
+    return z if z.imag else z.real
+
+
 def _buf(t):
     # the recorded entry holds the device tensor itself, so a recorded buffer
     # stays allocated (and its address valid) for as long as the trace lives
@@ -79,7 +104,7 @@ class TensorMap:
         if data.dim() != 1 or data.numel() != space.dim():
             raise SpaceMismatch(f"data length {data.numel()} does not match dim {space.dim()} of {space}")
         if data.dtype not in _DT:
-            raise TypeError("tkmi TensorMap data must be float64 or float32")
+            raise TypeError("tkmi TensorMap data must be float64, float32, complex128 or complex64")
         self.data = data
         self.space = space
 
@@ -96,14 +121,18 @@ class TensorMap:
 
     @staticmethod
     def from_numpy(arr, space, dtype=torch.float64):
-        arr = np.ascontiguousarray(arr, dtype=np.float64 if dtype == torch.float64 else np.float32)
+        arr = np.ascontiguousarray(arr, dtype=_NP[dtype])
         return TensorMap(torch.from_numpy(arr).to(_device()), space)
 
     @staticmethod
     def randn(space, rng: np.random.Generator, dtype=torch.float64):
         """Seeded standard-normal data drawn on the host (the seed stream is
-        this package's, not Julia's Xoshiro)."""
-        return TensorMap.from_numpy(rng.standard_normal(space.dim()), space, dtype)
+        this package's, not Julia's Xoshiro); complex types draw the real parts, then
+        the imaginary parts."""
+        x = rng.standard_normal(space.dim())
+        if dtype in _COMPLEX:
+            x = x + 1j * rng.standard_normal(space.dim())
+        return TensorMap.from_numpy(x, space, dtype)
 
     def numpy(self):
         return self.data.detach().cpu().numpy()
@@ -194,11 +223,9 @@ def _exec_flat(dst, src, alpha, beta):
     if n == 0:
         return
     plan = flat_plan(n)
-    lib = _lib.load()
     if _TRACE is not None:
-        _TRACE.append(("transform", plan, (0,), _buf(src), _buf(dst), float(alpha), float(beta)))
-    _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[dst.dtype], src.data.data_ptr(), dst.data.data_ptr(),
-                                       float(alpha), float(beta), _stream()), "add!")
+        _TRACE.append(("transform", plan, (0,), _buf(src), _buf(dst), _scalar(alpha), _scalar(beta)))
+    _texec(plan, dst.dtype, src.data.data_ptr(), dst.data.data_ptr(), alpha, beta, what="add!")
 
 
 def add_(tdst, tsrc, alpha=1.0, beta=1.0):
@@ -240,12 +267,11 @@ def add_transform_(tdst, tsrc, p1, p2, alpha, beta, levels=None, dst_twist=()):
         return add_(tdst, tsrc, alpha, beta)
     plan = treebraider(tdst.space, tsrc.space, tuple(p1), tuple(p2), levels, adjoint_src=is_adj,
                        dst_twist=tuple(dst_twist))
-    lib = _lib.load()
     if _TRACE is not None:
-        _TRACE.append(("transform", plan, tuple(p1) + tuple(p2), _buf(tsrc), _buf(tdst), float(alpha),
-                       float(beta)))
-    _lib.check(lib.tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(), tdst.data.data_ptr(),
-                                       float(alpha), float(beta), _stream()), "add_transform!")
+        _TRACE.append(("transform", plan, tuple(p1) + tuple(p2), _buf(tsrc), _buf(tdst), _scalar(alpha),
+                       _scalar(beta)))
+    _texec(plan, tdst.dtype, tsrc.data.data_ptr(), tdst.data.data_ptr(), alpha, beta, conj=is_adj,
+           what="add_transform!")
     return tdst
 
 
@@ -383,10 +409,9 @@ def transpose_(tdst, tsrc, p1, p2, alpha=1.0, beta=0.0):
         return add_(tdst, tsrc, alpha, beta)
     plan = TT.treetransposer(tdst.space, tsrc.space, p1, p2, adjoint_src=is_adj)
     if _TRACE is not None:
-        _TRACE.append(("transform", plan, p1 + p2, _buf(tsrc), _buf(tdst), float(alpha), float(beta)))
-    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(),
-                                               tdst.data.data_ptr(), float(alpha), float(beta), _stream()),
-               "transpose!")
+        _TRACE.append(("transform", plan, p1 + p2, _buf(tsrc), _buf(tdst), _scalar(alpha), _scalar(beta)))
+    _texec(plan, tdst.dtype, tsrc.data.data_ptr(), tdst.data.data_ptr(), alpha, beta, conj=is_adj,
+           what="transpose!")
     return tdst
 
 
@@ -521,8 +546,7 @@ def flip(t, inds, inv=False):
     W2 = t.space.flip(inds)
     plan = TT.flipper(W2, t.space, inds, inv)
     out = TensorMap.empty(W2, t.dtype)
-    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[t.dtype], t.data.data_ptr(), out.data.data_ptr(),
-                                               1.0, 0.0, _stream()), "flip")
+    _texec(plan, t.dtype, t.data.data_ptr(), out.data.data_ptr(), 1.0, 0.0, what="flip")
     return out
 
 
@@ -552,9 +576,7 @@ def trace_permute_(tdst, tsrc, p, q, alpha=1.0, beta=0.0):
     if isinstance(tsrc, AdjointTensorMap):
         tsrc = permute(tsrc, tsrc.codomainind(), tsrc.domainind(), copy=True)
     plan = tracer(tdst.space, W, p1, p2, q1, q2)
-    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[tdst.dtype], tsrc.data.data_ptr(),
-                                               tdst.data.data_ptr(), float(alpha), float(beta), _stream()),
-               "trace_permute!")
+    _texec(plan, tdst.dtype, tsrc.data.data_ptr(), tdst.data.data_ptr(), alpha, beta, what="trace_permute!")
     return tdst
 
 
@@ -598,8 +620,7 @@ def planartrace_(C, A, p, q, alpha=1.0, beta=0.0):
     if isinstance(A, AdjointTensorMap):
         A = transpose(A, A.codomainind(), A.domainind())
     plan = tracer(C.space, W, p1, p2, q1, q2, planar=True)
-    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[C.dtype], A.data.data_ptr(), C.data.data_ptr(),
-                                               float(alpha), float(beta), _stream()), "planartrace!")
+    _texec(plan, C.dtype, A.data.data_ptr(), C.data.data_ptr(), alpha, beta, what="planartrace!")
     return C
 
 
@@ -624,9 +645,8 @@ def tr(t):
         plan = _make_trace_plan(rows, 0, 1, W.dim())
         _TRANSFORMER_CACHE.put(key, plan)
     out = torch.empty(1, dtype=t.dtype, device=_device())
-    _lib.check(_lib.load().tkmi_transform_exec(plan.handle, _DT[t.dtype], t.data.data_ptr(), out.data_ptr(),
-                                               1.0, 0.0, _stream()), "tr")
-    return float(out.item())
+    _texec(plan, t.dtype, t.data.data_ptr(), out.data_ptr(), 1.0, 0.0, what="tr")
+    return complex(out.item()) if t.dtype in _COMPLEX else float(out.item())
 
 
 # ---------------------------------------------------------------------------
@@ -725,6 +745,12 @@ def gemm_precision(dtype, precision=None):
         if prec == _lib.PREC_F64:
             raise TypeError("mul!: float32 tensors multiply in FP32 or 3×BF16, not in FP64")
         return prec
+    if dtype in _COMPLEX:
+        # zgemm / cgemm: ComplexF64 in FP64, ComplexF32 in FP32 unless told otherwise
+        prec = (_lib.PREC_F64 if dtype == torch.complex128 else _lib.PREC_F32) if precision is None else precision
+        if prec == _lib.PREC_BF16X3:
+            raise TypeError("mul!: complex tensors multiply in FP64 or FP32")
+        return prec
     return DEFAULT_PRECISION if precision is None else precision
 
 
@@ -737,6 +763,16 @@ def mul_(tC, tA, tB, alpha=1.0, beta=0.0, precision=None):
     plan = gemm_plan(tC, tA, tB)
     prec = gemm_precision(tC.dtype, precision)
     lib = _lib.load()
+    if tC.dtype in _COMPLEX:
+        # an adjoint operand's blocks are conjugate-transposed
+        a, b = complex(alpha), complex(beta)
+        if _TRACE is not None:
+            _TRACE.append(("gemm", plan, int(prec), _buf(tA), _buf(tB), _buf(tC), _scalar(alpha), _scalar(beta)))
+        _lib.check(lib.tkmi_gemm_exec_z(plan.handle, int(prec), _DT[tC.dtype], int(isinstance(tA, AdjointTensorMap)),
+                                        int(isinstance(tB, AdjointTensorMap)), a.real, a.imag, b.real, b.imag,
+                                        tA.data.data_ptr(), tB.data.data_ptr(), tC.data.data_ptr(), _stream()),
+                   "mul!")
+        return tC
     if _TRACE is not None:
         _TRACE.append(("gemm", plan, int(prec), _buf(tA), _buf(tB), _buf(tC), float(alpha), float(beta)))
     _lib.check(lib.tkmi_gemm_exec_dtype(plan.handle, int(prec), _DT[tC.dtype], 3, float(alpha), float(beta),
