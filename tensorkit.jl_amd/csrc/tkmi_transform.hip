@@ -12,7 +12,13 @@
 //     dW != dR  -> "transpose" mode: a (tW × tR) tile (powers of two) is read with
 //                  lanes along dR, staged in LDS, and written with lanes along dW;
 //                  when tW·tR leaves the tile's area unused (small dims), the tile
-//                  takes tB slices along a third dim dB as well.
+//                  takes tB slices along a third dim dB as well;
+//     dW == dR with a short row (≤ 24 elements) and different next dims on the two
+//                  sides -> "vector transpose": the short dim dV is carried whole and
+//                  the tile is a tW × tR transpose of those vectors (tR along the read
+//                  side's next dim, tW along the write side's), staged in LDS; both
+//                  sides then move runs of |dV|·tR or |dV|·tW elements instead of rows
+//                  of |dV|.
 // Work items are split into classes by the number of trees they touch (L = 1, 2, 4,
 // 8); each class is one launch of transform_kernel<T, L>.  Inside a launch the tiles
 // of all items are numbered consecutively (tile_start prefix sums) and each
@@ -45,19 +51,21 @@ namespace {
 constexpr int MAXD = 12;
 constexpr int NTHREADS = 256;
 constexpr int NCLASS = 4;  // L = 1, 2, 4, 8
+constexpr int NSIDE = 2 * NCLASS;  // per class: the other tiles, then the vector-transpose tiles
 constexpr int MAXW = 8;    // destination trees per work item
 constexpr int64_t SMALL_PLAN = (int64_t)1 << 25;  // written elements below which classes merge
 constexpr int64_t KEEP_BYTES = (int64_t)128 << 20;  // written bytes up to which stores stay cached
 
 struct TBlock {
     int32_t ndim;
-    int32_t mode;           // 0 copy, 1 transpose
+    int32_t mode;           // 0 copy, 1 transpose, 2 transpose of short vectors along dV
     int32_t nread, nwrite;  // nwrite ≤ L; nread may exceed L (chunked in the kernel)
     int32_t dW, dR;
     int32_t tW, tR;  // tile extents (transpose mode: powers of two)
     int32_t lgW, lgR;
     int32_t nother;
     int32_t u_ld;  // leading dimension of the (column-major) coefficient matrix
+    int32_t dV;    // mode 2: the short dim, innermost on both sides, carried whole in a tile
     double inv_w;  // copy mode: 1 / size[dW], for the row of a tile's first element
     int32_t odims[MAXD];
     int64_t size[MAXD];
@@ -148,7 +156,7 @@ struct Scal {
 // One tile of one work item.  COPY: the tile is nr runs of nw elements along dW
 // (contiguous on both sides), numbered run after run; otherwise a tW × tR tile
 // read with lanes along dR and written with lanes along dW through LDS.
-template <typename T, int L, bool COPY, bool SLICED>
+template <typename T, int L, bool COPY, bool SLICED, bool VEC = false>
 __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__restrict__ trees,
                                                const double *__restrict__ Umat, const T *__restrict__ src,
                                                T *__restrict__ dst, const Scal &sc, bool keep,
@@ -164,8 +172,9 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     t /= B.ntW;
     const int64_t iR = t % B.ntR;
     t /= B.ntR;
-    // slices along dB only in the SLICED instantiation (the others keep their registers)
-    const int dB = SLICED ? dB_in : -1;
+    // slices along dB only in the SLICED instantiation (the others keep their registers);
+    // a vector-transpose tile carries its short dim dV whole, addressed like a slice dim
+    const int dB = SLICED ? dB_in : (VEC ? B.dV : -1);
     const int tB = SLICED ? 1 << lgB : 1;
     int64_t iB = 0;
     if (SLICED && dB >= 0) {
@@ -175,7 +184,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     }
     const int64_t bt = t;
     const int64_t b0 = iB * tB;
-    const int nb = dB >= 0 ? (int)min((int64_t)tB, B.size[dB] - b0) : 1;
+    const int nb = (SLICED && dB >= 0) ? (int)min((int64_t)tB, B.size[dB] - b0) : 1;
     int64_t w0, r0;
     int nw, nr;
     // copy mode: tile iR holds elements [iR·area, iR·area + cnt) of the item's plane
@@ -232,8 +241,27 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     const int ws = long_rows ? 0 : (int)w0;
     const int wshort = long_rows ? 1 : (int)W;
     const float inv = 1.0f / (float)(COPY ? wshort : nw);
+    // vector transpose: element e of the tile is (v, e / V) with v = e mod V; the quotient
+    // from a float reciprocal (e < 2048·V ≪ 2^24), corrected by ±1
+    const int V = VEC ? (int)B.size[B.dV] : 1;
+    const float vinv = VEC ? (float)B.inv_w : 1.0f;
+    auto vsplit = [&](int e, int &v) {
+        int q = (int)((float)e * vinv);
+        const int rem = e - q * V;
+        if (rem < 0) --q;
+        else if (rem >= V) ++q;
+        v = e - q * V;
+        return q;
+    };
     auto pos_read = [&](int k, int &w, int &r, int &b) {
         b = 0;
+        if (VEC) {
+            // lanes along (dV, dR): runs of V·tR elements on the read side
+            const int q = vsplit(tid + k * NTHREADS, b);
+            r = q & (tR - 1);
+            w = q >> B.lgR;
+            return w < nw && r < nr;
+        }
         if (COPY) {
             const int e = tid + k * NTHREADS;
             if (long_rows) {
@@ -264,6 +292,13 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
     auto pos_write = [&](int k, int &w, int &r, int &b) {
         if (COPY) return pos_read(k, w, r, b);
         b = 0;
+        if (VEC) {
+            // lanes along (dV, dW): runs of V·tW elements on the write side
+            const int q = vsplit(tid + k * NTHREADS, b);
+            w = q & (tW - 1);
+            r = q >> B.lgW;
+            return w < nw && r < nr;
+        }
         w = tid & (tW - 1);
         const int rv = (tid >> B.lgW) + k * (NTHREADS >> B.lgW);
         if (!SLICED) {
@@ -313,8 +348,14 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
             }
         }
         if (!COPY) {
-            // stage through LDS: read layout (lanes along dR) -> write layout (lanes along dW)
+            // stage through LDS: read layout (lanes along dR) -> write layout (lanes along dW).
+            // Vector tiles: [w][r][v] with a row pitch of V·tR + 1 per w (odd: the write
+            // side's lanes, consecutive in v then w, fall in distinct banks)
             const int pitch = tR + 1;
+            const int vpitch = V * tR + 1;
+            auto lidx = [&](int i, int w, int r, int b) {
+                return VEC ? (i * tW + w) * vpitch + r * V + b : ((i * tB + b) * tW + w) * pitch + r;
+            };
             if (ic > 0) __syncthreads();
 #pragma unroll
             for (int i = 0; i < L; ++i)
@@ -322,7 +363,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 #pragma unroll
                     for (int k = 0; k < E; ++k) {
                         int w, r, b;
-                        if (pos_read(k, w, r, b)) lds[((i * tB + b) * tW + w) * pitch + r] = x[i][k];
+                        if (pos_read(k, w, r, b)) lds[lidx(i, w, r, b)] = x[i][k];
                     }
             __syncthreads();
 #pragma unroll
@@ -331,7 +372,7 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 #pragma unroll
                     for (int k = 0; k < E; ++k) {
                         int w, r, b;
-                        x[i][k] = pos_write(k, w, r, b) ? lds[((i * tB + b) * tW + w) * pitch + r] : (T)0;
+                        x[i][k] = pos_write(k, w, r, b) ? lds[lidx(i, w, r, b)] : (T)0;
                     }
         }
 #pragma unroll
@@ -457,6 +498,27 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     }
 }
 
+// Vector-transpose tiles (mode 2) only: a launch of its own, with the same LDS staging bound.
+template <typename T, int L>
+__global__ __launch_bounds__(NTHREADS) void transform_vec_kernel(
+    const TBlock *__restrict__ blocks, int64_t nblocks, const TTree *__restrict__ trees,
+    const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, Scal sc, int keep,
+    int64_t tile0) {
+    const int64_t gt = tile0 + (int64_t)blockIdx.x;
+    int64_t lo = 0, hi = nblocks - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (blocks[mid].tile_start <= gt)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    constexpr int LDSN = L * (class_area(L) + class_twmax(L));
+    __shared__ T lds[LDSN];
+    transform_tile<T, L, false, false, true>(blocks[lo], trees, Umat, src, dst, sc, keep, gt - blocks[lo].tile_start,
+                                             lds, -1, 0);
+}
+
 // U[e] = coef[e]·g[idx[e]] (idx < 0: 0): the coefficients of a transform that depend on
 // a small device tensor (a contraction with a small operand folded into the transform).
 // A launch of its own ahead of the transform: forming them inside every tile puts two
@@ -519,7 +581,7 @@ struct tkmi_transform_plan_s {
     bool inplace = false;
     std::vector<TTree> trees;
     std::vector<double> U, Ut;
-    PlanSide fwd[NCLASS], adj[NCLASS];
+    PlanSide fwd[NSIDE], adj[NSIDE];
     TTree *d_trees = nullptr;
     double *d_U = nullptr, *d_Ut = nullptr;
 };
@@ -552,6 +614,27 @@ void choose_tile(int64_t sW, int64_t sR, int area, int twmax, int &tW, int &tR) 
     }
 }
 
+// Vector-transpose tile extents (mode 2): powers of two, V·tW·tR ≤ area and tW ≤ twmax
+// (the LDS staging bound), by the cost model of choose_tile.
+void choose_vec_tile(int64_t V, int64_t sW, int64_t sR, int area, int twmax, int &tW, int &tR) {
+    const int64_t pW = pow2ceil(sW), pR = pow2ceil(sR);
+    double best = 1e300;
+    tW = tR = 1;
+    for (int64_t r = 1; r <= NTHREADS && r <= pR; r <<= 1)
+        for (int64_t w = 1; w <= twmax && w <= pW && V * w * r <= area; w <<= 1) {
+            const double tiles = (double)((sW + w - 1) / w) * (double)((sR + r - 1) / r);
+            const double cost = tiles * (256.0 + 0.25 * (double)(V * w * r));
+            if (cost < best || (cost == best && std::min(w, r) > std::min<int64_t>(tW, tR))) {
+                best = cost;
+                tW = (int)w;
+                tR = (int)r;
+            }
+        }
+}
+
+// rows up to this many elements take the vector transpose when the two sides' next dims differ
+constexpr int64_t VEC_MAX = 24;
+
 // Tiles of a block: ntW · ntR · (slices along dB) · nbatch.
 int64_t block_tiles(const TBlock &b, int dB, int lgB) {
     const int64_t ntB = dB >= 0 ? (b.size[dB] + ((int64_t)1 << lgB) - 1) >> lgB : 1;
@@ -572,7 +655,27 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L, bool allow_sl
         if (sw < bestW) { bestW = sw; dW = d; }
         if (sr < bestR) { bestR = sr; dR = d; }
     }
+    // the next dims after the shared innermost one, on the read and on the write side
+    int aR = -1, aW = -1;
     if (dW == dR) {
+        int64_t br = INT64_MAX, bw = INT64_MAX;
+        for (int d = 0; d < b.ndim; ++d) {
+            if (d == dW || b.size[d] <= 1) continue;
+            const int64_t sr = std::llabs(tr.str[d]), sw = std::llabs(tw.str[d]);
+            if (sr < br) { br = sr; aR = d; }
+            if (sw < bw) { bw = sw; aW = d; }
+        }
+    }
+    if (dW == dR && b.size[dW] <= VEC_MAX && aR >= 0 && aW >= 0 && aR != aW) {
+        b.mode = 2;
+        b.dV = dW;
+        b.inv_w = 1.0 / (double)b.size[dW];
+        dW = aW;
+        dR = aR;
+        choose_vec_tile(b.size[b.dV], b.size[dW], b.size[dR], area, twmax, b.tW, b.tR);
+        b.lgW = log2i(b.tW);
+        b.lgR = log2i(b.tR);
+    } else if (dW == dR) {
         b.mode = 0;
         // second tiled dim: next smallest write stride among the others (or a size-1 dim)
         int second = -1;
@@ -631,7 +734,7 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L, bool allow_sl
     b.nother = 0;
     b.nbatch = 1;
     for (int d = 0; d < b.ndim; ++d) {
-        if (d == dW || d == dR || d == dB) continue;
+        if (d == dW || d == dR || d == dB || (b.mode == 2 && d == b.dV)) continue;
         b.odims[b.nother++] = d;
         b.nbatch *= b.size[d];
     }
@@ -676,7 +779,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
     // into chunks of at most MAXW per work item
     std::vector<TTree> &trees = p->trees;
     auto build = [&](bool adjoint, PlanSide *sides) {
-        for (int c = 0; c < NCLASS; ++c) {
+        for (int c = 0; c < NSIDE; ++c) {
             sides[c].blocks.clear();
             sides[c].ntiles = 0;
             sides[c].staged = false;
@@ -699,7 +802,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
             for (int jc = 0; jc < R; jc += MAXW) cmax = std::max(cmax, class_of(Cn, std::min(MAXW, R - jc)));
         }
         const bool merge = moved <= SMALL_PLAN && class_L(cmax) <= 4;
-        for (int c = 0; c < NCLASS; ++c) sides[c].written = moved;
+        for (int c = 0; c < NSIDE; ++c) sides[c].written = moved;
         // Slicing is decided per class: its launch then uses the kernel instantiation with
         // slices, which holds more registers (the L = 1 kernel drops from 7 to 6 waves per
         // SIMD), so a class takes it only when slices cut its tile count by at least 60%
@@ -752,13 +855,16 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
                 const int c = merge ? cmax : class_of(b.nread, b.nwrite);
                 int sdb, slg;
                 tile_block(b, trees, class_L(c), slice_class[c], sdb, slg);
-                b.tile_start = sides[c].ntiles;
-                sides[c].ntiles += block_tiles(b, sdb, slg);
-                sides[c].slices.push_back(sdb >= 0 ? (int32_t)(sdb | (slg << 8)) : -1);
-                sides[c].staged |= (b.mode != 0);
-                sides[c].reads |= (b.nread > 0);
-                sides[c].sliced |= (sdb >= 0);
-                sides[c].blocks.push_back(b);
+                // vector-transpose tiles run in a launch of their own kernel, so that the
+                // other kernels keep their register counts
+                PlanSide &sd = sides[c + (b.mode == 2 ? NCLASS : 0)];
+                b.tile_start = sd.ntiles;
+                sd.ntiles += block_tiles(b, sdb, slg);
+                sd.slices.push_back(sdb >= 0 ? (int32_t)(sdb | (slg << 8)) : -1);
+                sd.staged |= (b.mode != 0);
+                sd.reads |= (b.nread > 0);
+                sd.sliced |= (sdb >= 0);
+                sd.blocks.push_back(b);
             }
         }
     };
@@ -770,7 +876,7 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
     if ((rc = tkmi::upload((void **)&p->d_trees, trees.data(), trees.size() * sizeof(TTree)))) return rc;
     if ((rc = tkmi::upload((void **)&p->d_U, p->U.data(), p->U.size() * sizeof(double)))) return rc;
     if ((rc = tkmi::upload((void **)&p->d_Ut, p->Ut.data(), p->Ut.size() * sizeof(double)))) return rc;
-    for (int c = 0; c < NCLASS; ++c) {
+    for (int c = 0; c < NSIDE; ++c) {
         for (PlanSide *side : {&p->fwd[c], &p->adj[c]}) {
             if (side->blocks.empty()) continue;
             if ((rc = tkmi::upload((void **)&side->d_blocks, side->blocks.data(),
@@ -791,7 +897,7 @@ void free_plan(tkmi_transform_plan_s *p) {
     (void)hipFree(p->d_trees);
     (void)hipFree(p->d_U);
     (void)hipFree(p->d_Ut);
-    for (int c = 0; c < NCLASS; ++c) {
+    for (int c = 0; c < NSIDE; ++c) {
         (void)hipFree(p->fwd[c].d_blocks);
         (void)hipFree(p->adj[c].d_blocks);
         (void)hipFree(p->fwd[c].d_slices);
@@ -865,6 +971,18 @@ void launch_class(const tkmi_transform_plan_s *p, const PlanSide &side, const do
     }
 }
 
+template <typename T, int L>
+void launch_vec(const tkmi_transform_plan_s *p, const PlanSide &side, const double *dU, const T *src, T *dst,
+                const Scal &sc, hipStream_t stream) {
+    const int keep = side.written * (int64_t)sizeof(T) <= KEEP_BYTES;
+    const int64_t chunk = ((int64_t)1 << 31) / NTHREADS;
+    for (int64_t t0 = 0; t0 < side.ntiles; t0 += chunk) {
+        const unsigned grid = (unsigned)std::min(chunk, side.ntiles - t0);
+        hipLaunchKernelGGL((transform_vec_kernel<T, L>), dim3(grid), dim3(NTHREADS), 0, stream, side.d_blocks,
+                           (int64_t)side.blocks.size(), p->d_trees, dU, src, dst, sc, keep, t0);
+    }
+}
+
 template <typename T>
 int launch_side(const tkmi_transform_plan_s *p, const PlanSide *sides, const double *dU, const T *src, T *dst,
                 const Scal &sc, hipStream_t stream) {
@@ -872,13 +990,17 @@ int launch_side(const tkmi_transform_plan_s *p, const PlanSide *sides, const dou
     if (sides[1].ntiles) launch_class<T, 2>(p, sides[1], dU, src, dst, sc, stream);
     if (sides[2].ntiles) launch_class<T, 4>(p, sides[2], dU, src, dst, sc, stream);
     if (sides[3].ntiles) launch_class<T, 8>(p, sides[3], dU, src, dst, sc, stream);
+    if (sides[NCLASS + 0].ntiles) launch_vec<T, 1>(p, sides[NCLASS + 0], dU, src, dst, sc, stream);
+    if (sides[NCLASS + 1].ntiles) launch_vec<T, 2>(p, sides[NCLASS + 1], dU, src, dst, sc, stream);
+    if (sides[NCLASS + 2].ntiles) launch_vec<T, 4>(p, sides[NCLASS + 2], dU, src, dst, sc, stream);
+    if (sides[NCLASS + 3].ntiles) launch_vec<T, 8>(p, sides[NCLASS + 3], dU, src, dst, sc, stream);
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
 
 int64_t total_tiles(const PlanSide *sides) {
     int64_t n = 0;
-    for (int c = 0; c < NCLASS; ++c) n += sides[c].ntiles;
+    for (int c = 0; c < NSIDE; ++c) n += sides[c].ntiles;
     return n;
 }
 
@@ -902,7 +1024,7 @@ int exec_common(tkmi_transform_plan p, int32_t dtype, const void *src, void *dst
     // a source may be empty (null) when no block reads it: e.g. a contraction with an
     // empty operand into a non-empty C only scales C by β
     bool reads = false;
-    for (int c = 0; c < NCLASS; ++c) reads |= sides[c].ntiles > 0 && sides[c].reads;
+    for (int c = 0; c < NSIDE; ++c) reads |= sides[c].ntiles > 0 && sides[c].reads;
     (void)alpha;
     if ((total_tiles(sides) > 0 && !dst) || (reads && has_alpha && !src))
         TKMI_FAIL(TKMI_ERR_ARGUMENT, "transform: null data pointer");

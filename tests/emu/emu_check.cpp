@@ -301,11 +301,21 @@ static void run_transform(int N, const std::vector<int32_t> &perm, std::vector<T
 }
 
 static void transform_cases() {
-    for (int rep = 0; rep < 48; ++rep) {
-        const int N = irand(1, 4);
+    // fixed cases first: short innermost rows kept innermost while the next dims swap
+    // (vector-transpose tiles), then random ranks, sizes and permutations
+    const std::vector<std::pair<std::vector<int64_t>, std::vector<int32_t>>> fixed = {
+        {{8, 5, 7}, {0, 2, 1}}, {{12, 9, 10, 3}, {0, 2, 1, 3}}, {{24, 17, 3, 20}, {0, 3, 2, 1}},
+        {{3, 40, 33}, {0, 2, 1}}};
+    for (int rep = 0; rep < 48 + 2 * (int)fixed.size(); ++rep) {
+        const int fx = rep / 2;
+        const bool is_fixed = fx < (int)fixed.size();
+        const int N = is_fixed ? (int)fixed[fx].first.size() : irand(1, 4);
         std::vector<int32_t> perm(N);
         for (int d = 0; d < N; ++d) perm[d] = d;
-        std::shuffle(perm.begin(), perm.end(), rng);
+        if (is_fixed)
+            perm = fixed[fx].second;
+        else
+            std::shuffle(perm.begin(), perm.end(), rng);
         const bool abelian = rep % 2 == 0;
         std::vector<TBlockSpec> blocks;
         int64_t dst_len = 0, src_len = 0;
@@ -314,7 +324,7 @@ static void transform_cases() {
             TBlockSpec B;
             B.R = abelian ? 1 : irand(1, 9);
             B.C = abelian ? 1 : (irand(0, 4) == 0 ? irand(9, 12) : irand(1, 4));
-            B.sz_src = rand_sizes(N);
+            B.sz_src = is_fixed ? fixed[fx].first : rand_sizes(N);
             if (!abelian) {
                 int64_t tot = 1;
                 for (auto x : B.sz_src) tot *= x;

@@ -228,6 +228,9 @@ def test_su2_permute_vs_c_loop_cfg3_scaled(T, div):
             nt = ctypes.c_int64()
             _lib.check(_lib.load().tkmi_transform_plan_info(plan.handle, ctypes.byref(nt), None, None), "info")
             assert nt.value <= sliced_bound, (div, p1, p2, nt.value)
+        if True:
+            # (the first permutation keeps its short innermost leg innermost: vector-transpose
+            # tiles, forward and adjoint)
             # the adjoint exec of the same plan: ⟨d, permute(A)⟩ = ⟨pullback(d), A⟩
             yb, pb = CR.rrule_permute(A, p1, p2)
             d = T.TensorMap.randn(yb.space, rng)
