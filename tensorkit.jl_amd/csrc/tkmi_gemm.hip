@@ -1301,14 +1301,20 @@ struct SkinnySet {
 // 3×BF16 kernel always uses BK = 32).
 struct Cfg {
     int bm, bn, bk, nbuf;  // FP64/FP32 kernel: 2 gemm_k4 (two LDS buffers), 1 gemm_k4s, 3 gemm_k4p, 5 gemm_pks
+    int split;             // plan-wide split-K factor: 0 = the schedule model's choice
 };
-// nbuf 5: the persistent kernel with a host-made schedule (gemm_pks_kernel)
-constexpr Cfg CFGS[] = {{64, 64, 16, 2},  {128, 128, 16, 2}, {128, 64, 16, 2}, {64, 128, 16, 2},
-                        {64, 64, 32, 2},  {64, 64, 16, 1},   {64, 64, 32, 1},  {128, 128, 16, 1},
-                        {64, 64, 16, 3},  {64, 64, 16, 5}};
+// nbuf 5: the persistent kernel with a host-made schedule (gemm_pks_kernel).  Configurations
+// 10–12 are configuration 5 with the split-K factor fixed at 1, 2 and 3: the autotuner
+// measures them against the model's choice (the model prices a split group's partial
+// products and reduction only roughly)
+constexpr Cfg CFGS[] = {{64, 64, 16, 2, 0},  {128, 128, 16, 2, 0}, {128, 64, 16, 2, 0}, {64, 128, 16, 2, 0},
+                        {64, 64, 32, 2, 0},  {64, 64, 16, 1, 0},   {64, 64, 32, 1, 0},  {128, 128, 16, 1, 0},
+                        {64, 64, 16, 3, 0},  {64, 64, 16, 5, 0},   {64, 64, 16, 1, 1},  {64, 64, 16, 1, 2},
+                        {64, 64, 16, 1, 3}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
-// of a configuration matter to it, and the prefetch depth of configuration 8
-constexpr bool bf16_duplicate(int c) { return c >= 4 && c != 8; }  // 8: two prefetch stages
+// of a configuration matter to it, the prefetch depth of configuration 8 and the fixed
+// split factors of configurations 10–12
+constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9; }
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 }  // namespace
@@ -1392,7 +1398,8 @@ int64_t cfg_slots(int c) {
             case 6: occ = kernel_occupancy<64, 64, 32, 1>(); break;
             case 7: occ = kernel_occupancy<128, 128, 16, 1>(); break;
             case 8: occ = kernel_occupancy<64, 64, 16, 3>(); break;
-            default: occ = kernel_occupancy<64, 64, 16, 5>(); break;
+            case 9: occ = kernel_occupancy<64, 64, 16, 5>(); break;
+            default: occ = kernel_occupancy<64, 64, 16, 1>(); break;
         }
         v = occ * cus;
     }
@@ -1453,12 +1460,16 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     // the smaller factor)
     const int64_t slots = cfg_slots(c);
     int64_t S = 1;
-    double best = split_model(p->base, 1, bm, bn, bk, slots);
-    for (int64_t cand = 2; cand <= MAX_SPLIT; ++cand) {
-        const double t = split_model(p->base, cand, bm, bn, bk, slots);
-        if (t < best * 0.98) {
-            best = t;
-            S = cand;
+    if (CFGS[c].split > 0) {
+        S = CFGS[c].split;
+    } else {
+        double best = split_model(p->base, 1, bm, bn, bk, slots);
+        for (int64_t cand = 2; cand <= MAX_SPLIT; ++cand) {
+            const double t = split_model(p->base, cand, bm, bn, bk, slots);
+            if (t < best * 0.98) {
+                best = t;
+                S = cand;
+            }
         }
     }
     ts.split = S;
@@ -1695,7 +1706,8 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const TD *A, const TD *B, T
         case 6: launch_k<TD, 64, 64, 32, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         case 7: launch_k<TD, 128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         case 8: launch_k<TD, 64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
-        default: launch_k<TD, 64, 64, 16, 5>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 9: launch_k<TD, 64, 64, 16, 5>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        default: launch_k<TD, 64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
     }
     TKMI_HIP(hipGetLastError());
     return TKMI_OK;

@@ -420,7 +420,7 @@ static void gemm_cases() {
         };
         // real data: every configuration in FP64 (each kernel template once per transpose
         // case), the FP32 and 3×BF16 forms on the configurations with distinct kernels
-        for (int cfg = 0; cfg < 10; ++cfg) {
+        for (int cfg = 0; cfg < 13; ++cfg) {
             for (int prec = 0; prec < 3; ++prec) {
                 const bool run = variant == 0 || cfg == 5 || cfg == 8 || cfg == 9 || (cfg == 1 && prec == 2);
                 if (!run) continue;

@@ -85,7 +85,7 @@ SHAPES = [(1, 1, 1), (16, 16, 4), (29, 24, 5), (64, 64, 16), (65, 63, 17), (130,
           (5, 300, 7), (300, 4, 4), (17, 33, 0), (0, 5, 3)]
 
 
-@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009])
+@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_f64_exact_integer(lib, ta, tb, tile):
     rng = np.random.default_rng(10 * ta + tb)
@@ -123,7 +123,7 @@ def test_reduced_precision_forms(lib, prec, tol):
         assert err <= tol, (prec, tile, err)
 
 
-@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009])
+@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012])
 def test_split_k_exact(lib, tile):
     """Few output tiles and long K engage split-K (deterministic workspace
     reduction); integer data keeps the check exact."""
@@ -314,7 +314,7 @@ def _f32(*xs):
     return [x.astype(np.float32) for x in xs]
 
 
-@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009])
+@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_f32_data_exact_integer(lib, ta, tb, tile):
     rng = np.random.default_rng(50 + 10 * ta + tb)
