@@ -81,10 +81,18 @@ def key_to_json(key):
         return {"kind": kind, "dst": homspace_to_json(Vd), "src": homspace_to_json(Vs), "p1": list(p1),
                 "p2": list(p2), "levels": None if levels is None else [list(levels[0]), list(levels[1])],
                 "adjoint_src": bool(adj), "dst_twist": list(twist)}
-    if kind in ("trace", "trace_adj"):
-        _, Vd, Vs, p1, p2, q1, q2 = key
+    if kind == "transpose":
+        _, Vd, Vs, p1, p2, adj = key
         return {"kind": kind, "dst": homspace_to_json(Vd), "src": homspace_to_json(Vs), "p1": list(p1),
-                "p2": list(p2), "q1": list(q1), "q2": list(q2)}
+                "p2": list(p2), "adjoint_src": bool(adj)}
+    if kind in ("trace", "trace_adj"):
+        # trace keys carry a trailing planar flag (planartrace!); adjoint trace keys do not
+        _, Vd, Vs, p1, p2, q1, q2 = key[:7]
+        d = {"kind": kind, "dst": homspace_to_json(Vd), "src": homspace_to_json(Vs), "p1": list(p1),
+             "p2": list(p2), "q1": list(q1), "q2": list(q2)}
+        if kind == "trace":
+            d["planar"] = bool(key[7])
+        return d
     if kind in ("tr", "tr_adj"):
         return {"kind": kind, "space": homspace_to_json(key[1])}
     if kind == "flip":
@@ -102,9 +110,13 @@ def key_from_json(d):
         lv = None if d["levels"] is None else (tuple(d["levels"][0]), tuple(d["levels"][1]))
         return ("braid", homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
                 tuple(d["p2"]), lv, bool(d["adjoint_src"]), tuple(d["dst_twist"]))
+    if kind == "transpose":
+        return ("transpose", homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
+                tuple(d["p2"]), bool(d["adjoint_src"]))
     if kind in ("trace", "trace_adj"):
-        return (kind, homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
-                tuple(d["p2"]), tuple(d["q1"]), tuple(d["q2"]))
+        key = (kind, homspace_from_json(d["dst"]), homspace_from_json(d["src"]), tuple(d["p1"]),
+               tuple(d["p2"]), tuple(d["q1"]), tuple(d["q2"]))
+        return key + (bool(d["planar"]),) if kind == "trace" else key
     if kind in ("tr", "tr_adj"):
         return (kind, homspace_from_json(d["space"]))
     if kind == "flip":

@@ -40,11 +40,17 @@ def test_round_trip(tmp_path):
     # SU(2) partial trace and a scalar (empty) destination
     Wt = Ws.select((1,), (3,))
     trows = TT.trace_table(Wt, Ws, (1,), (3,), (0,), (2,))
-    entries.append(_entry(("trace", Wt, Ws, (1,), (3,), (0,), (2,)), "trace", (0, 1), (Wt.dim(), Ws.dim()),
+    entries.append(_entry(("trace", Wt, Ws, (1,), (3,), (0,), (2,), False), "trace", (0, 1), (Wt.dim(), Ws.dim()),
                           TT.pack_trace(trows)))
     W0 = Ws.select((), ())
-    entries.append(_entry(("trace", W0, Ws, (), (), (0, 1), (2, 3)), "trace", (), (1, Ws.dim()),
+    entries.append(_entry(("trace", W0, Ws, (), (), (0, 1), (2, 3), True), "trace", (), (1, Ws.dim()),
                           TT.pack_trace(TT.trace_table(W0, Ws, (), (), (0, 1), (2, 3)))))
+    # planar transpose of a fermionic space, and an adjoint-trace key (no planar flag)
+    Wfp = Wf.permute((0,), (2, 1))
+    entries.append(_entry(("transpose", Wfp, Wf, (0,), (2, 1), True), "abelian", (0, 2, 1),
+                          (Wfp.dim(), Wf.dim()), TT.pack_abelian(TT.abelian_table(Wfp, Wf, (0,), (2, 1)))))
+    entries.append(_entry(("trace_adj", Ws, Wt, (1,), (3,), (0,), (2,)), "abelian", (0, 1, 2, 3),
+                          (Ws.dim(), Wt.dim()), TT.pack_abelian(tab)))
     entries.append(_entry(("flat", 17), "abelian", (0,), (17, 17), TT.pack_abelian(
         {"coeffs": np.ones(1), "dst": np.array([[17, 1, 0]]), "src": np.array([[17, 1, 0]])})))
     path = tmp_path / "tables.npz"
