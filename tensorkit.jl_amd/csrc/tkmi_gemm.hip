@@ -664,25 +664,54 @@ __global__ __launch_bounds__(NT, 4) void gemm_pks_kernel(
     gemm_pk_body<TC, TD, BM, BN, BK>(groups, ngroups, sched, slot_start, A, Bm, C, ws, alpha, beta);
 }
 
-// Split-K reduction: one thread per output element of the split groups.
+// Split-K reduction (and, with ws = nullptr, C = β·C for α = 0).  Each group's elements
+// start at a multiple of RCH in the element numbering (elem_start), so a workgroup's RCH
+// consecutive elements lie in one group, found by one search per workgroup; each thread
+// sums RPT elements, partials in fixed z order, loads issued before the stores.  Column-
+// major C blocks with ldc = m take the element index as the offset; others divide in 32
+// bits when the block has fewer than 2^31 elements.
+constexpr int RPT = 4;
+constexpr int64_t RCH = (int64_t)RPT * NT;
+inline int64_t rch_round(int64_t e) { return (e + RCH - 1) / RCH * RCH; }
+
 template <typename TD>
 __global__ __launch_bounds__(NT) void splitk_reduce(const RGroup *__restrict__ rg, int64_t ng, int64_t total,
                                                     const double *__restrict__ ws, TD *__restrict__ C,
                                                     double alpha, double beta) {
     const bool has_beta = (beta != 0.0);
-    for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < total; e += (int64_t)gridDim.x * NT) {
+    const int64_t nchunks = total / RCH;
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const int64_t e0 = ch * RCH;
         int64_t lo = 0, hi = ng - 1;
         while (lo < hi) {
             const int64_t mid = (lo + hi + 1) >> 1;
-            if (rg[mid].elem_start <= e) lo = mid; else hi = mid - 1;
+            if (rg[mid].elem_start <= e0) lo = mid; else hi = mid - 1;
         }
         const RGroup &R = rg[lo];
-        const int64_t le = e - R.elem_start;
-        const int64_t mn = R.m * R.n;
-        double acc = 0.0;
-        for (int64_t zz = 0; zz < R.splits; ++zz) acc += ws[R.ws_off + zz * mn + le];
-        const int64_t row = le % R.m, col = le / R.m;
-        put_c(C + R.c_off + row + col * R.ldc, alpha * acc, beta, has_beta);
+        const int64_t mn = R.m * R.n, base = e0 - R.elem_start;
+        double acc[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t le = base + q * NT + threadIdx.x;
+            acc[q] = 0.0;
+            if (ws && le < mn)
+                for (int64_t zz = 0; zz < R.splits; ++zz) acc[q] += ws[R.ws_off + zz * mn + le];
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t le = base + q * NT + threadIdx.x;
+            if (le >= mn) break;
+            int64_t off = le;
+            if (R.ldc != R.m) {
+                if (mn < ((int64_t)1 << 31)) {
+                    const uint32_t l32 = (uint32_t)le, m32 = (uint32_t)R.m;
+                    off = (int64_t)(l32 % m32) + (int64_t)(l32 / m32) * R.ldc;
+                } else {
+                    off = le % R.m + (le / R.m) * R.ldc;
+                }
+            }
+            put_c(C + R.c_off + off, alpha * acc[q], beta, has_beta);
+        }
     }
 }
 
@@ -1488,7 +1517,7 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
             ws_len += sg * g.m * g.n;
             RGroup r{g.m, g.n, g.c_off, g.ldc, g.ws_off, sg, ts.relems};
             ts.rgroups.push_back(r);
-            ts.relems += g.m * g.n;
+            ts.relems += rch_round(g.m * g.n);
         }
         order.push_back(g);
     }
@@ -1673,7 +1702,7 @@ void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const TD *A, con
                                (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
     }
     if (reduce && ts.relems > 0) {
-        const int64_t blocks = std::min<int64_t>((ts.relems + NT - 1) / NT, 8192);
+        const int64_t blocks = std::min<int64_t>(ts.relems / RCH, (int64_t)1 << 20);
         hipLaunchKernelGGL(splitk_reduce<TD>, dim3((unsigned)blocks), dim3(NT), 0, s, ts.d_rgroups,
                            (int64_t)ts.rgroups.size(), ts.relems, ts.ws, C, alpha, beta);
     }
@@ -1784,7 +1813,7 @@ int exec_parts(tkmi_gemm_plan plan, int32_t precision, int32_t parts, double alp
         // BLAS convention: A and B are not read, C = β·C (β = 0 writes zeros)
         for (int q = 0; q < 2; ++q) {
             if (!(parts & (1 << q)) || plan->scale_total[q] == 0) continue;
-            const int64_t blocks = std::min<int64_t>((plan->scale_total[q] + NT - 1) / NT, 8192);
+            const int64_t blocks = std::min<int64_t>(plan->scale_total[q] / RCH, (int64_t)1 << 20);
             hipLaunchKernelGGL(splitk_reduce<TD>, dim3((unsigned)blocks), dim3(NT), 0, s, plan->d_scale[q],
                                (int64_t)plan->scale[q].size(), plan->scale_total[q], (const double *)nullptr, C,
                                0.0, beta);
@@ -2019,7 +2048,7 @@ int tkmi_gemm_plan_create(tkmi_gemm_plan *plan, int64_t ngroups, const int64_t *
             for (const GGroup &G : *list) {
                 if (G.m * G.n == 0) continue;
                 p->scale[q].push_back(RGroup{G.m, G.n, G.c_off, G.ldc, 0, 0, p->scale_total[q]});
-                p->scale_total[q] += G.m * G.n;
+                p->scale_total[q] += rch_round(G.m * G.n);
             }
         }
         if (!rc && !p->scale[q].empty())
