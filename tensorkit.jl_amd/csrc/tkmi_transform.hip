@@ -75,6 +75,8 @@ struct TBlock {
     int64_t u_start;
 };
 
+static_assert(sizeof(TBlock) % sizeof(int64_t) == 0, "the compact tile_start array follows the blocks");
+
 struct TTree {
     int64_t off;
     int64_t str[MAXD];
@@ -469,10 +471,11 @@ __global__ __launch_bounds__(NTHREADS) void transform_kernel(
     int64_t tile0, const int32_t *__restrict__ slices) {
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
     // binary search: last item with tile_start <= gt
+    const int64_t *__restrict__ tstart = reinterpret_cast<const int64_t *>(blocks + nblocks);
     int64_t lo = 0, hi = nblocks - 1;
     while (lo < hi) {
         int64_t mid = (lo + hi + 1) >> 1;
-        if (blocks[mid].tile_start <= gt)
+        if (tstart[mid] <= gt)
             lo = mid;
         else
             hi = mid - 1;
@@ -505,10 +508,11 @@ __global__ __launch_bounds__(NTHREADS) void transform_vec_kernel(
     const double *__restrict__ Umat, const T *__restrict__ src, T *__restrict__ dst, Scal sc, int keep,
     int64_t tile0) {
     const int64_t gt = tile0 + (int64_t)blockIdx.x;
+    const int64_t *__restrict__ tstart = reinterpret_cast<const int64_t *>(blocks + nblocks);
     int64_t lo = 0, hi = nblocks - 1;
     while (lo < hi) {
         int64_t mid = (lo + hi + 1) >> 1;
-        if (blocks[mid].tile_start <= gt)
+        if (tstart[mid] <= gt)
             lo = mid;
         else
             hi = mid - 1;
@@ -879,9 +883,16 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
     for (int c = 0; c < NSIDE; ++c) {
         for (PlanSide *side : {&p->fwd[c], &p->adj[c]}) {
             if (side->blocks.empty()) continue;
-            if ((rc = tkmi::upload((void **)&side->d_blocks, side->blocks.data(),
-                                   side->blocks.size() * sizeof(TBlock))))
-                return rc;
+            // the blocks, then their tile_start values as a compact array: the kernels'
+            // binary search reads 8 bytes per probe from a few cache lines (the scalar
+            // cache keeps them) instead of one probe per ~250-byte block record
+            const size_t nb = side->blocks.size();
+            std::vector<unsigned char> buf(nb * (sizeof(TBlock) + sizeof(int64_t)));
+            std::memcpy(buf.data(), side->blocks.data(), nb * sizeof(TBlock));
+            for (size_t b = 0; b < nb; ++b)
+                std::memcpy(buf.data() + nb * sizeof(TBlock) + b * sizeof(int64_t), &side->blocks[b].tile_start,
+                            sizeof(int64_t));
+            if ((rc = tkmi::upload((void **)&side->d_blocks, buf.data(), buf.size()))) return rc;
             if (side->sliced && (rc = tkmi::upload((void **)&side->d_slices, side->slices.data(),
                                                    side->slices.size() * sizeof(int32_t))))
                 return rc;

@@ -1011,7 +1011,7 @@ def contract_labels(A, la, B, lb, lc, nout, conjA=False, conjB=False, out=None, 
     return tensorcontract_(out, A, pA, conjA, B, pB, conjB, pAB, alpha, beta, precision)
 
 
-def autotune_trace(trace, precisions=(DEFAULT_PRECISION,), reps=5):
+def autotune_trace(trace, precisions=(DEFAULT_PRECISION,), reps=20):
     """Run the GEMM autotuner on every grouped GEMM of a recorded trace, with the
     recorded operands (outputs are overwritten; re-run the computation after)."""
     lib = _lib.load()

@@ -96,6 +96,25 @@ def test_heff_graph_replays(T):
         torch.cuda.synchronize()
 
 
+def test_graph_gathers_read_the_mpo_at_every_replay(T):
+    """The captured chain gathers the MPO coefficients of its two small-operand steps at
+    every replay: changing M1 and M2 in place after the capture changes the replayed result
+    exactly as it changes the eager one (nothing of M is frozen into the graph)."""
+    from tkmi.heff import TwoSiteHeff, two_site_spaces
+    V, P, W = two_site_spaces(D=256, qmax=6)
+    H = TwoSiteHeff(V, P, W, np.random.default_rng(3))
+    H.capture()
+    for scale in (2.0, -0.5):
+        H.M1.data.mul_(scale)
+        H.M2.data.add_(1.0)
+        y = H.replay()
+        torch.cuda.synchronize()
+        got = y.numpy().copy()
+        eager = H.apply().numpy()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got, eager)
+
+
 def test_graph_survives_cache_eviction(T):
     """A captured graph pins the plans it launches: emptying every plan cache (and
     collecting the evicted plans) leaves the replay intact."""
