@@ -457,7 +457,19 @@ def main():
         kept = sum(int(v.numel()) for v in Sk.values())
         ybf = T.permute(T.TensorMap(H32b.graph_y.data.double(), H32b.graph_y.space), (0, 1), (2, 3), copy=True)
         _, Skb, _, epsbf = svd_trunc(ybf, 1024, 1e-10)
+        # rocSOLVER's divide and conquer (MatrixAlgebraKit's default GPU algorithm) on the same
+        # blocks, for comparison: warm call, then timed
+        svd_trunc(y64t, 1024, 1e-10, "divide_and_conquer")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, Skd, _, epsd = svd_trunc(y64t, 1024, 1e-10, "divide_and_conquer")
+        torch.cuda.synchronize()
+        dc_ms = (time.perf_counter() - t0) * 1e3
         svd_report = {"ms": round(svd_ms, 2), "algorithm": DEFAULT_SVD_ALGORITHM,
+                      "rocsolver_gesdd_ms": round(dc_ms, 2),
+                      "rel_eps_diff_vs_gesdd": abs(epsd - eps64) / eps64 if eps64 else None,
+                      "same_kept_set_as_gesdd": {c: int(v.numel()) for c, v in Skd.items()}
+                      == {c: int(v.numel()) for c, v in Sk.items()},
                       "block_shapes": sorted({(d1, d2) for _, (d1, d2), _ in y64t.space.degeneracystructure()[1]},
                                              reverse=True),
                       "kept": kept, "blocks_kept": len(Sk), "eps_fp64": eps64,
@@ -470,7 +482,8 @@ def main():
             for b in blocks_h:
                 np.linalg.svd(b, full_matrices=False)
             svd_report["host_lapack_gesdd_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
-        log(f"svd_trunc: {svd_ms:.1f} ms, kept {kept}, eps64 {eps64:.6e}, eps_bf16 {epsbf:.6e}")
+        log(f"svd_trunc: {svd_ms:.1f} ms ({DEFAULT_SVD_ALGORITHM}; rocSOLVER gesdd {dc_ms:.1f} ms), kept {kept}, "
+            f"eps64 {eps64:.6e}, eps_bf16 {epsbf:.6e}")
 
     # non-abelian permute at the full SU(2) rank-4 size (82.8 GB per tensor): HBM-resident
     su2_report = None

@@ -284,6 +284,11 @@ int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, dou
 #define TKMI_SVD_QRITERATION 0
 #define TKMI_SVD_DIVIDEANDCONQUER 1
 #define TKMI_SVD_JACOBI 2
+/* This library's own block one-sided Jacobi SVD: every block of a call in the same
+ * launches (one workgroup per pair of 16-column blocks per round, sweeps until the columns
+ * are orthogonal to √m·ε), σ accurate to ε·‖A‖ as with the drivers above.  It has no
+ * MatrixAlgebraKit counterpart; SVD plans only (tkmi_eigh_* refuse it). */
+#define TKMI_SVD_BLOCKJACOBI 3
 int tkmi_svd_plan_set_algorithm(tkmi_svd_plan plan, int32_t alg);
 int tkmi_svd_plan_destroy(tkmi_svd_plan plan);
 

@@ -36,4 +36,13 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // Per-device set-up of the reductions' static scratch (called by tkmi_init).
 int reduce_setup();
 
+// Block one-sided Jacobi SVD of a set of column-major FP64 blocks (tkmi_jacobi.hip), with
+// the output packing of tkmi_svd_exec; the workspace is made once per block layout.
+struct JacobiWork;
+int jacobi_create(JacobiWork **out, int64_t nblocks, const int64_t *m, const int64_t *n, const int64_t *a_off,
+                  const int64_t *lda, const int64_t *u_off, const int64_t *s_off, const int64_t *vt_off);
+void jacobi_destroy(JacobiWork *w);
+// `sweeps` (may be null) receives the number of sweeps the slowest block needed
+int jacobi_svd(JacobiWork *w, const double *A, double *U, double *S, double *Vt, hipStream_t s, int *sweeps);
+
 }  // namespace tkmi

@@ -34,14 +34,24 @@ constexpr int ST = 256;
 __global__ __launch_bounds__(ST) void rank_select(const double *__restrict__ S, const double *__restrict__ w,
                                                   int64_t R, double howmany, double atol,
                                                   unsigned char *__restrict__ keep, double *__restrict__ disc) {
+    // the values and weights pass through LDS in tiles of ST, read by every thread
+    __shared__ double ts[ST], tw[ST];
     const int64_t i = (int64_t)blockIdx.x * ST + threadIdx.x;
-    if (i >= R) return;
-    const double si = fabs(S[i]);
-    double before = 0.0;  // weight of values ranked before i
-    for (int64_t j = 0; j < R; ++j) {
-        const double sj = fabs(S[j]);
-        if (sj > si || (sj == si && j < i)) before += w[j];
+    const double si = i < R ? fabs(S[i]) : 0.0;
+    double before = 0.0;  // weight of values ranked before i, added in index order
+    for (int64_t j0 = 0; j0 < R; j0 += ST) {
+        const int64_t j = j0 + threadIdx.x;
+        ts[threadIdx.x] = j < R ? fabs(S[j]) : 0.0;
+        tw[threadIdx.x] = j < R ? w[j] : 0.0;
+        __syncthreads();
+        const int n = (int)min((int64_t)ST, R - j0);
+        for (int q = 0; q < n; ++q) {
+            const double sj = ts[q];
+            if (sj > si || (sj == si && j0 + q < i)) before += tw[q];
+        }
+        __syncthreads();
     }
+    if (i >= R) return;
     const bool k = (before + w[i] <= howmany) && (si >= atol);
     keep[i] = k ? 1 : 0;
     disc[i] = k ? 0.0 : w[i] * S[i] * S[i];
@@ -88,6 +98,7 @@ struct tkmi_svd_plan_s {
     int device = 0;
     hipEvent_t start = nullptr;
     std::vector<SvdLane> lanes;
+    tkmi::JacobiWork *jac = nullptr;  // block Jacobi workspace, made by the first call with that algorithm
 };
 
 namespace {
@@ -107,6 +118,7 @@ void free_svd(tkmi_svd_plan_s *p) {
         (void)hipFree(l.sweeps);
     }
     if (p->start) (void)hipEventDestroy(p->start);
+    tkmi::jacobi_destroy(p->jac);
     delete p;
 }
 
@@ -268,6 +280,16 @@ int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, dou
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: null plan");
     if (plan->lanes.empty()) return TKMI_OK;
     hipStream_t s = tkmi::as_stream(stream);
+    if (plan->alg == TKMI_SVD_BLOCKJACOBI) {
+        // every block in the same launches on the caller's stream
+        if (!plan->jac) {
+            const size_t nb = plan->m.size();
+            int rc = tkmi::jacobi_create(&plan->jac, (int64_t)nb, plan->m.data(), plan->n.data(), plan->a_off.data(),
+                                         plan->lda.data(), plan->u_off.data(), plan->s_off.data(), plan->v_off.data());
+            if (rc) return rc;
+        }
+        return tkmi::jacobi_svd(plan->jac, A, U, S, Vt, s, nullptr);
+    }
     // fork: every lane starts after the work already queued on the caller's stream
     TKMI_HIP(hipEventRecord(plan->start, s));
     for (auto &l : plan->lanes) TKMI_HIP(hipStreamWaitEvent(l.stream, plan->start, 0));
@@ -291,8 +313,11 @@ int tkmi_svd_exec(tkmi_svd_plan plan, const double *A, double *U, double *S, dou
 
 int tkmi_svd_plan_set_algorithm(tkmi_svd_plan plan, int32_t alg) {
     if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: null plan");
-    if (alg != TKMI_SVD_QRITERATION && alg != TKMI_SVD_DIVIDEANDCONQUER && alg != TKMI_SVD_JACOBI)
+    if (alg != TKMI_SVD_QRITERATION && alg != TKMI_SVD_DIVIDEANDCONQUER && alg != TKMI_SVD_JACOBI &&
+        alg != TKMI_SVD_BLOCKJACOBI)
         TKMI_FAIL(TKMI_ERR_ARGUMENT, "svd: unknown algorithm");
+    if (alg == TKMI_SVD_BLOCKJACOBI && plan->kind == 1)
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "eigh: block Jacobi is an SVD algorithm");
     plan->alg = alg;
     return TKMI_OK;
 }

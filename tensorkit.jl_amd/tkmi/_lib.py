@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libtkmi.so")
 TKMI_OK = 0
 TKMI_F64, TKMI_F32, TKMI_C128, TKMI_C64 = 0, 1, 2, 3
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
-SVD_QRITERATION, SVD_DIVIDEANDCONQUER, SVD_JACOBI = 0, 1, 2
+SVD_QRITERATION, SVD_DIVIDEANDCONQUER, SVD_JACOBI, SVD_BLOCKJACOBI = 0, 1, 2, 3
 
 c_i64p = C.POINTER(C.c_int64)
 c_i32p = C.POINTER(C.c_int32)

@@ -469,8 +469,88 @@ static void gemm_cases() {
     }
 }
 
+// Block Jacobi SVD (tkmi_jacobi.hip), called through its internal entry points: several
+// blocks of one call (square, tall with a partial 16-column block, wide (transposed),
+// tiny, one with an exactly zero column, one with two equal columns), each checked by
+// U·diag(S)·Vt = A, orthonormal U and Vt, and S non-negative and descending.
+struct emu_stream;
+namespace tkmi {
+struct JacobiWork;
+int jacobi_create(JacobiWork **out, int64_t nblocks, const int64_t *m, const int64_t *n, const int64_t *a_off,
+                  const int64_t *lda, const int64_t *u_off, const int64_t *s_off, const int64_t *vt_off);
+void jacobi_destroy(JacobiWork *w);
+int jacobi_svd(JacobiWork *w, const double *A, double *U, double *S, double *Vt, emu_stream *s, int *sweeps);
+}  // namespace tkmi
+
+static void jacobi_cases() {
+    const std::vector<std::pair<int64_t, int64_t>> shapes = {{40, 40}, {70, 33}, {20, 57}, {5, 5}, {50, 20}, {34, 34}};
+    std::vector<int64_t> m, n, aoff, lda, uoff, soff, voff;
+    int64_t alen = 0, ulen = 0, slen = 0, vlen = 0;
+    for (auto [mm, nn] : shapes) {
+        const int64_t r = std::min(mm, nn);
+        m.push_back(mm);
+        n.push_back(nn);
+        lda.push_back(mm + 3);
+        aoff.push_back(alen);
+        alen += (mm + 3) * nn;
+        uoff.push_back(ulen);
+        ulen += mm * r;
+        soff.push_back(slen);
+        slen += r;
+        voff.push_back(vlen);
+        vlen += r * nn;
+    }
+    std::vector<double> A(alen);
+    for (auto &x : A) x = drand();
+    for (int64_t i = 0; i < m[4]; ++i) A[aoff[4] + i + 7 * lda[4]] = 0.0;                      // a zero column
+    for (int64_t i = 0; i < m[5]; ++i) A[aoff[5] + i + 9 * lda[5]] = A[aoff[5] + i + 2 * lda[5]];  // equal columns
+    std::vector<double> U(ulen), S(slen), Vt(vlen);
+    tkmi::JacobiWork *w = nullptr;
+    int rc = tkmi::jacobi_create(&w, (int64_t)shapes.size(), m.data(), n.data(), aoff.data(), lda.data(), uoff.data(),
+                                 soff.data(), voff.data());
+    CHECK(rc == TKMI_OK, "jacobi_create rc=%d", rc);
+    int sweeps = 0;
+    rc = tkmi::jacobi_svd(w, A.data(), U.data(), S.data(), Vt.data(), nullptr, &sweeps);
+    CHECK(rc == TKMI_OK, "jacobi_svd rc=%d", rc);
+    tkmi::jacobi_destroy(w);
+    std::printf("block Jacobi: %d sweeps\n", sweeps);
+    for (size_t b = 0; b < shapes.size(); ++b) {
+        const int64_t mm = m[b], nn = n[b], r = std::min(mm, nn);
+        auto u = [&](int64_t i, int64_t k) { return U[uoff[b] + i + k * mm]; };
+        auto vt = [&](int64_t k, int64_t j) { return Vt[voff[b] + k + j * r]; };
+        double anorm = 0, rec = 0, ou = 0, ov = 0;
+        for (int64_t i = 0; i < mm; ++i)
+            for (int64_t j = 0; j < nn; ++j) {
+                double x = 0;
+                for (int64_t k = 0; k < r; ++k) x += u(i, k) * S[soff[b] + k] * vt(k, j);
+                const double a = A[aoff[b] + i + j * lda[b]];
+                anorm += a * a;
+                rec += (x - a) * (x - a);
+            }
+        for (int64_t k = 0; k < r; ++k)
+            for (int64_t l = 0; l < r; ++l) {
+                double du = 0, dv = 0;
+                for (int64_t i = 0; i < mm; ++i) du += u(i, k) * u(i, l);
+                for (int64_t j = 0; j < nn; ++j) dv += vt(k, j) * vt(l, j);
+                ou += (du - (k == l)) * (du - (k == l));
+                ov += (dv - (k == l)) * (dv - (k == l));
+            }
+        CHECK(std::sqrt(rec) <= 1e-12 * std::sqrt(anorm), "jacobi block %zu: reconstruction %g", b, std::sqrt(rec));
+        CHECK(std::sqrt(ou) <= 1e-12 * r && std::sqrt(ov) <= 1e-12 * r, "jacobi block %zu: orthonormality %g %g", b,
+              std::sqrt(ou), std::sqrt(ov));
+        for (int64_t k = 0; k < r; ++k) {
+            CHECK(S[soff[b] + k] >= 0.0, "jacobi block %zu: negative sigma", b);
+            if (k) CHECK(S[soff[b] + k] <= S[soff[b] + k - 1], "jacobi block %zu: sigma not descending", b);
+        }
+    }
+    CHECK(S[soff[4] + 19] == 0.0, "the zero column gives an exact zero sigma (%g)", S[soff[4] + 19]);
+}
+
 int main() {
     const double t0 = now();
+    jacobi_cases();
+    std::printf("jacobi cases done (%d failures)\n", g_fail);
+    std::fflush(stdout);
     transform_cases();
     std::printf("transform cases: %.1f s\n", now() - t0);
     std::printf("transform cases done (%d failures)\n", g_fail);

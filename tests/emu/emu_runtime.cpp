@@ -59,7 +59,7 @@ struct Barrier {
     }
 };
 
-constexpr int MAXT = 256, WAVE = 64;
+constexpr int MAXT = 1024, WAVE = 64;
 Barrier g_block;
 Barrier g_wave[MAXT / WAVE];
 double g_area[MAXT / WAVE][2][WAVE * 8];

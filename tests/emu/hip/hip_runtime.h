@@ -1,5 +1,5 @@
 // CPU stand-in for the part of the HIP runtime and of the device built-ins that libtkmi's
-// kernels use, so that tkmi_transform.hip and tkmi_gemm.hip compile unchanged as host C++
+// kernels use, so that tkmi_transform.hip, tkmi_gemm.hip and tkmi_jacobi.hip compile unchanged as host C++
 // (clang++ -x c++) for a bounds-checked run under AddressSanitizer.  Test infrastructure:
 // nothing here is part of the library.
 //
@@ -59,6 +59,11 @@ inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) {
     std::memcpy(d, s, n);
     return hipSuccess;
 }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) {
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipMemset(void *d, int v, size_t n) {
     std::memset(d, v, n);
     return hipSuccess;

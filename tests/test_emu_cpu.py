@@ -27,6 +27,7 @@ def test_kernels_bounds_checked_on_cpu():
         jobs = [
             (["-I", emu, "-x", "c++", os.path.join(csrc, "tkmi_transform.hip")], "transform.o"),
             (["-I", emu, "-x", "c++", os.path.join(csrc, "tkmi_gemm.hip")], "gemm.o"),
+            (["-I", emu, "-x", "c++", os.path.join(csrc, "tkmi_jacobi.hip")], "jacobi.o"),
             (["-I", emu, os.path.join(emu, "emu_runtime.cpp")], "runtime.o"),
             ([os.path.join(emu, "emu_check.cpp")], "check.o"),
         ]

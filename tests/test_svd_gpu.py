@@ -47,7 +47,8 @@ def reference_truncation(sv, howmany, atol):
     return counts, np.sqrt(err2)
 
 
-ALGS = ["qr_iteration", "divide_and_conquer", "jacobi"]
+ALGS = ["qr_iteration", "divide_and_conquer", "jacobi", "block_jacobi"]
+EIGH_ALGS = ["qr_iteration", "divide_and_conquer", "jacobi"]
 
 
 def _svd_checks(T, t, howmany, atol, alg=None):
@@ -161,7 +162,7 @@ def reference_eigh_truncation(vals, howmany, atol):
     return masks, np.sqrt(err2)
 
 
-@pytest.mark.parametrize("alg", ALGS)
+@pytest.mark.parametrize("alg", EIGH_ALGS)
 def test_eigh_full_and_trunc(T, alg):
     from tkmi.factorizations import eigh_full, eigh_trunc
     rng = np.random.default_rng(30)
