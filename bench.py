@@ -302,6 +302,9 @@ def main():
     ap.add_argument("--gemm-configs", default="", metavar="C0,C1,…",
                     help="counter workload: FP64 tile configuration per grouped GEMM (as the "
                          "timed run logs them) instead of autotuning under the profiler")
+    ap.add_argument("--counter-form", choices=("f64", "bf16x3"), default="f64",
+                    help="counter workload: the FP64 chain, or the float32 chain with 3×BF16 "
+                         "GEMMs (environments split once, as in the timed graph)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -373,10 +376,16 @@ def main():
     y_eager = H.apply()
     torch.cuda.synchronize()
     if args.counter_workload:
-        for _ in range(args.counter_workload):
-            H.apply()
+        if args.counter_form == "bf16x3":
+            Hc = H.astype(torch.float32)
+            Hc.mark_constant_operands(_lib.PREC_BF16X3)
+            for _ in range(args.counter_workload):
+                Hc.apply(precision=_lib.PREC_BF16X3)
+        else:
+            for _ in range(args.counter_workload):
+                H.apply()
         torch.cuda.synchronize()
-        log(f"counter workload: {args.counter_workload} eager FP64 applications")
+        log(f"counter workload: {args.counter_workload} eager {args.counter_form} applications")
         return
 
     # FP64 graph

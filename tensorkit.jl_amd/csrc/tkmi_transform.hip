@@ -66,6 +66,8 @@ struct TBlock {
     int32_t nother;
     int32_t u_ld;  // leading dimension of the (column-major) coefficient matrix
     int32_t dV;    // mode 2: the short dim, innermost on both sides, carried whole in a tile
+    int32_t off32; // every element of a tile lies within ±2^31 elements of the tile's corner in
+                   // every tree: offsets from the corner are formed in 32-bit arithmetic
     double inv_w;  // copy mode: 1 / size[dW], for the row of a tile's first element
     int32_t odims[MAXD];
     int64_t size[MAXD];
@@ -134,6 +136,8 @@ __device__ __forceinline__ T narrow(typename Acc<T>::type v) {
     else
         return (T)v;
 }
+__device__ __forceinline__ double fma_acc(double u, double x, double o) { return fma(u, x, o); }
+__device__ __forceinline__ d2_t fma_acc(double u, d2_t x, d2_t o) { return d2_t{fma(u, x.x, o.x), fma(u, x.y, o.y)}; }
 template <typename T>
 __device__ __forceinline__ T conj_of(T v) {
     if constexpr (Acc<T>::cplx) v.y = -v.y;
@@ -328,9 +332,23 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
 #pragma unroll
         for (int k = 0; k < E; ++k) o[j][k] = (A)0.0;
 
+    // positions of this thread's elements in the tile (the same in every tree)
+    int pw[E], pr[E], pb[E];
+    bool pok[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) pok[k] = pos_read(k, pw[k], pr[k], pb[k]);
+
     for (int ic = 0; ic < nread && has_alpha; ic += L) {
         const int nc = min(L, nread - ic);
         T x[L][E];
+        // Each present tree's E loads without a branch per element: elements outside the
+        // tile read the tile's first element (always inside it) and keep that value, which
+        // only reaches sums that are never stored (stores and LDS staging test the position
+        // again).  Absent trees (i ≥ nc) hold zeros.  The sums below run over all L trees
+        // with the absent ones' coefficients zero, so nothing right after a tree's loads
+        // needs their values, and the loads of every tree issue before the first sum (a
+        // per-tree branch around loads and sums together had it load, sum and wait tree
+        // after tree).
 #pragma unroll
         for (int i = 0; i < L; ++i) {
             if (i < nc) {
@@ -338,15 +356,31 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 const T *sp = src + corner(tr);
                 const int64_t sW = tr.str[dW], sR = COPY ? tr.str[dR] - W * tr.str[dW] : tr.str[dR];
                 const int64_t sB = dB >= 0 ? tr.str[dB] : 0;
+                if (B.off32) {
+                    // formed for every position (unsigned: wraps harmlessly outside the tile),
+                    // then 0 selected outside it: a select, not a branch per element
+                    const uint32_t sw = (uint32_t)sW, sr = (uint32_t)sR, sb = (uint32_t)sB;
 #pragma unroll
-                for (int k = 0; k < E; ++k) {
-                    int w, r, b;
-                    x[i][k] = pos_read(k, w, r, b)
-                                  ? __builtin_nontemporal_load(&sp[(int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB])
-                                  : (T)0;
-                    if constexpr (Acc<T>::cplx)
-                        if (sc.conj) x[i][k] = conj_of(x[i][k]);
+                    for (int k = 0; k < E; ++k) {
+                        const uint32_t ou = (uint32_t)pw[k] * sw + (uint32_t)pr[k] * sr + (uint32_t)pb[k] * sb;
+                        const int32_t o = pok[k] ? (int32_t)ou : 0;
+                        x[i][k] = __builtin_nontemporal_load(&sp[o]);
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < E; ++k) {
+                        const int64_t o =
+                            pok[k] ? (int64_t)pw[k] * sW + (int64_t)pr[k] * sR + (int64_t)pb[k] * sB : 0;
+                        x[i][k] = __builtin_nontemporal_load(&sp[o]);
+                    }
                 }
+                if constexpr (Acc<T>::cplx)
+                    if (sc.conj)
+#pragma unroll
+                        for (int k = 0; k < E; ++k) x[i][k] = conj_of(x[i][k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < E; ++k) x[i][k] = (T)0;
             }
         }
         if (!COPY) {
@@ -382,8 +416,10 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
             if (j < nwrite) {
 #pragma unroll
                 for (int i = 0; i < L; ++i) {
-                    if (i < nc) {
-                        const double uc = Umat[B.u_start + j + (int64_t)(ic + i) * B.u_ld];
+                    {
+                        // absent trees: coefficient 0 and zero sources (+0 added to a sum that
+                        // starts at +0 changes nothing, so results stay bit for bit)
+                        const double uc = i < nc ? Umat[B.u_start + j + (int64_t)(ic + i) * B.u_ld] : 0.0;
                         if (Acc<T>::cplx && single && sc.ai != 0.0) {
                             if constexpr (Acc<T>::cplx) {
                                 const d2_t u = {alpha * uc, sc.ai * uc};
@@ -391,9 +427,11 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                                 for (int k = 0; k < E; ++k) o[j][k] += cmul(u, widen(x[i][k]));
                             }
                         } else {
+                            // an explicit fused multiply-add per term (per component for
+                            // complex data): the same arithmetic for every element type
                             const double u = single ? alpha * uc : uc;
 #pragma unroll
-                            for (int k = 0; k < E; ++k) o[j][k] += u * widen(x[i][k]);
+                            for (int k = 0; k < E; ++k) o[j][k] = fma_acc(u, widen(x[i][k]), o[j][k]);
                         }
                     }
                 }
@@ -423,6 +461,20 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
             return scaled_sum(beta, (double)c, v);
         }
     };
+    // write positions (the read positions for copy tiles), the same in every tree
+    int qw[E], qr[E], qb[E];
+    bool qok[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+        if (COPY) {
+            qw[k] = pw[k];
+            qr[k] = pr[k];
+            qb[k] = pb[k];
+            qok[k] = pok[k];
+        } else {
+            qok[k] = pos_write(k, qw[k], qr[k], qb[k]);
+        }
+    }
     auto store_all = [&](auto store) {
 #pragma unroll
         for (int j = 0; j < L; ++j) {
@@ -431,23 +483,13 @@ __device__ __forceinline__ void transform_tile(const TBlock &B, const TTree *__r
                 T *dp = dst + corner(tw);
                 const int64_t sW = tw.str[dW], sR = COPY ? tw.str[dR] - W * tw.str[dW] : tw.str[dR];
                 const int64_t sB = dB >= 0 ? tw.str[dB] : 0;
-                if (has_beta) {
+                const uint32_t sw = (uint32_t)sW, sr = (uint32_t)sR, sb = (uint32_t)sB;
 #pragma unroll
-                    for (int k = 0; k < E; ++k) {
-                        int w, r, b;
-                        if (pos_write(k, w, r, b)) {
-                            T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
-                            store(narrow<T>(with_beta(o[j][k], *a)), a);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < E; ++k) {
-                        int w, r, b;
-                        if (pos_write(k, w, r, b)) {
-                            T *a = dp + (int64_t)w * sW + (int64_t)r * sR + (int64_t)b * sB;
-                            store(narrow<T>(o[j][k]), a);
-                        }
+                for (int k = 0; k < E; ++k) {
+                    if (qok[k]) {
+                        T *a = B.off32 ? dp + (int32_t)((uint32_t)qw[k] * sw + (uint32_t)qr[k] * sr + (uint32_t)qb[k] * sb)
+                                       : dp + ((int64_t)qw[k] * sW + (int64_t)qr[k] * sR + (int64_t)qb[k] * sB);
+                        store(narrow<T>(has_beta ? with_beta(o[j][k], *a) : o[j][k]), a);
                     }
                 }
             }
@@ -744,6 +786,43 @@ void tile_block(TBlock &b, const std::vector<TTree> &trees, int L, bool allow_sl
     }
 }
 
+// Whether every element of every tile of block b lies within 2^31 − 1 elements of its tile's
+// corner in each of the trees [rt, rt + nr) and [wt, wt + nw): the largest in-tile position
+// along each tiled dim times |stride| (copy tiles: up to `area` along dW and the rows a
+// tile spans, with the row step of the kernel's copy numbering), summed.
+bool tile_offsets_fit_32(const TBlock &b, const std::vector<TTree> &trees, int64_t rt, int nr, int64_t wt, int nw,
+                         int area, int dB, int lgB) {
+    const int64_t W = b.size[b.dW];
+    int64_t mw, mr, mb = 0;
+    int db = -1;
+    if (b.mode == 0) {
+        mw = area;
+        mr = W >= area ? 1 : (area + W - 1) / W + 1;
+    } else {
+        mw = b.tW;
+        mr = b.tR;
+        if (b.mode == 2) {
+            db = b.dV;
+            mb = b.size[b.dV];
+        } else if (dB >= 0) {
+            db = dB;
+            mb = (int64_t)1 << lgB;
+        }
+    }
+    auto fits = [&](const TTree &t) {
+        const int64_t sw = std::llabs(t.str[b.dW]);
+        const int64_t sr = b.mode == 0 ? std::llabs(t.str[b.dR] - W * t.str[b.dW]) : std::llabs(t.str[b.dR]);
+        const int64_t sb = db >= 0 ? std::llabs(t.str[db]) : 0;
+        const double span = (double)mw * (double)sw + (double)mr * (double)sr + (double)mb * (double)sb;
+        return span < 2147483647.0;
+    };
+    for (int i = 0; i < nr; ++i)
+        if (!fits(trees[rt + i])) return false;
+    for (int j = 0; j < nw; ++j)
+        if (!fits(trees[wt + j])) return false;
+    return true;
+}
+
 // Canonicalise one work item: `sizes` are the dst-order sizes; `strides[t][k]` the
 // strides (in dst order) of every tree t that takes part.  Returns the kept dims.
 std::vector<std::pair<int64_t, std::vector<int>>> canon_dims(
@@ -859,6 +938,8 @@ int finalize_plan(tkmi_transform_plan_s *p, std::vector<HostItem> &items) {
                 const int c = merge ? cmax : class_of(b.nread, b.nwrite);
                 int sdb, slg;
                 tile_block(b, trees, class_L(c), slice_class[c], sdb, slg);
+                b.off32 = tile_offsets_fit_32(b, trees, rt, Cn, b.write_tree, b.nwrite, class_area(class_L(c)),
+                                              sdb, slg) ? 1 : 0;
                 // vector-transpose tiles run in a launch of their own kernel, so that the
                 // other kernels keep their register counts
                 PlanSide &sd = sides[c + (b.mode == 2 ? NCLASS : 0)];
