@@ -47,7 +47,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
 # Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
 TRAFFIC_PMC = {
-    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): round((80849.4 * 2 + 68539.3) * 1024),  # L·x
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): round((80852.8 * 2 + 68587.2) * 1024),  # L·x
+    ("gemm_k4s_kernel<double, 64, 64, 16>", 243712): round((115131.4 * 2 + 27080.2) * 1024),  # ·R, split-K 2 partials
     ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): round((96597.6 * 2 + 57122.2) * 1024),  # ·R, split-K 5 partials
 }
 
@@ -102,7 +103,8 @@ def time_launches(fn, reps, torch):
     """Average duration (ms) of `fn()` launches on the current stream, HIP events."""
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    fn()
+    for _ in range(3):
+        fn()
     e0.record(st)
     for _ in range(reps):
         fn()
@@ -111,14 +113,48 @@ def time_launches(fn, reps, torch):
     return e0.elapsed_time(e1) / reps
 
 
+def time_in_chain(fns, timed, reps, torch, lib):
+    """Average execution time (ms) of the grouped-GEMM MFMA launches `timed` (indices into
+    `fns`, each an (fn, plan handle) pair), measured where they run: the whole chain `fns`
+    is re-issued `reps` times in its own order on the current stream, and each timed
+    launch records its kernel's own start and end in a pair of HIP events
+    (tkmi_gemm_plan_time_next, hipExtLaunchKernel), with no event packets around it.  A
+    launch timed in a loop of its own runs in another power state: back-to-back FP64 MFMA
+    launches hold lower clocks than the same launches between the chain's
+    bandwidth-bound ones."""
+    import ctypes as C
+    L = lib.load()
+    st = torch.cuda.current_stream()
+    for _ in range(3):
+        for f, _ in fns:
+            f()
+    ev = {i: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+          for i in timed}
+    for pairs in ev.values():  # create the events (torch creates them at the first record)
+        for a, b in pairs:
+            a.record(st)
+            b.record(st)
+    torch.cuda.synchronize()
+    for r in range(reps):
+        for i, (f, h) in enumerate(fns):
+            if i in ev:
+                a, b = ev[i][r]
+                lib.check(L.tkmi_gemm_plan_time_next(h, C.c_void_p(a.cuda_event), C.c_void_p(b.cuda_event)),
+                          "time_next")
+            f()
+    torch.cuda.synchronize()
+    return {i: sum(a.elapsed_time(b) for a, b in pairs) / reps for i, pairs in ev.items()}
+
+
 def kernel_breakdown(trace, T, lib, torch, reps=20):
-    """Per-launch timing of every kernel in one chain (eager re-issue on the
-    current stream); a grouped GEMM is timed as its MFMA launch and its skinny
-    (bandwidth-bound) launch separately."""
+    """Per-launch timing of every kernel in one chain (eager re-issue on the current
+    stream); a grouped GEMM is timed as its MFMA launch, its split-K reduction and its
+    skinny (bandwidth-bound) launch separately."""
     import ctypes as C
     L = lib.load()
     s = torch.cuda.current_stream().cuda_stream
     gemms, perms = [], []
+    fns, slots = [], []  # launch closures in chain order; slots[i] = (record, key) it times
     for op in trace:
         if op[0] == "gemm":
             _, plan, prec, (ta_, _), (tb_, _), (tc_, nc), a, b = op
@@ -127,8 +163,8 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
             L.tkmi_gemm_plan_query(plan.handle, prec, C.byref(tile), C.byref(nt))
             fm, fs, ns = C.c_double(), C.c_double(), C.c_int64()
             L.tkmi_gemm_plan_split(plan.handle, C.byref(fm), C.byref(fs), C.byref(ns))
-            sk, slots = C.c_int64(1), C.c_int64(0)
-            L.tkmi_gemm_plan_splitk(plan.handle, tile.value, C.byref(sk), C.byref(slots))
+            sk, nslots = C.c_int64(1), C.c_int64(0)
+            L.tkmi_gemm_plan_splitk(plan.handle, tile.value, C.byref(sk), C.byref(nslots))
             # algorithmic HBM bytes of the MFMA launch: A, B read once, C written once (and
             # read once with β≠0), over the groups the plan gives to the MFMA kernel
             # (smaller output side > 32 or K > 64)
@@ -143,37 +179,50 @@ def kernel_breakdown(trace, T, lib, torch, reps=20):
                 # the MFMA launch alone (part 1 | 4: without its split-K reduction, the
                 # kernel a trace reports under this name); the reduction timed on its own
                 bits = 5 if part == 1 else part
-                fn = lambda plan=plan, prec=prec, bits=bits, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
-                    plan.handle, prec, bits, a, b, pa, pb, pc, s)
-                ms = time_launches(fn, reps, torch)
-                red_ms = None
+                rec = {"flops": flops, "ms": None, "reduce_ms": None, "kernel": name,
+                       "tiles": nt.value if part == 1 else None,
+                       "splitk": sk.value if part == 1 else None, "slots": nslots.value if part == 1 else None,
+                       "grid_x": nt.value * 256 if part == 1 else None,
+                       "alg_bytes": alg_bytes if part == 1 else None}
+                gemms.append(rec)
+                fns.append((lambda plan=plan, prec=prec, bits=bits, pa=pa, pb=pb, pc=pc, a=a, b=b:
+                            L.tkmi_gemm_exec_parts(plan.handle, prec, bits, a, b, pa, pb, pc, s), plan.handle))
+                slots.append((rec, "ms"))
                 if part == 1 and sk.value > 1:
-                    fr = lambda plan=plan, prec=prec, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_gemm_exec_parts(
-                        plan.handle, prec, 8, a, b, pa, pb, pc, s)
-                    red_ms = time_launches(fr, reps, torch)
-                    # leave C as a full call writes it
-                    L.tkmi_gemm_exec_parts(plan.handle, prec, 1, a, b, pa, pb, pc, s)
-                gemms.append({"flops": flops, "ms": ms, "reduce_ms": red_ms, "kernel": name,
-                              "tiles": nt.value if part == 1 else None,
-                              "splitk": sk.value if part == 1 else None, "slots": slots.value if part == 1 else None,
-                              "grid_x": nt.value * 256 if part == 1 else None,
-                              "alg_bytes": alg_bytes if part == 1 else None})
+                    fns.append((lambda plan=plan, prec=prec, pa=pa, pb=pb, pc=pc, a=a, b=b:
+                                L.tkmi_gemm_exec_parts(plan.handle, prec, 8, a, b, pa, pb, pc, s), plan.handle))
+                    slots.append((rec, "reduce_ms"))
         elif op[0] == "fused":
             _, plan, _, (ta_, na), (tb_, nb), (tc_, nc), a, b = op
             pa, pb, pc = ta_.data_ptr(), tb_.data_ptr(), tc_.data_ptr()
-            fn = lambda plan=plan, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_transform_exec_gather(
-                plan.handle, 0, pa, pc, pb, a, b, s)
-            ms = time_launches(fn, reps, torch)
             # algorithmic bytes: A read once, C written once (and read with β≠0)
-            perms.append({"elements": nc, "bytes": 8 * (na + nc * (2 if b != 0 else 1)), "ms": ms,
-                          "entries": plan.nentries, "fused": True})
+            rec = {"elements": nc, "bytes": 8 * (na + nc * (2 if b != 0 else 1)), "ms": None,
+                   "entries": plan.nentries, "fused": True}
+            perms.append(rec)
+            fns.append((lambda plan=plan, pa=pa, pb=pb, pc=pc, a=a, b=b: L.tkmi_transform_exec_gather(
+                plan.handle, 0, pa, pc, pb, a, b, s), None))
+            slots.append((rec, "ms"))
         else:
             _, plan, perm, (ts_, ns_), (td_, nd), a, b = op
             ps, pd = ts_.data_ptr(), td_.data_ptr()
-            fn = lambda plan=plan, ps=ps, pd=pd, a=a, b=b: L.tkmi_transform_exec(plan.handle, 0, ps, pd, a, b, s)
-            ms = time_launches(fn, reps, torch)
-            perms.append({"elements": nd, "bytes": nd * (24 if b != 0 else 16), "ms": ms,
-                          "entries": plan.nentries})
+            rec = {"elements": nd, "bytes": nd * (24 if b != 0 else 16), "ms": None, "entries": plan.nentries}
+            perms.append(rec)
+            fns.append((lambda plan=plan, ps=ps, pd=pd, a=a, b=b: L.tkmi_transform_exec(plan.handle, 0, ps, pd, a, b, s),
+                        None))
+            slots.append((rec, "ms"))
+    # grouped-GEMM MFMA launches: timed inside the re-issued chain; the other launches
+    # (bandwidth-bound, and split-K reductions): each in a loop of its own, where the
+    # event pair around the loop costs nothing per launch
+    timed = [i for i, (rec, key) in enumerate(slots) if key == "ms" and rec.get("tiles")]
+    for i, ms in time_in_chain(fns, timed, reps, torch, lib).items():
+        slots[i][0]["ms"] = ms
+    for i, (rec, key) in enumerate(slots):
+        if i not in timed:
+            rec[key] = time_launches(fns[i][0], reps, torch)
+    # leave every output as the chain writes it
+    for f, _ in fns:
+        f()
+    torch.cuda.synchronize()
     return gemms, perms
 
 
@@ -418,6 +467,37 @@ def main():
     log(f"FP64 step {ms64:.4f} ms (wall {wall/args.steps*1e3:.4f} ms), {flops/ms64/1e6:.1f} GFLOP/s")
     eager_err = float(np.abs(y_ref - y_eager.numpy()).max())
 
+    # per-kernel breakdown and roofline of the dominant kernel, right after the FP64 timed
+    # window so that the launches run at the clocks that window ran at (the phases below
+    # leave the GPU idle for host work, and its clocks fall meanwhile)
+    gemms, perms = kernel_breakdown(trace, T, _lib, torch, reps=100)
+    by_kernel = {}
+    for g in gemms:
+        d = by_kernel.setdefault(g["kernel"], {"flops": 0.0, "ms": 0.0, "calls": 0, "launches": []})
+        d["flops"] += g["flops"]
+        d["ms"] += g["ms"]
+        d["calls"] += 1
+        d["launches"].append(g)
+    mfma_kernels = {k: v for k, v in by_kernel.items() if k != "gemm_skinny_kernel"}
+    dom_name, dom = max(mfma_kernels.items(), key=lambda kv: kv[1]["ms"])
+    achieved_tf = dom["flops"] / dom["ms"] / 1e9
+    # measured HBM traffic per launch (FETCH_SIZE·2 + WRITE_SIZE, copy-calibrated), known
+    # only for the launches profiled in that configuration; null otherwise
+    measured = [TRAFFIC_PMC.get((dom_name, g["grid_x"])) for g in dom["launches"]]
+    traffic = (sum(measured) / len(measured)) if measured and all(x is not None for x in measured) else None
+    alg_bytes = sum(g["alg_bytes"] for g in dom["launches"]) / len(dom["launches"])
+    perm_bytes = sum(p["bytes"] for p in perms)
+    perm_ms = sum(p["ms"] for p in perms)
+    big = max(perms, key=lambda p: p["bytes"]) if perms else None
+    log("gemm launches: " + "; ".join(f"{g['kernel']} {g['flops']/1e9:.3f} GF {g['ms']*1e3:.1f} us "
+                                       f"{g['flops']/g['ms']/1e9:.2f} TF/s"
+                                       + (f" ({g['tiles']} tiles, split-K {g['splitk']}, {g['slots']} slots)"
+                                          if g.get('tiles') else "")
+                                       + (f" + splitk_reduce {g['reduce_ms']*1e3:.1f} us" if g.get('reduce_ms') else "")
+                                       for g in gemms))
+    log("permutes: " + "; ".join(f"{p['elements']} el {p['ms']*1e3:.1f} us {p['bytes']/p['ms']/1e6:.0f} GB/s"
+                                 for p in perms))
+
     # reduced-precision forms (separate graphs): the 3×BF16 GEMMs on the FP64 chain, then the
     # float32 chain (operands and x rounded to FP32 once, every intermediate FP32) with FP32
     # MFMA GEMMs (a Float32 mul!) and with 3×BF16 GEMMs, its GEMMs tuned on FP32 data
@@ -530,34 +610,6 @@ def main():
                                                             if "cpu_port_one_thread_ms" in r else ""))
         torch.cuda.empty_cache()
 
-    # per-kernel breakdown and roofline of the dominant kernel
-    gemms, perms = kernel_breakdown(trace, T, _lib, torch, reps=100)
-    by_kernel = {}
-    for g in gemms:
-        d = by_kernel.setdefault(g["kernel"], {"flops": 0.0, "ms": 0.0, "calls": 0, "launches": []})
-        d["flops"] += g["flops"]
-        d["ms"] += g["ms"]
-        d["calls"] += 1
-        d["launches"].append(g)
-    mfma_kernels = {k: v for k, v in by_kernel.items() if k != "gemm_skinny_kernel"}
-    dom_name, dom = max(mfma_kernels.items(), key=lambda kv: kv[1]["ms"])
-    achieved_tf = dom["flops"] / dom["ms"] / 1e9
-    # measured HBM traffic per launch (FETCH_SIZE·2 + WRITE_SIZE, copy-calibrated), known
-    # only for the launches profiled in that configuration; null otherwise
-    measured = [TRAFFIC_PMC.get((dom_name, g["grid_x"])) for g in dom["launches"]]
-    traffic = (sum(measured) / len(measured)) if measured and all(x is not None for x in measured) else None
-    alg_bytes = sum(g["alg_bytes"] for g in dom["launches"]) / len(dom["launches"])
-    perm_bytes = sum(p["bytes"] for p in perms)
-    perm_ms = sum(p["ms"] for p in perms)
-    big = max(perms, key=lambda p: p["bytes"]) if perms else None
-    log("gemm launches: " + "; ".join(f"{g['kernel']} {g['flops']/1e9:.3f} GF {g['ms']*1e3:.1f} us "
-                                       f"{g['flops']/g['ms']/1e9:.2f} TF/s"
-                                       + (f" ({g['tiles']} tiles, split-K {g['splitk']}, {g['slots']} slots)"
-                                          if g.get('tiles') else "")
-                                       + (f" + splitk_reduce {g['reduce_ms']*1e3:.1f} us" if g.get('reduce_ms') else "")
-                                       for g in gemms))
-    log("permutes: " + "; ".join(f"{p['elements']} el {p['ms']*1e3:.1f} us {p['bytes']/p['ms']/1e6:.0f} GB/s"
-                                 for p in perms))
 
     out = {
         "metric": METRIC,

@@ -261,6 +261,14 @@ int tkmi_gemm_plan_autotune_dtype(tkmi_gemm_plan plan, int32_t precision, int32_
  * graph holds only the splits its capture made: make one eager 3×BF16 call first.     */
 int tkmi_gemm_plan_set_constant(tkmi_gemm_plan plan, int32_t operands);
 int tkmi_gemm_plan_invalidate(tkmi_gemm_plan plan);
+/* Launch timing: the next MFMA-tile kernel this plan launches (any precision; not its
+ * split pass, split-K reduction or skinny launch) records its own start and end in the
+ * two events (hipEvent_t, created with timing enabled) through hipExtLaunchKernel, so
+ * hipEventElapsedTime gives that kernel's execution time with no event packets of
+ * their own around it on the stream.  One-shot: cleared by that launch; NULL, NULL
+ * clears it.  No counterpart in the reference: a measurement hook for timing a launch
+ * where it runs, between the other launches of a chain; not for graph capture.      */
+int tkmi_gemm_plan_time_next(tkmi_gemm_plan plan, void *start_event, void *stop_event);
 /* Configuration index and tile count the next exec with `precision` uses.      */
 int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *config, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);

@@ -22,6 +22,8 @@
 //       (B likewise with j), D as above.
 #include "tkmi_internal.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstring>
 #include <functional>
@@ -1381,6 +1383,9 @@ struct tkmi_gemm_plan_s {
     std::vector<RGroup> zscale;
     int64_t zscale_total = 0;
     RGroup *d_zscale = nullptr;
+    // tkmi_gemm_plan_time_next: events that the next MFMA-tile launch records its own
+    // start and end in (hipExtLaunchKernel), then cleared
+    hipEvent_t time_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1648,23 +1653,34 @@ int auto_cfg(const tkmi_gemm_plan_s *) {
 
 int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
 
+// A kernel launch on s; with timing events (tkmi_gemm_plan_time_next) the launch records
+// its own start and end in them.
+template <typename K, typename... Args>
+void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t *ev, Args... args) {
+    if (ev && ev[0])
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
 // One launch of an FP64 / FP32 MFMA kernel over the tiles of a configuration.
 template <typename TC, typename TD, int BM, int BN, int BK, int NBUF>
-void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha, double beta, hipStream_t s) {
+void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha, double beta, hipStream_t s,
+               const hipEvent_t *ev) {
     const dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
     if constexpr (NBUF == 5)
-        hipLaunchKernelGGL((gemm_pks_kernel<TC, TD, BM, BN, BK>), dim3((unsigned)ts.nslots), block, 0, s,
-                           ts.d_groups, ng, ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
+        launch_timed(gemm_pks_kernel<TC, TD, BM, BN, BK>, dim3((unsigned)ts.nslots), block, s, ev, ts.d_groups, ng,
+                     ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
     else if constexpr (NBUF == 3)
-        hipLaunchKernelGGL((gemm_k4p_kernel<TC, TD, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
+        launch_timed(gemm_k4p_kernel<TC, TD, BM, BN, BK>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C,
+                     ts.ws, alpha, beta);
     else if constexpr (NBUF == 2)
-        hipLaunchKernelGGL((gemm_k4_kernel<TC, TD, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
+        launch_timed(gemm_k4_kernel<TC, TD, BM, BN, BK>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C,
+                     ts.ws, alpha, beta);
     else
-        hipLaunchKernelGGL((gemm_k4s_kernel<TC, TD, BM, BN, BK>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles, A,
-                           B, C, ts.ws, alpha, beta);
+        launch_timed(gemm_k4s_kernel<TC, TD, BM, BN, BK>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C,
+                     ts.ws, alpha, beta);
 }
 
 constexpr int dtype_of(double *) { return TKMI_F64; }
@@ -1675,11 +1691,18 @@ void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const TD *A, con
               double beta, hipStream_t s, bool tiles, bool reduce) {
     dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (tiles) {  // the timing events go to this launch's MFMA kernel only
+        ev[0] = p->time_ev[0];
+        ev[1] = p->time_ev[1];
+        p->time_ev[0] = p->time_ev[1] = nullptr;
+    }
     if (!tiles) {
     } else if (prec == TKMI_PREC_F64) {
-        if constexpr (std::is_same<TD, double>::value) launch_k4<double, TD, BM, BN, BK, NBUF>(ts, A, B, C, alpha, beta, s);
+        if constexpr (std::is_same<TD, double>::value)
+            launch_k4<double, TD, BM, BN, BK, NBUF>(ts, A, B, C, alpha, beta, s, ev);
     } else if (prec == TKMI_PREC_F32) {
-        launch_k4<float, TD, BM, BN, BK, NBUF>(ts, A, B, C, alpha, beta, s);
+        launch_k4<float, TD, BM, BN, BK, NBUF>(ts, A, B, C, alpha, beta, s, ev);
     } else {
         // split the operands whose panels do not already hold them (constant operands from
         // the same pointer and element type as the last split keep theirs)
@@ -1695,11 +1718,11 @@ void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const TD *A, con
         p->split_src[1] = (p->const_ops & 2) ? (const void *)B : nullptr;
         p->split_dt[0] = p->split_dt[1] = dt;
         if constexpr (NBUF == 3)
-            hipLaunchKernelGGL((gemm_bf16x3_kernel<TD, BM, BN, 2>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
-                               (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
+            launch_timed(gemm_bf16x3_kernel<TD, BM, BN, 2>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles,
+                         (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
         else
-            hipLaunchKernelGGL((gemm_bf16x3_kernel<TD, BM, BN, 1>), grid, block, 0, s, ts.d_groups, ng, ts.ntiles,
-                               (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
+            launch_timed(gemm_bf16x3_kernel<TD, BM, BN, 1>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles,
+                         (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
     }
     if (reduce && ts.relems > 0) {
         const int64_t blocks = std::min<int64_t>(ts.relems / RCH, (int64_t)1 << 20);
@@ -2169,6 +2192,14 @@ int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *config
     const int c = cfg_of(plan, precision);
     if (config) *config = c;
     if (ntiles) *ntiles = plan->ts[c].ntiles;
+    return TKMI_OK;
+}
+
+int tkmi_gemm_plan_time_next(tkmi_gemm_plan plan, void *start_event, void *stop_event) {
+    if (!plan) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: null plan");
+    if (!start_event != !stop_event) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: time_next needs both events or neither");
+    plan->time_ev[0] = (hipEvent_t)start_event;
+    plan->time_ev[1] = (hipEvent_t)stop_event;
     return TKMI_OK;
 }
 

@@ -276,6 +276,55 @@ def test_exec_parts_split_reduction_separately(lib):
         L.tkmi_gemm_plan_destroy(h)
 
 
+def test_time_next_records_the_mfma_launch(lib):
+    """tkmi_gemm_plan_time_next: the next MFMA-tile launch records its own start and end
+    in the two events (elapsed time positive, within the time of a loop around it), the
+    result is the untimed launch's bit for bit in every precision form, the hook is
+    one-shot, and a single event is refused."""
+    rng = np.random.default_rng(31)
+    shapes = [(512, 384, 290), (200, 150, 3000), (300, 8, 20)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0, 1, 0], [0, 0, 1])
+    arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+    cols = [lib.i64(arr[:, i]) for i in range(11)]
+    ta, tb = lib.i32(arr[:, 5]), lib.i32(arr[:, 8])
+    L = lib.load()
+    h = C.c_void_p()
+    lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                      cols[4][1], ta[1], cols[6][1], cols[7][1], tb[1], cols[9][1], cols[10][1]),
+              "plan")
+    try:
+        A, B, C0 = rng.standard_normal(na), rng.standard_normal(nb), rng.standard_normal(nc)
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        st = torch.cuda.current_stream()
+        s = st.cuda_stream
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        w0, w1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        e1.record(st)
+        torch.cuda.synchronize()
+        for prec in (0, 1, 2):
+            plain = torch.from_numpy(C0).cuda()
+            lib.check(L.tkmi_gemm_exec(h, prec, 0.75, 0.5, dA.data_ptr(), dB.data_ptr(), plain.data_ptr(), s), "exec")
+            timed = torch.from_numpy(C0).cuda()
+            lib.check(L.tkmi_gemm_plan_time_next(h, C.c_void_p(e0.cuda_event), C.c_void_p(e1.cuda_event)), "time")
+            w0.record(st)
+            lib.check(L.tkmi_gemm_exec(h, prec, 0.75, 0.5, dA.data_ptr(), dB.data_ptr(), timed.data_ptr(), s), "exec")
+            w1.record(st)
+            torch.cuda.synchronize()
+            assert torch.equal(timed, plain), prec
+            ms, outer = e0.elapsed_time(e1), w0.elapsed_time(w1)
+            assert 0.0 < ms <= outer, (prec, ms, outer)
+            # one-shot: a later launch leaves the events as they are
+            lib.check(L.tkmi_gemm_exec(h, prec, 0.75, 0.5, dA.data_ptr(), dB.data_ptr(), timed.data_ptr(), s), "exec")
+            torch.cuda.synchronize()
+            assert e0.elapsed_time(e1) == ms, prec
+        assert L.tkmi_gemm_plan_time_next(h, C.c_void_p(e0.cuda_event), None) != 0
+        lib.check(L.tkmi_gemm_plan_time_next(h, None, None), "clear")
+    finally:
+        torch.cuda.synchronize()
+        L.tkmi_gemm_plan_destroy(h)
+
+
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_dot_groups(lib, ta, tb):
     """1×1 results with a long K (full contractions to a scalar) run as two-pass dot
