@@ -47,6 +47,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
 # Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
 TRAFFIC_PMC = {
+    ("gemm_k4d_kernel", 545280): round((79805.7 * 2 + 77352.4) * 1024),  # L·x, direct-load kernel
+    ("gemm_k4d_kernel", 243712): round((98405.4 * 2 + 28097.3) * 1024),  # ·R, split-K 2 partials
     ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): round((80852.8 * 2 + 68587.2) * 1024),  # L·x
     ("gemm_k4s_kernel<double, 64, 64, 16>", 243712): round((115131.4 * 2 + 27080.2) * 1024),  # ·R, split-K 2 partials
     ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): round((96597.6 * 2 + 57122.2) * 1024),  # ·R, split-K 5 partials
@@ -94,7 +96,12 @@ def gemm_kernel_name(prec, cfg, L):
     bm, bn, bk, nb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     L.tkmi_gemm_config_info(cfg, C.byref(bm), C.byref(bn), C.byref(bk), C.byref(nb))
     bm, bn, bk = bm.value, bn.value, bk.value
-    k4 = {2: "gemm_k4_kernel", 1: "gemm_k4s_kernel", 3: "gemm_k4p_kernel"}[nb.value]
+    # nbuf 6 is the direct-load FP64 kernel (untransposed operands, the H_eff GEMMs' case);
+    # in FP32 it runs the one-buffer kernel
+    k4 = {2: "gemm_k4_kernel", 1: "gemm_k4s_kernel", 3: "gemm_k4p_kernel", 5: "gemm_pks_kernel",
+          6: "gemm_k4s_kernel"}[nb.value]
+    if prec == 0 and nb.value == 6:
+        return "gemm_k4d_kernel"
     return {0: f"{k4}<double, {bm}, {bn}, {bk}>", 1: f"{k4}<float, {bm}, {bn}, {bk}>",
             2: f"gemm_bf16x3_kernel<{bm}, {bn}>"}[prec]
 

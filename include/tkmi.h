@@ -197,9 +197,13 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
 /* Tile configuration i: BM×BN output tile, BK-deep K slab and the FP64/FP32 kernel's
  * staging (2: double-buffered LDS, gemm_k4_kernel; 1: one LDS buffer and one register
  * stage, gemm_k4s_kernel; 3: one LDS buffer and two register stages, gemm_k4p_kernel;
- * 5: the persistent gemm_pks_kernel).  Configurations 0–9 take the split-K factor of
- * the schedule model; 10, 11 and 12 are configuration 5 with the factor fixed at 1, 2
- * and 3 (tkmi_gemm_plan_splitk reports the factor of each).                     */
+ * 5: the persistent gemm_pks_kernel; 6: gemm_k4d_kernel, direct global → LDS loads into
+ * two LDS stages, FP64 arithmetic with untransposed A and B only).  Configurations 0–9
+ * and 13 take the split-K factor of the schedule model; 10, 11 and 12 are configuration 5
+ * with the factor fixed at 1, 2 and 3, and 14, 15 and 16 configuration 13 likewise.
+ * Where the direct-load kernel does not apply (FP32 or 3×BF16 arithmetic, FP32 data, a
+ * transposed operand) configurations 13–16 run configuration 5's kernel with their own
+ * split factor (tkmi_gemm_plan_splitk reports the factor of each).               */
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
  * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64) with their split-K

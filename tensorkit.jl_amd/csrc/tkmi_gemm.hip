@@ -398,6 +398,200 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
 }
 
 // --------------------------------------------------------------------------------
+// FP64 kernel with direct global → LDS loads (A and B both untransposed: A m×k and B k×n
+// column-major, the blocks of plain TensorMaps).  Slabs move with global_load_lds_dwordx4:
+// each lane fetches 16 bytes that land at LDS base + 16·lane, so no VGPR staging and no LDS
+// store instructions; two LDS stages let the loads of slab kt + 1 run while slab kt is
+// multiplied, with one barrier per slab.  A 64×64 tile, 4 waves (2×2), 16-deep slabs;
+// 16 KB of LDS per stage and 80 VGPRs, five workgroups per CU.  (A third stage measured
+// slower: it leaves room for three workgroups per CU.)
+//
+// LDS layouts of a stage (16-byte chunks):
+//  A: chunk (k, p) = rows 2p, 2p+1 of column k (contiguous in memory), at slot 32k + p.  A
+//     wave's two 16-row fragments take the even and the odd rows of its 32 rows, so one
+//     chunk read gives a lane its A value for both fragments.
+//  B: chunk (q, j) = rows 2q, 2q+1 (k) of column j, at slot 8j + (q ^ (j & 7)): a column's
+//     16 k stay contiguous (coalesced fetch) and the 16 lanes that read one k pair of 16
+//     columns hit distinct banks.
+// Edge tiles load directly too, from clamped addresses: a B column past n reads column
+// n − 1, an A row pair at or past m − 1 reads rows (m − 1, m); what lands in rows or
+// columns past C only reaches outputs that are not stored.  Row m of column k lies inside
+// A's block for every k but the group's last (lda ≥ m), so on a tile past m the slab that
+// holds the group's last column goes through registers, as does a split's partial last
+// slab (zero filled past K).  Byte offsets inside a slab are 32-bit (the host picks this
+// kernel only when they fit).
+constexpr int DL_CH = 64 * 16 / 2;  // 16-byte chunks per operand slab (512)
+
+__global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                         int64_t ntiles, const double *__restrict__ A,
+                                                         const double *__restrict__ Bm, double *__restrict__ C,
+                                                         double *__restrict__ ws, double alpha, double beta) {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    __shared__ __attribute__((aligned(16))) double Ls[2][2][2 * DL_CH];
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t tm = lmn / G.tiles_n, tn = lmn % G.tiles_n;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const int64_t lda = G.lda, ldb = G.ldb;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const bool m_edge = row0 + BM > m, n_edge = col0 + BN > n;
+    const int nfull = (int)((K - kbeg) / BK);  // whole slabs
+    const bool tail = (K - kbeg) % BK != 0;    // a last partial slab
+    const int nslab = nfull + (tail ? 1 : 0);
+    const int ndirect = (m_edge && !tail && K == G.k) ? max(nfull - 1, 0) : nfull;
+    const double *__restrict__ Aslab = A + G.a_off + row0 + kbeg * lda;
+    const double *__restrict__ Bslab = Bm + G.b_off + col0 * ldb + kbeg;
+
+    d4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0, 0, 0, 0};
+
+    // direct path: wave w issues chunks [64(2w + e), 64(2w + e) + 64) of A and of B, e < 2;
+    // chunk c is A (k = c / 32, p = c % 32) and B (j = c / 8, q = (c % 8) ^ (j & 7))
+    const int rlast = (int)min((int64_t)(BM - 1), m - 1 - row0), clast = (int)min((int64_t)(BN - 1), n - 1 - col0);
+    uint32_t aoff[2], boff[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int c = 64 * (2 * wave + e) + lane;
+        aoff[e] = (uint32_t)(8 * (min(2 * (c & 31), rlast) + (c >> 5) * lda));
+        const int j = c >> 3, q = (c & 7) ^ (j & 7);
+        boff[e] = (uint32_t)(8 * (2 * q + min(j, clast) * ldb));
+    }
+    auto issue = [&](int kt, int st) {
+        const char *ab = reinterpret_cast<const char *>(Aslab + (int64_t)kt * BK * lda);
+        const char *bb = reinterpret_cast<const char *>(Bslab + (int64_t)kt * BK);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(ab + aoff[e]),
+                (__attribute__((address_space(3))) void *)&Ls[st][0][2 * 64 * (2 * wave + e)], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(bb + boff[e]),
+                (__attribute__((address_space(3))) void *)&Ls[st][1][2 * 64 * (2 * wave + e)], 16, 0, 0);
+        }
+    };
+    // register path: thread t fills chunks t and t + 256 of A and of B, zero past C's
+    // rows/columns and past K
+    auto fill = [&](int kt, int st) {
+        const int64_t k0 = kbeg + (int64_t)kt * BK;
+        const int ilim = rlast + 1, jlim = clast + 1;
+        const int klim = (int)min((int64_t)BK, K - k0);
+        const double *ab = Aslab + (int64_t)kt * BK * lda;
+        const double *bb = Bslab + (int64_t)kt * BK;
+        d2v va[2], vb[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int c = tid + e * NT;
+            const int k = c >> 5, i = 2 * (c & 31);
+            va[e].x = (i < ilim && k < klim) ? ab[i + k * lda] : 0.0;
+            va[e].y = (i + 1 < ilim && k < klim) ? ab[i + 1 + k * lda] : 0.0;
+            const int j = c >> 3, q = (c & 7) ^ (j & 7);
+            vb[e].x = (j < jlim && 2 * q < klim) ? bb[2 * q + j * ldb] : 0.0;
+            vb[e].y = (j < jlim && 2 * q + 1 < klim) ? bb[2 * q + 1 + j * ldb] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int c = tid + e * NT;
+            *reinterpret_cast<d2v *>(&Ls[st][0][2 * c]) = va[e];
+            *reinterpret_cast<d2v *>(&Ls[st][1][2 * c]) = vb[e];
+        }
+    };
+    // fragments that hold some row and column of C (edge tiles skip the others, a test
+    // uniform over the wave): fragment fi holds rows wm·32 + 2i + fi, fj columns
+    // wn·32 + 16fj + j
+    bool frag_ok[2][2];
+#pragma unroll
+    for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+        for (int fj = 0; fj < 2; ++fj) frag_ok[fi][fj] = wm * 32 + fi <= rlast && wn * 32 + fj * 16 <= clast;
+    // one slab from stage st: two rounds of 8 k; in step s of a round, lane group g4 holds
+    // k = 8·round + 2·g4 + s for A and B alike
+    auto mma = [&](int st, auto edge) {
+        constexpr bool EDGE = decltype(edge)::value;
+        const double *As = Ls[st][0], *Bs = Ls[st][1];
+#pragma unroll
+        for (int rd = 0; rd < 2; ++rd) {
+            d2v a[2], b[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                a[s] = *reinterpret_cast<const d2v *>(&As[2 * ((8 * rd + 2 * g4 + s) * 32 + wm * 16 + r16)]);
+#pragma unroll
+            for (int fj = 0; fj < 2; ++fj) {
+                const int j = wn * 32 + fj * 16 + r16, q = 4 * rd + g4;
+                b[fj] = *reinterpret_cast<const d2v *>(&Bs[2 * (8 * j + (q ^ (j & 7)))]);
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+                    for (int fj = 0; fj < 2; ++fj)
+                        if (!EDGE || frag_ok[fi][fj])
+                            acc[fi][fj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][fi], b[fj][s], acc[fi][fj], 0, 0, 0);
+        }
+    };
+    auto run = [&](auto edge) {
+        if (ndirect > 0) issue(0, 0);
+        for (int kt = 0; kt < ndirect; ++kt) {
+            // this wave's loads of slab kt have landed; the barrier then covers every
+            // wave's, and every wave is done reading the stage that slab kt + 1 refills.
+            // A plain barrier: __syncthreads' release fence would also wait for loads
+            // issued later; the waves' LDS reads of the previous slab are complete (their
+            // MFMAs consumed them), and the asm statements keep the compiler from moving
+            // LDS accesses across.
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (kt + 1 < ndirect) issue(kt + 1, (kt + 1) & 1);
+            mma(kt & 1, edge);
+        }
+        for (int kt = ndirect; kt < nslab; ++kt) {
+            __syncthreads();
+            fill(kt, kt & 1);
+            __syncthreads();
+            mma(kt & 1, edge);
+        }
+    };
+    if (m_edge || n_edge)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+
+    // epilogue: fragment fi holds rows wm·32 + 2i + fi, i = fragment row
+    const bool partial = G.splits > 1;
+    double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
+    const int64_t ldc = partial ? m : G.ldc;
+    const double a_ = partial ? 1.0 : alpha;
+    const bool has_beta = !partial && beta != 0.0;
+#pragma unroll
+    for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+        for (int fj = 0; fj < 2; ++fj) {
+            const int64_t col = col0 + wn * 32 + fj * 16 + r16;
+            if (col >= n) continue;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = row0 + wm * 32 + 2 * (g4 + 4 * v) + fi;
+                if (row < m) put_c(Cg + row + col * ldc, a_ * acc[fi][fj][v], beta, has_beta);
+            }
+        }
+}
+
+// --------------------------------------------------------------------------------
 // Persistent FP64 / FP32 kernel.  The grid is the device's workgroup slots (occupancy ×
 // CUs); workgroup w computes the tiles sched[slot_start[w] .. slot_start[w+1]) in order, a
 // longest-processing-time assignment of every tile to a slot made on the host, so the
@@ -1319,6 +1513,9 @@ struct TileSet {
     std::vector<int32_t> sched, slot_start;
     int32_t *d_sched = nullptr, *d_slot_start = nullptr;
     int64_t nslots = 0;
+    // the direct-load FP64 kernel (gemm_k4d_kernel) can run these tiles: every group
+    // untransposed, with slab offsets that fit in 32 bits
+    bool direct_ok = false;
 };
 
 struct SkinnySet {
@@ -1331,21 +1528,27 @@ struct SkinnySet {
 // Tile shapes the autotuner chooses from (BM × BN output tile, BK slab depth; the
 // 3×BF16 kernel always uses BK = 32).
 struct Cfg {
-    int bm, bn, bk, nbuf;  // FP64/FP32 kernel: 2 gemm_k4 (two LDS buffers), 1 gemm_k4s, 3 gemm_k4p, 5 gemm_pks
+    int bm, bn, bk, nbuf;  // FP64/FP32 kernel: 2 gemm_k4 (two LDS buffers), 1 gemm_k4s, 3 gemm_k4p, 5 gemm_pks,
+                           // 6 gemm_k4d (direct global → LDS loads; FP64, untransposed operands)
     int split;             // plan-wide split-K factor: 0 = the schedule model's choice
 };
 // nbuf 5: the persistent kernel with a host-made schedule (gemm_pks_kernel).  Configurations
 // 10–12 are configuration 5 with the split-K factor fixed at 1, 2 and 3: the autotuner
 // measures them against the model's choice (the model prices a split group's partial
-// products and reduction only roughly)
+// products and reduction only roughly).  Configurations 13–16 are the direct-load FP64
+// kernel with the model's split factor and with the factor fixed at 1, 2 and 3; where it
+// does not apply (FP32 or 3×BF16 arithmetic, transposed operands) they run configuration
+// 5's kernel with their split factor.
 constexpr Cfg CFGS[] = {{64, 64, 16, 2, 0},  {128, 128, 16, 2, 0}, {128, 64, 16, 2, 0}, {64, 128, 16, 2, 0},
                         {64, 64, 32, 2, 0},  {64, 64, 16, 1, 0},   {64, 64, 32, 1, 0},  {128, 128, 16, 1, 0},
                         {64, 64, 16, 3, 0},  {64, 64, 16, 5, 0},   {64, 64, 16, 1, 1},  {64, 64, 16, 1, 2},
-                        {64, 64, 16, 1, 3}};
+                        {64, 64, 16, 1, 3},  {64, 64, 16, 6, 0},   {64, 64, 16, 6, 1},  {64, 64, 16, 6, 2},
+                        {64, 64, 16, 6, 3}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
 // of a configuration matter to it, the prefetch depth of configuration 8 and the fixed
 // split factors of configurations 10–12
-constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9; }
+constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9 || c >= 13; }
+constexpr bool direct_cfg(int c) { return CFGS[c].nbuf == 6; }
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 }  // namespace
@@ -1405,6 +1608,9 @@ int kernel_occupancy() {
     else if constexpr (NBUF == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void *>(&gemm_k4_kernel<double, double, BM, BN, BK>), NT, 0);
+    else if constexpr (NBUF == 6)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&gemm_k4d_kernel), NT, 0);
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void *>(&gemm_k4s_kernel<double, double, BM, BN, BK>), NT, 0);
@@ -1433,6 +1639,7 @@ int64_t cfg_slots(int c) {
             case 7: occ = kernel_occupancy<128, 128, 16, 1>(); break;
             case 8: occ = kernel_occupancy<64, 64, 16, 3>(); break;
             case 9: occ = kernel_occupancy<64, 64, 16, 5>(); break;
+            case 13: case 14: case 15: case 16: occ = kernel_occupancy<64, 64, 16, 6>(); break;
             default: occ = kernel_occupancy<64, 64, 16, 1>(); break;
         }
         v = occ * cus;
@@ -1537,6 +1744,12 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     }
     ts.ntiles = start;
     if (ts.groups.empty()) return TKMI_OK;
+    // the direct-load kernel's 32-bit slab offsets: A chunk rows up to 62 + 15·lda, B up to
+    // 14 + 63·ldb elements
+    ts.direct_ok = true;
+    for (const GGroup &g : ts.groups)
+        if (g.ta || g.tb || 8 * (62 + 15 * g.lda) >= ((int64_t)1 << 32) || 8 * (14 + 63 * g.ldb) >= ((int64_t)1 << 32))
+            ts.direct_ok = false;
     int rc = TKMI_OK;
     if (CFGS[c].nbuf == 5) {
         // Workgroup w runs on XCD w % 8 (workgroups are dealt round-robin to the XCDs).
@@ -1645,13 +1858,15 @@ int build_ztiles(tkmi_gemm_plan_s *p) {
     return rc;
 }
 
-int auto_cfg(const tkmi_gemm_plan_s *) {
-    // 64×64×16 with one LDS buffer: the fastest on the sector GEMMs measured (K or N of
-    // a few hundred); the autotuner moves a plan elsewhere when another is > 3% faster
-    return 5;
+int auto_cfg(const tkmi_gemm_plan_s *p, int prec) {
+    // FP64 with untransposed operands: the direct-load kernel with the model's split factor
+    // (8–10% faster than the one-buffer kernel on the sector GEMMs measured).  Otherwise
+    // 64×64×16 with one LDS buffer: the fastest on the sector GEMMs measured (K or N of a
+    // few hundred).  The autotuner moves a plan elsewhere when another is > 3% faster.
+    return (prec == TKMI_PREC_F64 && p->ts[13].direct_ok) ? 13 : 5;
 }
 
-int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p); }
+int cfg_of(const tkmi_gemm_plan_s *p, int prec) { return p->choice[prec] >= 0 ? p->choice[prec] : auto_cfg(p, prec); }
 
 // A kernel launch on s; with timing events (tkmi_gemm_plan_time_next) the launch records
 // its own start and end in them.
@@ -1669,7 +1884,19 @@ void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha,
                const hipEvent_t *ev) {
     const dim3 grid((unsigned)ts.ntiles), block(NT);
     const int64_t ng = (int64_t)ts.groups.size();
-    if constexpr (NBUF == 5)
+    if constexpr (NBUF == 6) {
+        // the direct-load kernel: FP64 arithmetic on FP64 data with untransposed operands;
+        // otherwise the one-buffer kernel of the same tiles
+        if constexpr (std::is_same<TC, double>::value && std::is_same<TD, double>::value) {
+            if (ts.direct_ok) {
+                launch_timed(gemm_k4d_kernel, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws, alpha,
+                             beta);
+                return;
+            }
+        }
+        launch_timed(gemm_k4s_kernel<TC, TD, BM, BN, BK>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C,
+                     ts.ws, alpha, beta);
+    } else if constexpr (NBUF == 5)
         launch_timed(gemm_pks_kernel<TC, TD, BM, BN, BK>, dim3((unsigned)ts.nslots), block, s, ev, ts.d_groups, ng,
                      ts.d_sched, ts.d_slot_start, A, B, C, ts.ws, alpha, beta);
     else if constexpr (NBUF == 3)
@@ -1759,6 +1986,9 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const TD *A, const TD *B, T
         case 7: launch_k<TD, 128, 128, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         case 8: launch_k<TD, 64, 64, 16, 3>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         case 9: launch_k<TD, 64, 64, 16, 5>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 13: case 14: case 15: case 16:
+            launch_k<TD, 64, 64, 16, 6>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce);
+            break;
         default: launch_k<TD, 64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
     }
     TKMI_HIP(hipGetLastError());
@@ -1908,6 +2138,10 @@ int autotune(tkmi_gemm_plan plan, int32_t precision, const TD *A, const TD *B, T
         for (int c = 0; c < NCFG && rc == TKMI_OK; ++c) {
             if (plan->ts[c].ntiles == 0) continue;
             if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
+            // the direct-load configurations run configuration 5's kernel where they do not apply
+            if (direct_cfg(c) && (precision != TKMI_PREC_F64 || !std::is_same<TD, double>::value ||
+                                  !plan->ts[c].direct_ok))
+                continue;
             if ((rc = launch_cfg<TD>(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
             if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }
             for (int r = 0; r < reps && rc == TKMI_OK; ++r) rc = launch_cfg<TD>(plan, c, precision, A, B, C, 1.0, 0.0, s);
@@ -1918,7 +2152,7 @@ int autotune(tkmi_gemm_plan plan, int32_t precision, const TD *A, const TD *B, T
             tbest[c] = std::min(tbest[c], ms);
         }
     }
-    const int d = auto_cfg(plan);
+    const int d = auto_cfg(plan, precision);
     int best_c = d;
     for (int c = 0; c < NCFG; ++c)
         if (tbest[c] < tbest[best_c]) best_c = c;

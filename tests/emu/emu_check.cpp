@@ -355,10 +355,16 @@ struct GSpec {
 static void gemm_cases() {
     // every kind of group: ragged MFMA tiles, skinny (both orientations), k = 0, a dot
     // product, and one long-K group that the schedule model splits
-    const int64_t shapes[][3] = {{1, 1, 1},  {16, 16, 4}, {29, 24, 5}, {65, 63, 17}, {5, 70, 7},
-                                 {70, 4, 4}, {17, 33, 0}, {1, 1, 300}, {40, 8, 30},  {20, 20, 520}};
-    const int ng = sizeof(shapes) / sizeof(shapes[0]);
+    // (the last three, untransposed only, for the direct-load kernel: whole 64×64 tiles with
+    // a partial last slab; long enough to be split; and, last in A with lda = m, a tile
+    // past m whose K is whole slabs, so that a read of row m in the group's last column
+    // would leave the buffer)
+    const int64_t shapes[][3] = {{1, 1, 1},   {16, 16, 4},  {29, 24, 5},   {65, 63, 17},  {5, 70, 7},
+                                 {70, 4, 4},  {17, 33, 0},  {1, 1, 300},   {40, 8, 30},   {20, 20, 520},
+                                 {130, 70, 45}, {64, 64, 600}, {70, 40, 32}};
     for (int variant = 0; variant < 4; ++variant) {
+        const int nshapes = (int)(sizeof(shapes) / sizeof(shapes[0]));
+        const int ng = nshapes - (variant == 0 ? 0 : 3);
         std::vector<GSpec> gs;
         int64_t na = 0, nb = 0, nc = 0;
         for (int g = 0; g < ng; ++g) {
@@ -368,7 +374,7 @@ static void gemm_cases() {
             G.k = shapes[g][2];
             G.ta = variant & 1;
             G.tb = (variant >> 1) & 1;
-            G.lda = (G.ta ? G.k : G.m) + irand(0, 2);
+            G.lda = (G.ta ? G.k : G.m) + (g == nshapes - 1 ? 0 : irand(0, 2));
             G.ldb = (G.tb ? G.n : G.k) + irand(0, 2);
             G.ldc = G.m + irand(0, 2);
             G.lda = std::max<int64_t>(G.lda, 1);
@@ -420,7 +426,9 @@ static void gemm_cases() {
         };
         // real data: every configuration in FP64 (each kernel template once per transpose
         // case), the FP32 and 3×BF16 forms on the configurations with distinct kernels
-        for (int cfg = 0; cfg < 13; ++cfg) {
+        int ncfg = 0;
+        for (int32_t bm, bn, bk, nbuf; tkmi_gemm_config_info(ncfg, &bm, &bn, &bk, &nbuf) == TKMI_OK;) ++ncfg;
+        for (int cfg = 0; cfg < ncfg; ++cfg) {
             for (int prec = 0; prec < 3; ++prec) {
                 const bool run = variant == 0 || cfg == 5 || cfg == 8 || cfg == 9 || (cfg == 1 && prec == 2);
                 if (!run) continue;

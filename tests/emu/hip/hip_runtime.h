@@ -148,6 +148,19 @@ struct one {
 };
 }  // namespace emu
 
+// global → LDS loads (global_load_lds_dwordx4): lane l's `size` bytes land at the wave's
+// LDS pointer + offset + l·size, at once (so the waits before reading them are no-ops);
+// the address-space qualifier of the LDS pointer has no host meaning
+namespace emu {
+inline void global_load_lds(const void *src, void *lds, unsigned size, unsigned offset) {
+    std::memcpy(static_cast<char *>(lds) + offset + (threadIdx.x & 63) * size, src, size);
+}
+}  // namespace emu
+#define address_space(n)
+#define __builtin_amdgcn_global_load_lds(src, lds, size, offset, aux) ::emu::global_load_lds(src, lds, size, offset)
+#define __builtin_amdgcn_s_waitcnt(x) ((void)0)
+#define __builtin_amdgcn_s_barrier() ::emu::syncthreads()
+
 #define __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, x, y, z) \
     ::emu::mfma<1, true>(::emu::one<double>{a}, ::emu::one<double>{b}, c)
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) \

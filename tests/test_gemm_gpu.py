@@ -85,7 +85,10 @@ SHAPES = [(1, 1, 1), (16, 16, 4), (29, 24, 5), (64, 64, 16), (65, 63, 17), (130,
           (5, 300, 7), (300, 4, 4), (17, 33, 0), (0, 5, 3)]
 
 
-@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012])
+ALL_TILES = [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012, 1013, 1014, 1015, 1016]
+
+
+@pytest.mark.parametrize("tile", ALL_TILES)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_f64_exact_integer(lib, ta, tb, tile):
     rng = np.random.default_rng(10 * ta + tb)
@@ -97,6 +100,34 @@ def test_f64_exact_integer(lib, ta, tb, tile):
         out = _run(lib, tab, A, B, Cm, 0, alpha, beta, tile)
         ref = _ref(tab, A, B, Cm, alpha, beta)
         np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("tile", [1013, 1014, 1015, 1016])
+def test_direct_load_kernel_exact(lib, tile):
+    """The direct global → LDS kernel (configurations 13–16; every group untransposed):
+    whole 64×64 tiles with one, two and many slabs, partial last slabs, ragged edge tiles,
+    long K split (13: the model's factor, 15, 16: two and three), odd leading dimensions
+    and offsets (16-byte chunks only 8-byte aligned) — exact on integer data for β = 0, −1
+    and 1, and equal bit for bit on random data to the one-buffer kernel's result at the
+    same split factor (configurations 10–12)."""
+    rng = np.random.default_rng(40 + tile)
+    shapes = [(64, 64, 16), (128, 64, 32), (192, 130, 290), (64, 64, 1000), (70, 200, 45), (65, 65, 17),
+              (256, 128, 2050), (33, 300, 600)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0] * len(shapes), [0] * len(shapes))
+    assert any(r[4] % 2 for r in tab) or any(r[3] % 2 for r in tab)
+    A = rng.integers(-8, 9, na).astype(np.float64)
+    B = rng.integers(-8, 9, nb).astype(np.float64)
+    Cm = rng.integers(-8, 9, nc).astype(np.float64)
+    for alpha, beta in ((1.0, 0.0), (2.0, -1.0), (1.0, 1.0)):
+        np.testing.assert_array_equal(_run(lib, tab, A, B, Cm, 0, alpha, beta, tile), _ref(tab, A, B, Cm, alpha, beta))
+    A, B, Cm = rng.standard_normal(na), rng.standard_normal(nb), rng.standard_normal(nc)
+    out = _run(lib, tab, A, B, Cm, 0, 0.75, 0.5, tile)
+    ref = _ref(tab, A, B, Cm, 0.75, 0.5)
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+    # the one-buffer kernel with the same split factor forms the same sums in the same k order
+    if tile != 1013:  # configuration 13 takes the model's split factor, 14–16 fix it at 1–3
+        same = {1014: 1010, 1015: 1011, 1016: 1012}[tile]
+        np.testing.assert_array_equal(out, _run(lib, tab, A, B, Cm, 0, 0.75, 0.5, same))
 
 
 def test_f64_random(lib):
@@ -123,7 +154,7 @@ def test_reduced_precision_forms(lib, prec, tol):
         assert err <= tol, (prec, tile, err)
 
 
-@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012])
+@pytest.mark.parametrize("tile", ALL_TILES)
 def test_split_k_exact(lib, tile):
     """Few output tiles and long K engage split-K (deterministic workspace
     reduction); integer data keeps the check exact."""
