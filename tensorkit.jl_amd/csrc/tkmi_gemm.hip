@@ -571,24 +571,39 @@ __global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restric
     else
         run(std::false_type{});
 
-    // epilogue: fragment fi holds rows wm·32 + 2i + fi, i = fragment row
+    // epilogue: fragment fi holds rows wm·32 + 2i + fi, i = fragment row, so a lane holds
+    // the two adjacent rows (2i, 2i + 1) of its column in acc[0] and acc[1]: one 16-byte
+    // store writes both, and the 16 lanes of a row group cover 8 consecutive rows of 16
+    // columns (whole 64-byte segments rather than every other row)
     const bool partial = G.splits > 1;
     double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
     const int64_t ldc = partial ? m : G.ldc;
     const double a_ = partial ? 1.0 : alpha;
     const bool has_beta = !partial && beta != 0.0;
 #pragma unroll
-    for (int fi = 0; fi < 2; ++fi)
+    for (int fj = 0; fj < 2; ++fj) {
+        const int64_t col = col0 + wn * 32 + fj * 16 + r16;
+        if (col >= n) continue;
 #pragma unroll
-        for (int fj = 0; fj < 2; ++fj) {
-            const int64_t col = col0 + wn * 32 + fj * 16 + r16;
-            if (col >= n) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t row = row0 + wm * 32 + 2 * (g4 + 4 * v) + fi;
-                if (row < m) put_c(Cg + row + col * ldc, a_ * acc[fi][fj][v], beta, has_beta);
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = row0 + wm * 32 + 2 * (g4 + 4 * v);
+            double *p = Cg + row + col * ldc;
+            const double r0 = a_ * acc[0][fj][v], r1 = a_ * acc[1][fj][v];
+            if (row + 1 < m) {
+                d2v o;
+                if (has_beta) {
+                    d2v c;
+                    __builtin_memcpy(&c, p, sizeof(c));
+                    o = d2v{r0 + beta * c.x, r1 + beta * c.y};
+                } else {
+                    o = d2v{r0, r1};
+                }
+                __builtin_memcpy(p, &o, sizeof(o));
+            } else if (row < m) {
+                put_c(p, r0, beta, has_beta);
             }
         }
+    }
 }
 
 // --------------------------------------------------------------------------------
