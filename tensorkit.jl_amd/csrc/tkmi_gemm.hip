@@ -953,7 +953,10 @@ template <typename TD>
 __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restrict__ pg, int64_t ng,
                                                         const TD *__restrict__ A, const TD *__restrict__ Bm,
                                                         __bf16 *__restrict__ P, int which) {
-    __shared__ double tile[PACK_T][PACK_T + 1];
+    // staged in the element type (a float splits exactly in float arithmetic too): half
+    // the LDS for FP32 data, twice the workgroups per CU
+    typedef typename std::conditional<std::is_same<TD, float>::value, float, double>::type TS;
+    __shared__ TS tile[PACK_T][PACK_T + 1];
     const int64_t q = blockIdx.x;
     auto first = [&](const PackGroup &g) { return which == 0 ? g.start : (which == 1 ? g.startA : g.startB); };
     int64_t lo = 0, hi = ng - 1;
@@ -981,14 +984,14 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
         const int64_t gi = i0 + ii;
         for (int kk = t / PACK_T; kk < PACK_T; kk += NT / PACK_T) {
             const int64_t gk = k0 + kk;
-            tile[ii][kk] = (gi < lim && gk < G.k) ? (double)src[gi + gk * ld] : 0.0;
+            tile[ii][kk] = (gi < lim && gk < G.k) ? (TS)src[gi + gk * ld] : (TS)0;
         }
     } else {
         const int kk = t % PACK_T;
         const int64_t gk = k0 + kk;
         for (int ii = t / PACK_T; ii < PACK_T; ii += NT / PACK_T) {
             const int64_t gi = i0 + ii;
-            tile[ii][kk] = (gi < lim && gk < G.k) ? (double)src[gk + gi * ld] : 0.0;
+            tile[ii][kk] = (gi < lim && gk < G.k) ? (TS)src[gk + gi * ld] : (TS)0;
         }
     }
     __syncthreads();
@@ -1002,7 +1005,7 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
         __bf16 x[3][8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const double v = tile[ii][8 * o + e];
+            const double v = (double)tile[ii][8 * o + e];
             const __bf16 x0 = to_bf16((float)v);
             const double r1 = v - (double)(float)x0;
             const __bf16 x1 = to_bf16((float)r1);
