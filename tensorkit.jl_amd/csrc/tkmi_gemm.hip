@@ -1024,6 +1024,148 @@ __global__ __launch_bounds__(NT) void split_pack_kernel(const PackGroup *__restr
     }
 }
 
+// 3×BF16 with direct global → LDS loads (64×64 tiles): the split panels are padded (rows to
+// 128, K to 32, zero filled) and 16-byte aligned, so every slab loads whole with
+// global_load_lds_dwordx4, no bounds test and no VGPR staging, into two LDS stages (one
+// barrier per 32-deep slab).  A stage holds per operand and plane the 64 rows × 4 k
+// octets of the slab, octet g of row r at slot 4r + (g ^ ((r / 4) mod 4)): a row's 64
+// bytes stay contiguous for the fetch, and the 16 lanes that read one octet of 16 rows hit
+// distinct banks.  The products, their order and the epilogue are gemm_bf16x3_kernel's.
+template <typename TD>
+__global__ __launch_bounds__(NT) void gemm_bf16x3d_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                          int64_t ntiles, const __bf16 *__restrict__ P,
+                                                          TD *__restrict__ C, double *__restrict__ ws,
+                                                          double alpha, double beta) {
+    constexpr int BM = 64, BN = 64, BK = 32, FM = 2, FN = 2;
+    constexpr int SL = BM * (BK / 8);  // 16-byte slots per operand plane (256)
+    __shared__ __attribute__((aligned(16))) bf8_t Ls[2][2][3][SL];
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t tm = lmn / G.tiles_n, tn = lmn % G.tiles_n;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const int64_t kp = G.kp;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    d4_t acc64[FM][FN];
+    f4_t lo[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            acc64[i][j] = d4_t{0, 0, 0, 0};
+            lo[i][j] = f4_t{0, 0, 0, 0};
+        }
+
+    // slab loads: the 3 planes × 256 slots of A, then of B, are 24 runs of 64 slots; wave
+    // w issues runs 6w .. 6w + 5.  Slot s of a plane holds row s / 4, octet
+    // (s % 4) ^ ((s / 16) % 4).
+    const __bf16 *Ap = P + G.pa + row0 * kp + kbeg;
+    const __bf16 *Bp = P + G.pb + col0 * kp + kbeg;
+    uint32_t off[6];  // byte offset of this lane's chunk from the slab base of its run
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int run = 6 * wave + e, opnd = run / 12, pl = (run / 4) % 3;
+        const int sl = 64 * (run % 4) + lane, r = sl >> 2, g = (sl & 3) ^ ((sl >> 4) & 3);
+        off[e] = (uint32_t)(2 * (pl * (opnd ? G.sb : G.sa) + r * kp + 8 * g));
+    }
+    auto issue = [&](int64_t kt, int st) {
+        const char *ab = reinterpret_cast<const char *>(Ap + kt * BK);
+        const char *bb = reinterpret_cast<const char *>(Bp + kt * BK);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            const int run = 6 * wave + e, opnd = run / 12, pl = (run / 4) % 3;
+            __builtin_amdgcn_global_load_lds((const void *)((opnd ? bb : ab) + off[e]),
+                                             (__attribute__((address_space(3))) void *)&Ls[st][opnd][pl][64 * (run % 4)],
+                                             16, 0, 0);
+        }
+    };
+
+    bool frag_ok[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) frag_ok[i][j] = row0 + wm * 32 + i * 16 < m && col0 + wn * 32 + j * 16 < n;
+
+    const int64_t nk = (K - kbeg + BK - 1) / BK;
+    const int g4 = lane >> 4;
+    if (nk > 0) issue(0, 0);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int st = (int)(kt & 1);
+        // this wave's loads of slab kt have landed; the barrier covers every wave's and
+        // every wave's reads of the stage that slab kt + 1 refills (a plain barrier:
+        // __syncthreads' release fence would also wait for the loads issued next)
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (kt + 1 < nk) issue(kt + 1, st ^ 1);
+        bf8_t a[3][FM], b[3][FN];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int r = wm * 32 + i * 16 + (lane & 15);
+                a[p][i] = Ls[st][0][p][4 * r + (g4 ^ ((r >> 2) & 3))];
+            }
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int r = wn * 32 + j * 16 + (lane & 15);
+                b[p][j] = Ls[st][1][p][4 * r + (g4 ^ ((r >> 2) & 3))];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                if (!frag_ok[i][j]) continue;
+                f4_t h = f4_t{0, 0, 0, 0};
+                h = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], h, 0, 0, 0);
+                f4_t l = lo[i][j];
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], l, 0, 0, 0);
+                l = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], l, 0, 0, 0);
+                lo[i][j] = l;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)h[v];
+            }
+    }
+
+    auto emit = [&](auto *Cg, int64_t ldc, double a_, bool has_beta) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int64_t col = col0 + wn * 32 + j * 16 + (lane & 15);
+                if (col >= n) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int64_t row = row0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+                    if (row < m)
+                        put_c(Cg + row + col * ldc, a_ * (acc64[i][j][v] + (double)lo[i][j][v]), beta, has_beta);
+                }
+            }
+    };
+    const bool partial = G.splits > 1;
+    if constexpr (std::is_same<TD, double>::value)
+        emit(partial ? ws + G.ws_off + z * m * n : C + G.c_off, partial ? m : G.ldc, partial ? 1.0 : alpha,
+             !partial && beta != 0.0);
+    else if (partial)
+        emit(ws + G.ws_off + z * m * n, m, 1.0, false);
+    else
+        emit(C + G.c_off, G.ldc, alpha, beta != 0.0);
+}
+
 // STAGES: register stages of prefetched slabs (1: the next slab loads while the current
 // one is multiplied; 2: loads run two slabs ahead, for the short 32-deep bf16 slabs)
 template <typename TD, int BM, int BN, int STAGES>
@@ -1534,6 +1676,9 @@ struct TileSet {
     // the direct-load FP64 kernel (gemm_k4d_kernel) can run these tiles: every group
     // untransposed, with slab offsets that fit in 32 bits
     bool direct_ok = false;
+    // the direct-load 3×BF16 kernel (gemm_bf16x3d_kernel): panel offsets inside a slab fit
+    // in 32 bits
+    bool bf16_direct_ok = false;
 };
 
 struct SkinnySet {
@@ -1563,9 +1708,10 @@ constexpr Cfg CFGS[] = {{64, 64, 16, 2, 0},  {128, 128, 16, 2, 0}, {128, 64, 16,
                         {64, 64, 16, 1, 3},  {64, 64, 16, 6, 0},   {64, 64, 16, 6, 1},  {64, 64, 16, 6, 2},
                         {64, 64, 16, 6, 3}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
-// of a configuration matter to it, the prefetch depth of configuration 8 and the fixed
-// split factors of configurations 10–12
-constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9 || c >= 13; }
+// of a configuration matter to it, the prefetch depth of configuration 8, the fixed
+// split factors of configurations 10–12, and configurations 13–16, which run the
+// direct-load 3×BF16 kernel
+constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9; }
 constexpr bool direct_cfg(int c) { return CFGS[c].nbuf == 6; }
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
@@ -1768,6 +1914,10 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     for (const GGroup &g : ts.groups)
         if (g.ta || g.tb || 8 * (62 + 15 * g.lda) >= ((int64_t)1 << 32) || 8 * (14 + 63 * g.ldb) >= ((int64_t)1 << 32))
             ts.direct_ok = false;
+    // 3×BF16 panel offsets: up to two plane strides, 63 rows and 3 octets, in bytes
+    ts.bf16_direct_ok = true;
+    for (const GGroup &g : ts.groups)
+        if (2 * (2 * std::max(g.sa, g.sb) + 63 * g.kp + 24) >= ((int64_t)1 << 32)) ts.bf16_direct_ok = false;
     int rc = TKMI_OK;
     if (CFGS[c].nbuf == 5) {
         // Workgroup w runs on XCD w % 8 (workgroups are dealt round-robin to the XCDs).
@@ -1965,6 +2115,9 @@ void launch_k(tkmi_gemm_plan_s *p, const TileSet &ts, int prec, const TD *A, con
         if constexpr (NBUF == 3)
             launch_timed(gemm_bf16x3_kernel<TD, BM, BN, 2>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles,
                          (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
+        else if (NBUF == 6 && BM == 64 && BN == 64 && ts.bf16_direct_ok)
+            launch_timed(gemm_bf16x3d_kernel<TD>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles,
+                         (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
         else
             launch_timed(gemm_bf16x3_kernel<TD, BM, BN, 1>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles,
                          (const __bf16 *)p->panels, C, ts.ws, alpha, beta);
@@ -2157,8 +2310,10 @@ int autotune(tkmi_gemm_plan plan, int32_t precision, const TD *A, const TD *B, T
             if (plan->ts[c].ntiles == 0) continue;
             if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
             // the direct-load configurations run configuration 5's kernel where they do not apply
-            if (direct_cfg(c) && (precision != TKMI_PREC_F64 || !std::is_same<TD, double>::value ||
-                                  !plan->ts[c].direct_ok))
+            if (direct_cfg(c) &&
+                (precision == TKMI_PREC_F32 ||
+                 (precision == TKMI_PREC_F64 && (!std::is_same<TD, double>::value || !plan->ts[c].direct_ok)) ||
+                 (precision == TKMI_PREC_BF16X3 && !plan->ts[c].bf16_direct_ok)))
                 continue;
             if ((rc = launch_cfg<TD>(plan, c, precision, A, B, C, 1.0, 0.0, s))) break;
             if (hipEventRecord(e0, s) != hipSuccess) { rc = TKMI_ERR_HIP; break; }

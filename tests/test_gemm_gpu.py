@@ -109,7 +109,9 @@ def test_direct_load_kernel_exact(lib, tile):
     long K split (13: the model's factor, 15, 16: two and three), odd leading dimensions
     and offsets (16-byte chunks only 8-byte aligned) — exact on integer data for β = 0, −1
     and 1, and equal bit for bit on random data to the one-buffer kernel's result at the
-    same split factor (configurations 10–12)."""
+    same split factor (configurations 10–12); the same for the direct-load 3×BF16 kernel,
+    which these configurations run in that form (transposes included: its panels are
+    always untransposed)."""
     rng = np.random.default_rng(40 + tile)
     shapes = [(64, 64, 16), (128, 64, 32), (192, 130, 290), (64, 64, 1000), (70, 200, 45), (65, 65, 17),
               (256, 128, 2050), (33, 300, 600)]
@@ -128,6 +130,12 @@ def test_direct_load_kernel_exact(lib, tile):
     if tile != 1013:  # configuration 13 takes the model's split factor, 14–16 fix it at 1–3
         same = {1014: 1010, 1015: 1011, 1016: 1012}[tile]
         np.testing.assert_array_equal(out, _run(lib, tab, A, B, Cm, 0, 0.75, 0.5, same))
+        # 3×BF16: the direct-load kernel (configurations 13–16) forms the same six products
+        # per slab in the same order as the register-staged one (5, 10–12)
+        np.testing.assert_array_equal(_run(lib, tab, A, B, Cm, 2, 0.75, 0.5, tile),
+                                      _run(lib, tab, A, B, Cm, 2, 0.75, 0.5, same))
+    out = _run(lib, tab, A, B, Cm, 2, 0.75, 0.5, tile)
+    assert np.linalg.norm(out - ref) <= 1e-6 * np.linalg.norm(ref)
 
 
 def test_f64_random(lib):
