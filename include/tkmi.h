@@ -201,9 +201,10 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
  * two LDS stages, FP64 arithmetic with untransposed A and B only).  Configurations 0–9
  * and 13 take the split-K factor of the schedule model; 10, 11 and 12 are configuration 5
  * with the factor fixed at 1, 2 and 3, and 14, 15 and 16 configuration 13 likewise.
- * Where the direct-load kernel does not apply (FP32 or 3×BF16 arithmetic, FP32 data, a
- * transposed operand) configurations 13–16 run configuration 5's kernel with their own
- * split factor (tkmi_gemm_plan_splitk reports the factor of each).               */
+ * In the FP32 form on FP32 data they run gemm_k4d32_kernel (the same direct loads), in
+ * the 3×BF16 form gemm_bf16x3d_kernel; where no direct-load kernel applies (a transposed
+ * operand, FP32 arithmetic on FP64 data) configurations 13–16 run configuration 5's
+ * kernel with their own split factor (tkmi_gemm_plan_splitk reports the factor of each). */
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
  * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64) with their split-K

@@ -607,6 +607,145 @@ __global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restric
 }
 
 // --------------------------------------------------------------------------------
+// FP32 kernel with direct global → LDS loads (FP32 data, A and B untransposed): the
+// FP32 counterpart of gemm_k4d_kernel.  A 16-byte chunk holds 4 floats, so a wave takes a
+// 64×16 strip of the 64×64 tile (the four waves side by side along the columns): its four
+// 16-row fragments take the rows ≡ 0, 1, 2, 3 mod 4, and one chunk read (rows 4i … 4i + 3
+// of one k) gives a lane its A value for all four; one B chunk (k … k + 3 of one column)
+// gives it the four MFMA steps of a 16-deep slab.
+//  A: chunk (k, p) = rows 4p … 4p+3 of column k, at slot 16k + p;
+//  B: chunk (q, j) = k 4q … 4q+3 of column j, at slot 4j + (q ^ ((j / 4) mod 4)).
+// Two LDS stages of 8 KB, one barrier per slab.  Tiles past n load clamped columns;
+// tiles past m, and a split's partial last slab, go through registers with zero fill.
+// The MFMA steps are the one-buffer FP32 kernel's (the same sums in the same k order).
+__global__ __launch_bounds__(NT) void gemm_k4d32_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                        int64_t ntiles, const float *__restrict__ A,
+                                                        const float *__restrict__ Bm, float *__restrict__ C,
+                                                        double *__restrict__ ws, double alpha, double beta) {
+    constexpr int BM = 64, BN = 64, BK = 16, CH = 256;  // CH: 16-byte chunks per operand slab
+    __shared__ __attribute__((aligned(16))) f4_t Ls[2][2][CH];
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t tm = lmn / G.tiles_n, tn = lmn % G.tiles_n;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const int64_t lda = G.lda, ldb = G.ldb;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const bool m_edge = row0 + BM > m;
+    const int nfull = (int)((K - kbeg) / BK);
+    const int nslab = nfull + ((K - kbeg) % BK != 0 ? 1 : 0);
+    const int ndirect = m_edge ? 0 : nfull;
+    const float *__restrict__ Aslab = A + G.a_off + row0 + kbeg * lda;
+    const float *__restrict__ Bslab = Bm + G.b_off + col0 * ldb + kbeg;
+    const int rlast = (int)min((int64_t)(BM - 1), m - 1 - row0), clast = (int)min((int64_t)(BN - 1), n - 1 - col0);
+
+    f4_t acc[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) acc[f] = f4_t{0, 0, 0, 0};
+
+    // direct path: wave w issues chunks [64w, 64w + 64) of A and of B
+    const int c = 64 * wave + lane;
+    const uint32_t aoff = (uint32_t)(4 * (4 * (c & 15) + (c >> 4) * lda));
+    const int bj = c >> 2, bq = (c & 3) ^ ((bj >> 2) & 3);
+    const uint32_t boff = (uint32_t)(4 * (4 * bq + min(bj, clast) * ldb));
+    auto issue = [&](int kt, int st) {
+        const char *ab = reinterpret_cast<const char *>(Aslab + (int64_t)kt * BK * lda);
+        const char *bb = reinterpret_cast<const char *>(Bslab + (int64_t)kt * BK);
+        __builtin_amdgcn_global_load_lds((const void *)(ab + aoff),
+                                         (__attribute__((address_space(3))) void *)&Ls[st][0][64 * wave], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)(bb + boff),
+                                         (__attribute__((address_space(3))) void *)&Ls[st][1][64 * wave], 16, 0, 0);
+    };
+    // register path: thread t fills chunk t of A and of B, zero past C and past K
+    auto fill = [&](int kt, int st) {
+        const int64_t k0 = kbeg + (int64_t)kt * BK;
+        const int klim = (int)min((int64_t)BK, K - k0);
+        const float *ab = Aslab + (int64_t)kt * BK * lda;
+        const float *bb = Bslab + (int64_t)kt * BK;
+        const int k = tid >> 4, i = 4 * (tid & 15);
+        f4_t va, vb;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) va[e] = (i + e <= rlast && k < klim) ? ab[i + e + k * lda] : 0.0f;
+        const int j = tid >> 2, q = (tid & 3) ^ ((j >> 2) & 3);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vb[e] = (j <= clast && 4 * q + e < klim) ? bb[4 * q + e + j * ldb] : 0.0f;
+        Ls[st][0][tid] = va;
+        Ls[st][1][tid] = vb;
+    };
+    // one slab: step s of the MFMA sums k = 4·g4 + s for A and B alike
+    auto mma = [&](int st) {
+        const int j = 16 * wave + r16;
+        const f4_t b = Ls[st][1][4 * j + (g4 ^ ((j >> 2) & 3))];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const f4_t a = Ls[st][0][16 * (4 * g4 + s) + r16];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[f], b[s], acc[f], 0, 0, 0);
+        }
+    };
+    if (ndirect > 0) issue(0, 0);
+    for (int kt = 0; kt < ndirect; ++kt) {
+        // as in gemm_k4d_kernel: this wave's loads landed, then a plain barrier
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (kt + 1 < ndirect) issue(kt + 1, (kt + 1) & 1);
+        mma(kt & 1);
+    }
+    for (int kt = ndirect; kt < nslab; ++kt) {
+        __syncthreads();
+        fill(kt, kt & 1);
+        __syncthreads();
+        mma(kt & 1);
+    }
+
+    // epilogue: fragment f holds rows 4i + f, and the lane's value v of every fragment is
+    // fragment row 4·g4 + v, so the four fragments give rows 16·g4 + 4v … + 3 of its
+    // column: one 16-byte store (FP32 C) or four partial sums (FP64 workspace)
+    const bool partial = G.splits > 1;
+    const int64_t col = col0 + 16 * wave + r16;
+    if (col >= n) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int64_t row = row0 + 16 * g4 + 4 * v;
+        if (row >= m) continue;
+        if (partial) {
+            double *p = ws + G.ws_off + z * m * n + row + col * m;
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+                if (row + f < m) p[f] = (double)acc[f][v];
+        } else {
+            float *p = C + G.c_off + row + col * G.ldc;
+            const bool has_beta = beta != 0.0;
+            if (row + 3 < m) {
+                f4_t o;
+                f4_t cv = f4_t{0, 0, 0, 0};
+                if (has_beta) __builtin_memcpy(&cv, p, sizeof(cv));
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    const double r = alpha * (double)acc[f][v];
+                    o[f] = (float)(has_beta ? r + beta * (double)cv[f] : r);
+                }
+                __builtin_memcpy(p, &o, sizeof(o));
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+                    if (row + f < m) put_c(p + f, alpha * (double)acc[f][v], beta, has_beta);
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------------
 // Persistent FP64 / FP32 kernel.  The grid is the device's workgroup slots (occupancy ×
 // CUs); workgroup w computes the tiles sched[slot_start[w] .. slot_start[w+1]) in order, a
 // longest-processing-time assignment of every tile to a slot made on the host, so the
@@ -1698,10 +1837,10 @@ struct Cfg {
 // nbuf 5: the persistent kernel with a host-made schedule (gemm_pks_kernel).  Configurations
 // 10–12 are configuration 5 with the split-K factor fixed at 1, 2 and 3: the autotuner
 // measures them against the model's choice (the model prices a split group's partial
-// products and reduction only roughly).  Configurations 13–16 are the direct-load FP64
-// kernel with the model's split factor and with the factor fixed at 1, 2 and 3; where it
-// does not apply (FP32 or 3×BF16 arithmetic, transposed operands) they run configuration
-// 5's kernel with their split factor.
+// products and reduction only roughly).  Configurations 13–16 are the direct-load
+// kernels (FP64; FP32 on FP32 data; 3×BF16) with the model's split factor and with the
+// factor fixed at 1, 2 and 3; where they do not apply (transposed operands, FP32
+// arithmetic on FP64 data) they run configuration 5's kernel with their split factor.
 constexpr Cfg CFGS[] = {{64, 64, 16, 2, 0},  {128, 128, 16, 2, 0}, {128, 64, 16, 2, 0}, {64, 128, 16, 2, 0},
                         {64, 64, 32, 2, 0},  {64, 64, 16, 1, 0},   {64, 64, 32, 1, 0},  {128, 128, 16, 1, 0},
                         {64, 64, 16, 3, 0},  {64, 64, 16, 5, 0},   {64, 64, 16, 1, 1},  {64, 64, 16, 1, 2},
@@ -2062,6 +2201,13 @@ void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha,
                 return;
             }
         }
+        if constexpr (std::is_same<TC, float>::value && std::is_same<TD, float>::value) {
+            if (ts.direct_ok) {  // (its 32-bit offset bounds imply the FP32 kernel's)
+                launch_timed(gemm_k4d32_kernel, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws,
+                             alpha, beta);
+                return;
+            }
+        }
         launch_timed(gemm_k4s_kernel<TC, TD, BM, BN, BK>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C,
                      ts.ws, alpha, beta);
     } else if constexpr (NBUF == 5)
@@ -2311,7 +2457,7 @@ int autotune(tkmi_gemm_plan plan, int32_t precision, const TD *A, const TD *B, T
             if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
             // the direct-load configurations run configuration 5's kernel where they do not apply
             if (direct_cfg(c) &&
-                (precision == TKMI_PREC_F32 ||
+                ((precision == TKMI_PREC_F32 && (!std::is_same<TD, float>::value || !plan->ts[c].direct_ok)) ||
                  (precision == TKMI_PREC_F64 && (!std::is_same<TD, double>::value || !plan->ts[c].direct_ok)) ||
                  (precision == TKMI_PREC_BF16X3 && !plan->ts[c].bf16_direct_ok)))
                 continue;

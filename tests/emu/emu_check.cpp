@@ -440,7 +440,7 @@ static void gemm_cases() {
                                     (double *)C.dev, nullptr);
                 CHECK(rc == TKMI_OK, "exec rc=%d", rc);
                 compare(C, reference(A, B, C, 2.0, -1.0, 0, 0), "f64 data", cfg, prec);
-                if (prec > 0 && (cfg == 5 || cfg == 8 || cfg == 2)) {
+                if (prec > 0 && (cfg == 5 || cfg == 8 || cfg == 2 || cfg >= 13)) {
                     Buf A32(TKMI_F32, na, true), B32(TKMI_F32, nb, true), C32(TKMI_F32, nc, true);
                     rc = tkmi_gemm_exec_dtype(plan, prec, TKMI_F32, 3, 1.0, 1.0, A32.dev, B32.dev, C32.dev, nullptr);
                     CHECK(rc == TKMI_OK, "exec_dtype rc=%d", rc);

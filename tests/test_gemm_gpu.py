@@ -402,7 +402,7 @@ def _f32(*xs):
     return [x.astype(np.float32) for x in xs]
 
 
-@pytest.mark.parametrize("tile", [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012])
+@pytest.mark.parametrize("tile", ALL_TILES)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_f32_data_exact_integer(lib, ta, tb, tile):
     rng = np.random.default_rng(50 + 10 * ta + tb)
@@ -415,6 +415,32 @@ def test_f32_data_exact_integer(lib, ta, tb, tile):
             out = _run(lib, tab, *_f32(A, B, Cm), prec, alpha, beta, tile)
             assert out.dtype == np.float32
             np.testing.assert_array_equal(out, _ref(tab, A, B, Cm, alpha, beta).astype(np.float32))
+
+
+@pytest.mark.parametrize("tile", [1013, 1014, 1015, 1016])
+def test_f32_direct_load_kernel_exact(lib, tile):
+    """The direct-load FP32 kernel (configurations 13–16 on FP32 data, every group
+    untransposed): whole tiles, tiles past m and n, partial last slabs, long K split,
+    odd leading dimensions and offsets (16-byte chunks only 4-byte aligned) — exact on
+    integer data, and bit for bit the one-buffer FP32 kernel's result at the same split
+    factor on random data."""
+    rng = np.random.default_rng(70 + tile)
+    shapes = [(64, 64, 16), (128, 64, 32), (192, 130, 290), (64, 64, 1000), (70, 200, 45), (65, 65, 17),
+              (256, 128, 2050), (33, 300, 600), (3, 70, 100)]
+    tab, na, nb, nc = _make_groups(rng, shapes, [0] * len(shapes), [0] * len(shapes))
+    A = rng.integers(-8, 9, na).astype(np.float64)
+    B = rng.integers(-8, 9, nb).astype(np.float64)
+    Cm = rng.integers(-8, 9, nc).astype(np.float64)
+    for alpha, beta in ((1.0, 0.0), (2.0, -1.0), (1.0, 1.0)):
+        out = _run(lib, tab, *_f32(A, B, Cm), 1, alpha, beta, tile)
+        np.testing.assert_array_equal(out, _ref(tab, A, B, Cm, alpha, beta).astype(np.float32))
+    A, B, Cm = _f32(rng.standard_normal(na), rng.standard_normal(nb), rng.standard_normal(nc))
+    out = _run(lib, tab, A, B, Cm, 1, 0.75, 0.5, tile)
+    ref = _ref(tab, A.astype(np.float64), B.astype(np.float64), Cm.astype(np.float64), 0.75, 0.5)
+    assert np.linalg.norm(out - ref) <= 2e-6 * np.linalg.norm(ref)
+    if tile != 1013:
+        same = {1014: 1010, 1015: 1011, 1016: 1012}[tile]
+        np.testing.assert_array_equal(out, _run(lib, tab, A, B, Cm, 1, 0.75, 0.5, same))
 
 
 @pytest.mark.parametrize("tile", [1002, 1005, 1008, 1009])
