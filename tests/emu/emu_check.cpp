@@ -430,7 +430,12 @@ static void gemm_cases() {
         for (int32_t bm, bn, bk, nbuf; tkmi_gemm_config_info(ncfg, &bm, &bn, &bk, &nbuf) == TKMI_OK;) ++ncfg;
         for (int cfg = 0; cfg < ncfg; ++cfg) {
             for (int prec = 0; prec < 3; ++prec) {
-                const bool run = variant == 0 || cfg == 5 || cfg == 8 || cfg == 9 || (cfg == 1 && prec == 2);
+                // untransposed: every configuration in FP64, the distinct FP32 / 3×BF16 kernels
+                // (one-buffer, double-buffered 128-wide, two-stage, persistent, direct-load
+                // unsplit and split); transposed: the kernels those forms share
+                const bool distinct = cfg <= 2 || cfg == 5 || cfg == 8 || cfg == 9 || cfg == 13 || cfg == 15;
+                const bool run = (variant == 0 && (prec == 0 || distinct)) || cfg == 5 || cfg == 8 || cfg == 9 ||
+                                 (cfg == 1 && prec == 2);
                 if (!run) continue;
                 const double t0 = now();
                 rc = tkmi_gemm_plan_set_tile(plan, prec, 1000 + cfg);
