@@ -402,14 +402,17 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
 // column-major, the blocks of plain TensorMaps).  Slabs move with global_load_lds_dwordx4:
 // each lane fetches 16 bytes that land at LDS base + 16·lane, so no VGPR staging and no LDS
 // store instructions; two LDS stages let the loads of slab kt + 1 run while slab kt is
-// multiplied, with one barrier per slab.  A 64×64 tile, 4 waves (2×2), 16-deep slabs;
-// 16 KB of LDS per stage and 80 VGPRs, five workgroups per CU.  (A third stage measured
-// slower: it leaves room for three workgroups per CU.)
+// multiplied, with one barrier per slab.  A BM×BN tile (64 or 128 each), 4 waves (2×2), each
+// a (BM/2)×(BN/2) block of 16×16 fragments, 16-deep slabs.  64×64: 16 KB of LDS per stage,
+// 80 VGPRs, five workgroups per CU (a third stage measured slower: it leaves room for
+// three).  The larger tiles read each slab element for more MFMAs (LDS and L2 traffic per
+// flop down by a quarter at 128×64, a half at 128×128) and pay the per-tile prologue and
+// epilogue over more work, at three (128×64) or two (128×128) workgroups per CU.
 //
 // LDS layouts of a stage (16-byte chunks):
-//  A: chunk (k, p) = rows 2p, 2p+1 of column k (contiguous in memory), at slot 32k + p.  A
-//     wave's two 16-row fragments take the even and the odd rows of its 32 rows, so one
-//     chunk read gives a lane its A value for both fragments.
+//  A: chunk (k, p) = rows 2p, 2p+1 of column k (contiguous in memory), at slot (BM/2)k + p.
+//     A wave's rows split into halves of 32; a half's two 16-row fragments take its even
+//     and its odd rows, so one chunk read gives a lane its A value for both fragments.
 //  B: chunk (q, j) = rows 2q, 2q+1 (k) of column j, at slot 8j + (q ^ (j & 7)): a column's
 //     16 k stay contiguous (coalesced fetch) and the 16 lanes that read one k pair of 16
 //     columns hit distinct banks.
@@ -420,15 +423,20 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
 // holds the group's last column goes through registers, as does a split's partial last
 // slab (zero filled past K).  Byte offsets inside a slab are 32-bit (the host picks this
 // kernel only when they fit).
-constexpr int DL_CH = 64 * 16 / 2;  // 16-byte chunks per operand slab (512)
-
-__global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
-                                                         int64_t ntiles, const double *__restrict__ A,
-                                                         const double *__restrict__ Bm, double *__restrict__ C,
-                                                         double *__restrict__ ws, double alpha, double beta) {
-    constexpr int BM = 64, BN = 64, BK = 16;
+template <int BM, int BN>
+__global__ __launch_bounds__(NT, BM * BN > 8192 ? 2 : (BM * BN > 4096 ? 3 : 4)) void gemm_k4d_kernel(
+    const GGroup *__restrict__ groups, int64_t ngroups, int64_t ntiles, const double *__restrict__ A,
+    const double *__restrict__ Bm, double *__restrict__ C, double *__restrict__ ws, double alpha, double beta) {
+    static_assert((BM == 64 || BM == 128) && (BN == 64 || BN == 128), "tile edges of 64 or 128");
+    constexpr int BK = 16;
+    constexpr int H = BM / 64;              // 32-row halves of a wave's rows
+    constexpr int FN = BN / 32;             // 16-column fragments of a wave's columns
+    constexpr int ACH = 8 * BM, BCH = 8 * BN;  // 16-byte chunks per slab of A and of B
+    constexpr int AE = ACH / NT, BE = BCH / NT;  // chunks per lane (and 64-chunk blocks per wave)
+    constexpr int AP = BM / 2;                 // A chunks per column k
     typedef double d2v __attribute__((ext_vector_type(2)));
-    __shared__ __attribute__((aligned(16))) double Ls[2][2][2 * DL_CH];
+    __shared__ __attribute__((aligned(16))) double La[2][2 * ACH];
+    __shared__ __attribute__((aligned(16))) double Lb[2][2 * BCH];
 
     const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
     const GGroup &G = groups[find_group(groups, ngroups, tile)];
@@ -453,20 +461,26 @@ __global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restric
     const double *__restrict__ Aslab = A + G.a_off + row0 + kbeg * lda;
     const double *__restrict__ Bslab = Bm + G.b_off + col0 * ldb + kbeg;
 
-    d4_t acc[2][2];
+    d4_t acc[H][2][FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int h = 0; h < H; ++h)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0, 0, 0, 0};
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[h][i][j] = d4_t{0, 0, 0, 0};
 
-    // direct path: wave w issues chunks [64(2w + e), 64(2w + e) + 64) of A and of B, e < 2;
-    // chunk c is A (k = c / 32, p = c % 32) and B (j = c / 8, q = (c % 8) ^ (j & 7))
+    // direct path: wave w issues the 64-chunk blocks AE·w + e of A (e < AE) and BE·w + e of
+    // B; chunk c is A (k = c / AP, p = c % AP) and B (j = c / 8, q = (c % 8) ^ (j & 7))
     const int rlast = (int)min((int64_t)(BM - 1), m - 1 - row0), clast = (int)min((int64_t)(BN - 1), n - 1 - col0);
-    uint32_t aoff[2], boff[2];
+    uint32_t aoff[AE], boff[BE];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int c = 64 * (2 * wave + e) + lane;
-        aoff[e] = (uint32_t)(8 * (min(2 * (c & 31), rlast) + (c >> 5) * lda));
+    for (int e = 0; e < AE; ++e) {
+        const int c = 64 * (AE * wave + e) + lane;
+        aoff[e] = (uint32_t)(8 * (min(2 * (c % AP), rlast) + (c / AP) * lda));
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+        const int c = 64 * (BE * wave + e) + lane;
         const int j = c >> 3, q = (c & 7) ^ (j & 7);
         boff[e] = (uint32_t)(8 * (2 * q + min(j, clast) * ldb));
     }
@@ -474,73 +488,85 @@ __global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restric
         const char *ab = reinterpret_cast<const char *>(Aslab + (int64_t)kt * BK * lda);
         const char *bb = reinterpret_cast<const char *>(Bslab + (int64_t)kt * BK);
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(ab + aoff[e]),
-                (__attribute__((address_space(3))) void *)&Ls[st][0][2 * 64 * (2 * wave + e)], 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(
-                (const void *)(bb + boff[e]),
-                (__attribute__((address_space(3))) void *)&Ls[st][1][2 * 64 * (2 * wave + e)], 16, 0, 0);
-        }
+        for (int e = 0; e < AE; ++e)
+            __builtin_amdgcn_global_load_lds((const void *)(ab + aoff[e]),
+                                             (__attribute__((address_space(3))) void *)&La[st][2 * 64 * (AE * wave + e)],
+                                             16, 0, 0);
+#pragma unroll
+        for (int e = 0; e < BE; ++e)
+            __builtin_amdgcn_global_load_lds((const void *)(bb + boff[e]),
+                                             (__attribute__((address_space(3))) void *)&Lb[st][2 * 64 * (BE * wave + e)],
+                                             16, 0, 0);
     };
-    // register path: thread t fills chunks t and t + 256 of A and of B, zero past C's
-    // rows/columns and past K
+    // register path: thread t fills chunks t + e·NT of A (e < AE) and of B (e < BE), zero
+    // past C's rows/columns and past K
     auto fill = [&](int kt, int st) {
         const int64_t k0 = kbeg + (int64_t)kt * BK;
         const int ilim = rlast + 1, jlim = clast + 1;
         const int klim = (int)min((int64_t)BK, K - k0);
         const double *ab = Aslab + (int64_t)kt * BK * lda;
         const double *bb = Bslab + (int64_t)kt * BK;
-        d2v va[2], vb[2];
+        d2v va[AE], vb[BE];
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
+        for (int e = 0; e < AE; ++e) {
             const int c = tid + e * NT;
-            const int k = c >> 5, i = 2 * (c & 31);
+            const int k = c / AP, i = 2 * (c % AP);
             va[e].x = (i < ilim && k < klim) ? ab[i + k * lda] : 0.0;
             va[e].y = (i + 1 < ilim && k < klim) ? ab[i + 1 + k * lda] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            const int c = tid + e * NT;
             const int j = c >> 3, q = (c & 7) ^ (j & 7);
             vb[e].x = (j < jlim && 2 * q < klim) ? bb[2 * q + j * ldb] : 0.0;
             vb[e].y = (j < jlim && 2 * q + 1 < klim) ? bb[2 * q + 1 + j * ldb] : 0.0;
         }
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int c = tid + e * NT;
-            *reinterpret_cast<d2v *>(&Ls[st][0][2 * c]) = va[e];
-            *reinterpret_cast<d2v *>(&Ls[st][1][2 * c]) = vb[e];
-        }
+        for (int e = 0; e < AE; ++e) *reinterpret_cast<d2v *>(&La[st][2 * (tid + e * NT)]) = va[e];
+#pragma unroll
+        for (int e = 0; e < BE; ++e) *reinterpret_cast<d2v *>(&Lb[st][2 * (tid + e * NT)]) = vb[e];
     };
     // fragments that hold some row and column of C (edge tiles skip the others, a test
-    // uniform over the wave): fragment fi holds rows wm·32 + 2i + fi, fj columns
-    // wn·32 + 16fj + j
-    bool frag_ok[2][2];
+    // uniform over the wave): fragment (h, fi) holds rows wm·BM/2 + 32h + 2i + fi, fj
+    // columns wn·BN/2 + 16fj + j
+    bool frag_ok[H][2][FN];
 #pragma unroll
-    for (int fi = 0; fi < 2; ++fi)
+    for (int h = 0; h < H; ++h)
 #pragma unroll
-        for (int fj = 0; fj < 2; ++fj) frag_ok[fi][fj] = wm * 32 + fi <= rlast && wn * 32 + fj * 16 <= clast;
+        for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+            for (int fj = 0; fj < FN; ++fj)
+                frag_ok[h][fi][fj] = wm * (BM / 2) + 32 * h + fi <= rlast && wn * (BN / 2) + fj * 16 <= clast;
     // one slab from stage st: two rounds of 8 k; in step s of a round, lane group g4 holds
     // k = 8·round + 2·g4 + s for A and B alike
     auto mma = [&](int st, auto edge) {
         constexpr bool EDGE = decltype(edge)::value;
-        const double *As = Ls[st][0], *Bs = Ls[st][1];
+        const double *As = La[st], *Bs = Lb[st];
 #pragma unroll
         for (int rd = 0; rd < 2; ++rd) {
-            d2v a[2], b[2];
+            d2v a[H][2], b[FN];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
-                a[s] = *reinterpret_cast<const d2v *>(&As[2 * ((8 * rd + 2 * g4 + s) * 32 + wm * 16 + r16)]);
+            for (int h = 0; h < H; ++h)
 #pragma unroll
-            for (int fj = 0; fj < 2; ++fj) {
-                const int j = wn * 32 + fj * 16 + r16, q = 4 * rd + g4;
+                for (int s = 0; s < 2; ++s)
+                    a[h][s] = *reinterpret_cast<const d2v *>(
+                        &As[2 * ((8 * rd + 2 * g4 + s) * AP + wm * (BM / 4) + 16 * h + r16)]);
+#pragma unroll
+            for (int fj = 0; fj < FN; ++fj) {
+                const int j = wn * (BN / 2) + fj * 16 + r16, q = 4 * rd + g4;
                 b[fj] = *reinterpret_cast<const d2v *>(&Bs[2 * (8 * j + (q ^ (j & 7)))]);
             }
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int fi = 0; fi < 2; ++fi)
+                for (int h = 0; h < H; ++h)
 #pragma unroll
-                    for (int fj = 0; fj < 2; ++fj)
-                        if (!EDGE || frag_ok[fi][fj])
-                            acc[fi][fj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][fi], b[fj][s], acc[fi][fj], 0, 0, 0);
+                    for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+                        for (int fj = 0; fj < FN; ++fj)
+                            if (!EDGE || frag_ok[h][fi][fj])
+                                acc[h][fi][fj] =
+                                    __builtin_amdgcn_mfma_f64_16x16x4f64(a[h][s][fi], b[fj][s], acc[h][fi][fj], 0, 0, 0);
         }
     };
     auto run = [&](auto edge) {
@@ -571,38 +597,40 @@ __global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restric
     else
         run(std::false_type{});
 
-    // epilogue: fragment fi holds rows wm·32 + 2i + fi, i = fragment row, so a lane holds
-    // the two adjacent rows (2i, 2i + 1) of its column in acc[0] and acc[1]: one 16-byte
-    // store writes both, and the 16 lanes of a row group cover 8 consecutive rows of 16
-    // columns (whole 64-byte segments rather than every other row)
+    // epilogue: fragment (h, fi) holds rows wm·BM/2 + 32h + 2i + fi, i = fragment row, so a
+    // lane holds the two adjacent rows (2i, 2i + 1) of its column in acc[h][0] and
+    // acc[h][1]: one 16-byte store writes both, and the 16 lanes of a row group cover 8
+    // consecutive rows of 16 columns (whole 64-byte segments rather than every other row)
     const bool partial = G.splits > 1;
     double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
     const int64_t ldc = partial ? m : G.ldc;
     const double a_ = partial ? 1.0 : alpha;
     const bool has_beta = !partial && beta != 0.0;
 #pragma unroll
-    for (int fj = 0; fj < 2; ++fj) {
-        const int64_t col = col0 + wn * 32 + fj * 16 + r16;
+    for (int fj = 0; fj < FN; ++fj) {
+        const int64_t col = col0 + wn * (BN / 2) + fj * 16 + r16;
         if (col >= n) continue;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t row = row0 + wm * 32 + 2 * (g4 + 4 * v);
-            double *p = Cg + row + col * ldc;
-            const double r0 = a_ * acc[0][fj][v], r1 = a_ * acc[1][fj][v];
-            if (row + 1 < m) {
-                d2v o;
-                if (has_beta) {
-                    d2v c;
-                    __builtin_memcpy(&c, p, sizeof(c));
-                    o = d2v{r0 + beta * c.x, r1 + beta * c.y};
-                } else {
-                    o = d2v{r0, r1};
+        for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = row0 + wm * (BM / 2) + 32 * h + 2 * (g4 + 4 * v);
+                double *p = Cg + row + col * ldc;
+                const double r0 = a_ * acc[h][0][fj][v], r1 = a_ * acc[h][1][fj][v];
+                if (row + 1 < m) {
+                    d2v o;
+                    if (has_beta) {
+                        d2v c;
+                        __builtin_memcpy(&c, p, sizeof(c));
+                        o = d2v{r0 + beta * c.x, r1 + beta * c.y};
+                    } else {
+                        o = d2v{r0, r1};
+                    }
+                    __builtin_memcpy(p, &o, sizeof(o));
+                } else if (row < m) {
+                    put_c(p, r0, beta, has_beta);
                 }
-                __builtin_memcpy(p, &o, sizeof(o));
-            } else if (row < m) {
-                put_c(p, r0, beta, has_beta);
             }
-        }
     }
 }
 
@@ -1841,17 +1869,22 @@ struct Cfg {
 // kernels (FP64; FP32 on FP32 data; 3×BF16) with the model's split factor and with the
 // factor fixed at 1, 2 and 3; where they do not apply (transposed operands, FP32
 // arithmetic on FP64 data) they run configuration 5's kernel with their split factor.
+// Configurations 17–19 are the FP64 direct-load kernel on 128×128, 128×64 and 64×128
+// tiles (model split factor); in FP32 and 3×BF16 they run the one-buffer kernels of
+// their tiles.
 constexpr Cfg CFGS[] = {{64, 64, 16, 2, 0},  {128, 128, 16, 2, 0}, {128, 64, 16, 2, 0}, {64, 128, 16, 2, 0},
                         {64, 64, 32, 2, 0},  {64, 64, 16, 1, 0},   {64, 64, 32, 1, 0},  {128, 128, 16, 1, 0},
                         {64, 64, 16, 3, 0},  {64, 64, 16, 5, 0},   {64, 64, 16, 1, 1},  {64, 64, 16, 1, 2},
                         {64, 64, 16, 1, 3},  {64, 64, 16, 6, 0},   {64, 64, 16, 6, 1},  {64, 64, 16, 6, 2},
-                        {64, 64, 16, 6, 3}};
+                        {64, 64, 16, 6, 3},  {128, 128, 16, 6, 0}, {128, 64, 16, 6, 0}, {64, 128, 16, 6, 0}};
 // the 3×BF16 kernel has a fixed 32-deep slab and its own LDS use: only the tile edges
 // of a configuration matter to it, the prefetch depth of configuration 8, the fixed
 // split factors of configurations 10–12, and configurations 13–16, which run the
 // direct-load 3×BF16 kernel
-constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9; }
+constexpr bool bf16_duplicate(int c) { return (c >= 4 && c <= 7) || c == 9 || c >= 17; }
 constexpr bool direct_cfg(int c) { return CFGS[c].nbuf == 6; }
+// direct-load configurations with tiles other than 64×64: FP64 only
+constexpr bool wide_direct(int c) { return direct_cfg(c) && (CFGS[c].bm != 64 || CFGS[c].bn != 64); }
 constexpr int NCFG = sizeof(CFGS) / sizeof(CFGS[0]);
 
 }  // namespace
@@ -1913,7 +1946,7 @@ int kernel_occupancy() {
             &nb, reinterpret_cast<const void *>(&gemm_k4_kernel<double, double, BM, BN, BK>), NT, 0);
     else if constexpr (NBUF == 6)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&gemm_k4d_kernel), NT, 0);
+            &nb, reinterpret_cast<const void *>(&gemm_k4d_kernel<BM, BN>), NT, 0);
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void *>(&gemm_k4s_kernel<double, double, BM, BN, BK>), NT, 0);
@@ -1943,6 +1976,9 @@ int64_t cfg_slots(int c) {
             case 8: occ = kernel_occupancy<64, 64, 16, 3>(); break;
             case 9: occ = kernel_occupancy<64, 64, 16, 5>(); break;
             case 13: case 14: case 15: case 16: occ = kernel_occupancy<64, 64, 16, 6>(); break;
+            case 17: occ = kernel_occupancy<128, 128, 16, 6>(); break;
+            case 18: occ = kernel_occupancy<128, 64, 16, 6>(); break;
+            case 19: occ = kernel_occupancy<64, 128, 16, 6>(); break;
             default: occ = kernel_occupancy<64, 64, 16, 1>(); break;
         }
         v = occ * cus;
@@ -2047,11 +2083,12 @@ int build_tiles(tkmi_gemm_plan_s *p, int c) {
     }
     ts.ntiles = start;
     if (ts.groups.empty()) return TKMI_OK;
-    // the direct-load kernel's 32-bit slab offsets: A chunk rows up to 62 + 15·lda, B up to
-    // 14 + 63·ldb elements
+    // the direct-load kernel's 32-bit slab offsets: A chunk rows up to bm − 2 + 15·lda, B up
+    // to 14 + (bn − 1)·ldb elements
     ts.direct_ok = true;
     for (const GGroup &g : ts.groups)
-        if (g.ta || g.tb || 8 * (62 + 15 * g.lda) >= ((int64_t)1 << 32) || 8 * (14 + 63 * g.ldb) >= ((int64_t)1 << 32))
+        if (g.ta || g.tb || 8 * (bm - 2 + 15 * g.lda) >= ((int64_t)1 << 32) ||
+            8 * (14 + (bn - 1) * g.ldb) >= ((int64_t)1 << 32))
             ts.direct_ok = false;
     // 3×BF16 panel offsets: up to two plane strides, 63 rows and 3 octets, in bytes
     ts.bf16_direct_ok = true;
@@ -2196,12 +2233,12 @@ void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha,
         // otherwise the one-buffer kernel of the same tiles
         if constexpr (std::is_same<TC, double>::value && std::is_same<TD, double>::value) {
             if (ts.direct_ok) {
-                launch_timed(gemm_k4d_kernel, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws, alpha,
-                             beta);
+                launch_timed(gemm_k4d_kernel<BM, BN>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws,
+                             alpha, beta);
                 return;
             }
         }
-        if constexpr (std::is_same<TC, float>::value && std::is_same<TD, float>::value) {
+        if constexpr (std::is_same<TC, float>::value && std::is_same<TD, float>::value && BM == 64 && BN == 64) {
             if (ts.direct_ok) {  // (its 32-bit offset bounds imply the FP32 kernel's)
                 launch_timed(gemm_k4d32_kernel, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws,
                              alpha, beta);
@@ -2306,6 +2343,9 @@ int launch_cfg(tkmi_gemm_plan_s *p, int c, int prec, const TD *A, const TD *B, T
         case 13: case 14: case 15: case 16:
             launch_k<TD, 64, 64, 16, 6>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce);
             break;
+        case 17: launch_k<TD, 128, 128, 16, 6>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 18: launch_k<TD, 128, 64, 16, 6>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
+        case 19: launch_k<TD, 64, 128, 16, 6>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
         default: launch_k<TD, 64, 64, 16, 1>(p, ts, prec, A, B, C, alpha, beta, s, tiles, reduce); break;
     }
     TKMI_HIP(hipGetLastError());
@@ -2457,7 +2497,8 @@ int autotune(tkmi_gemm_plan plan, int32_t precision, const TD *A, const TD *B, T
             if (precision == TKMI_PREC_BF16X3 && bf16_duplicate(c)) continue;
             // the direct-load configurations run configuration 5's kernel where they do not apply
             if (direct_cfg(c) &&
-                ((precision == TKMI_PREC_F32 && (!std::is_same<TD, float>::value || !plan->ts[c].direct_ok)) ||
+                ((precision != TKMI_PREC_F64 && wide_direct(c)) ||
+                 (precision == TKMI_PREC_F32 && (!std::is_same<TD, float>::value || !plan->ts[c].direct_ok)) ||
                  (precision == TKMI_PREC_F64 && (!std::is_same<TD, double>::value || !plan->ts[c].direct_ok)) ||
                  (precision == TKMI_PREC_BF16X3 && !plan->ts[c].bf16_direct_ok)))
                 continue;

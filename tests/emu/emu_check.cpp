@@ -355,16 +355,16 @@ struct GSpec {
 static void gemm_cases() {
     // every kind of group: ragged MFMA tiles, skinny (both orientations), k = 0, a dot
     // product, and one long-K group that the schedule model splits
-    // (the last three, untransposed only, for the direct-load kernel: whole 64×64 tiles with
-    // a partial last slab; long enough to be split; and, last in A with lda = m, a tile
-    // past m whose K is whole slabs, so that a read of row m in the group's last column
-    // would leave the buffer)
+    // (the last four, untransposed only, for the direct-load kernels: whole 64×64 tiles with
+    // a partial last slab; long enough to be split; whole 128×128 and 128×64 tiles with
+    // whole slabs only; and, last in A with lda = m, a tile past m whose K is whole slabs,
+    // so that a read of row m in the group's last column would leave the buffer)
     const int64_t shapes[][3] = {{1, 1, 1},   {16, 16, 4},  {29, 24, 5},   {65, 63, 17},  {5, 70, 7},
                                  {70, 4, 4},  {17, 33, 0},  {1, 1, 300},   {40, 8, 30},   {20, 20, 520},
-                                 {130, 70, 45}, {64, 64, 600}, {70, 40, 32}};
+                                 {130, 70, 45}, {64, 64, 600}, {136, 130, 32}, {70, 40, 32}};
     for (int variant = 0; variant < 4; ++variant) {
         const int nshapes = (int)(sizeof(shapes) / sizeof(shapes[0]));
-        const int ng = nshapes - (variant == 0 ? 0 : 3);
+        const int ng = nshapes - (variant == 0 ? 0 : 4);
         std::vector<GSpec> gs;
         int64_t na = 0, nb = 0, nc = 0;
         for (int g = 0; g < ng; ++g) {

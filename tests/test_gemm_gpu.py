@@ -85,7 +85,8 @@ SHAPES = [(1, 1, 1), (16, 16, 4), (29, 24, 5), (64, 64, 16), (65, 63, 17), (130,
           (5, 300, 7), (300, 4, 4), (17, 33, 0), (0, 5, 3)]
 
 
-ALL_TILES = [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012, 1013, 1014, 1015, 1016]
+ALL_TILES = [64, 128, 1002, 1003, 1004, 1005, 1006, 1007, 1008, 1009, 1010, 1011, 1012, 1013, 1014, 1015, 1016,
+             1017, 1018, 1019]
 
 
 @pytest.mark.parametrize("tile", ALL_TILES)
@@ -102,11 +103,12 @@ def test_f64_exact_integer(lib, ta, tb, tile):
         np.testing.assert_array_equal(out, ref)
 
 
-@pytest.mark.parametrize("tile", [1013, 1014, 1015, 1016])
+@pytest.mark.parametrize("tile", [1013, 1014, 1015, 1016, 1017, 1018, 1019])
 def test_direct_load_kernel_exact(lib, tile):
-    """The direct global → LDS kernel (configurations 13–16; every group untransposed):
-    whole 64×64 tiles with one, two and many slabs, partial last slabs, ragged edge tiles,
-    long K split (13: the model's factor, 15, 16: two and three), odd leading dimensions
+    """The direct global → LDS kernel (configurations 13–16 on 64×64 tiles, 17–19 on
+    128×128, 128×64 and 64×128; every group untransposed): whole tiles with one, two and
+    many slabs, partial last slabs, ragged edge tiles, long K split (13, 17–19: the
+    model's factor, 15, 16: two and three), odd leading dimensions
     and offsets (16-byte chunks only 8-byte aligned) — exact on integer data for β = 0, −1
     and 1, and equal bit for bit on random data to the one-buffer kernel's result at the
     same split factor (configurations 10–12); the same for the direct-load 3×BF16 kernel,
@@ -127,7 +129,7 @@ def test_direct_load_kernel_exact(lib, tile):
     ref = _ref(tab, A, B, Cm, 0.75, 0.5)
     assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
     # the one-buffer kernel with the same split factor forms the same sums in the same k order
-    if tile != 1013:  # configuration 13 takes the model's split factor, 14–16 fix it at 1–3
+    if tile in (1014, 1015, 1016):  # 13 and 17–19 take the model's split factor, 14–16 fix it at 1–3
         same = {1014: 1010, 1015: 1011, 1016: 1012}[tile]
         np.testing.assert_array_equal(out, _run(lib, tab, A, B, Cm, 0, 0.75, 0.5, same))
         # 3×BF16: the direct-load kernel (configurations 13–16) forms the same six products
