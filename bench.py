@@ -47,8 +47,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE scaled ×2 and WRITE_SIZE ×1 by a 512 MiB copy): (kernel, grid_x) → bytes.
 # Source: profiles/hbm_traffic_pmc.md.  A launch not listed reports traffic null.
 TRAFFIC_PMC = {
-    ("gemm_k4d_kernel<64, 64>", 545280): round((78488.9 * 2 + 66341.9) * 1024),  # L·x, direct-load kernel
-    ("gemm_k4d_kernel<64, 64>", 243712): round((96842.0 * 2 + 26457.2) * 1024),  # ·R, split-K 2 partials
+    ("gemm_k4d_kernel", 545280): round((78488.9 * 2 + 66341.9) * 1024),  # L·x, direct-load kernel
+    ("gemm_k4d_kernel", 243712): round((96842.0 * 2 + 26457.2) * 1024),  # ·R, split-K 2 partials
     ("gemm_k4s_kernel<double, 64, 64, 16>", 545280): round((80852.8 * 2 + 68587.2) * 1024),  # L·x
     ("gemm_k4s_kernel<double, 64, 64, 16>", 243712): round((115131.4 * 2 + 27080.2) * 1024),  # ·R, split-K 2 partials
     ("gemm_k4s_kernel<double, 64, 64, 16>", 517632): round((96597.6 * 2 + 57122.2) * 1024),  # ·R, split-K 5 partials
@@ -101,7 +101,7 @@ def gemm_kernel_name(prec, cfg, L):
     k4 = {2: "gemm_k4_kernel", 1: "gemm_k4s_kernel", 3: "gemm_k4p_kernel", 5: "gemm_pks_kernel",
           6: "gemm_k4s_kernel"}[nb.value]
     if prec == 0 and nb.value == 6:
-        return f"gemm_k4d_kernel<{bm}, {bn}>"
+        return "gemm_k4d_kernel" if bm == bn == 64 else f"gemm_k4dw_kernel<{bm}, {bn}>"
     return {0: f"{k4}<double, {bm}, {bn}, {bk}>", 1: f"{k4}<float, {bm}, {bn}, {bk}>",
             2: f"gemm_bf16x3_kernel<{bm}, {bn}>"}[prec]
 

@@ -205,9 +205,11 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
  * the 3×BF16 form gemm_bf16x3d_kernel; where no direct-load kernel applies (a transposed
  * operand, FP32 arithmetic on FP64 data) configurations 13–16 run configuration 5's
  * kernel with their own split factor (tkmi_gemm_plan_splitk reports the factor of each).
- * Configurations 17, 18 and 19 are gemm_k4d_kernel on 128×128, 128×64 and 64×128 tiles
- * (model split factor; FP64 with untransposed operands, else the one-buffer kernel of
- * the same tiles).                                                                    */
+ * Configurations 17, 18 and 19 are the direct-load kernel on 128×128, 128×64 and 64×128
+ * tiles (gemm_k4dw_kernel; model split factor; FP64 with untransposed operands, else the
+ * one-buffer kernel of the same tiles).  Untuned FP64 plans of untransposed products
+ * whose 128×64 tiles fill at least four rounds of the GPU's workgroup slots run 18, other
+ * untransposed FP64 plans 13, the rest 5.                                              */
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
  * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64) with their split-K

@@ -402,17 +402,14 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
 // column-major, the blocks of plain TensorMaps).  Slabs move with global_load_lds_dwordx4:
 // each lane fetches 16 bytes that land at LDS base + 16·lane, so no VGPR staging and no LDS
 // store instructions; two LDS stages let the loads of slab kt + 1 run while slab kt is
-// multiplied, with one barrier per slab.  A BM×BN tile (64 or 128 each), 4 waves (2×2), each
-// a (BM/2)×(BN/2) block of 16×16 fragments, 16-deep slabs.  64×64: 16 KB of LDS per stage,
-// 80 VGPRs, five workgroups per CU (a third stage measured slower: it leaves room for
-// three).  The larger tiles read each slab element for more MFMAs (LDS and L2 traffic per
-// flop down by a quarter at 128×64, a half at 128×128) and pay the per-tile prologue and
-// epilogue over more work, at three (128×64) or two (128×128) workgroups per CU.
+// multiplied, with one barrier per slab.  A 64×64 tile, 4 waves (2×2), 16-deep slabs;
+// 16 KB of LDS per stage and 80 VGPRs, five workgroups per CU.  (A third stage measured
+// slower: it leaves room for three workgroups per CU.)
 //
 // LDS layouts of a stage (16-byte chunks):
-//  A: chunk (k, p) = rows 2p, 2p+1 of column k (contiguous in memory), at slot (BM/2)k + p.
-//     A wave's rows split into halves of 32; a half's two 16-row fragments take its even
-//     and its odd rows, so one chunk read gives a lane its A value for both fragments.
+//  A: chunk (k, p) = rows 2p, 2p+1 of column k (contiguous in memory), at slot 32k + p.  A
+//     wave's two 16-row fragments take the even and the odd rows of its 32 rows, so one
+//     chunk read gives a lane its A value for both fragments.
 //  B: chunk (q, j) = rows 2q, 2q+1 (k) of column j, at slot 8j + (q ^ (j & 7)): a column's
 //     16 k stay contiguous (coalesced fetch) and the 16 lanes that read one k pair of 16
 //     columns hit distinct banks.
@@ -423,11 +420,206 @@ __global__ __launch_bounds__(NT) void gemm_k4p_kernel(const GGroup *__restrict__
 // holds the group's last column goes through registers, as does a split's partial last
 // slab (zero filled past K).  Byte offsets inside a slab are 32-bit (the host picks this
 // kernel only when they fit).
+constexpr int DL_CH = 64 * 16 / 2;  // 16-byte chunks per operand slab (512)
+
+__global__ __launch_bounds__(NT, 4) void gemm_k4d_kernel(const GGroup *__restrict__ groups, int64_t ngroups,
+                                                         int64_t ntiles, const double *__restrict__ A,
+                                                         const double *__restrict__ Bm, double *__restrict__ C,
+                                                         double *__restrict__ ws, double alpha, double beta) {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    __shared__ __attribute__((aligned(16))) double Ls[2][2][2 * DL_CH];
+
+    const int64_t tile = xcd_remap((int64_t)blockIdx.x, ntiles);
+    const GGroup &G = groups[find_group(groups, ngroups, tile)];
+    const int64_t lt = tile - G.tile_start;
+    const int64_t tiles_mn = G.tiles_m * G.tiles_n;
+    const int64_t z = lt / tiles_mn, lmn = lt - z * tiles_mn;
+    const int64_t tm = lmn / G.tiles_n, tn = lmn % G.tiles_n;
+    const int64_t m = G.m, n = G.n;
+    const int64_t kbeg = z * G.kchunk;
+    const int64_t K = min(G.k, kbeg + G.kchunk);
+    const int64_t row0 = tm * BM, col0 = tn * BN;
+    const int64_t lda = G.lda, ldb = G.ldb;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const bool m_edge = row0 + BM > m, n_edge = col0 + BN > n;
+    const int nfull = (int)((K - kbeg) / BK);  // whole slabs
+    const bool tail = (K - kbeg) % BK != 0;    // a last partial slab
+    const int nslab = nfull + (tail ? 1 : 0);
+    const int ndirect = (m_edge && !tail && K == G.k) ? max(nfull - 1, 0) : nfull;
+    const double *__restrict__ Aslab = A + G.a_off + row0 + kbeg * lda;
+    const double *__restrict__ Bslab = Bm + G.b_off + col0 * ldb + kbeg;
+
+    d4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0, 0, 0, 0};
+
+    // direct path: wave w issues chunks [64(2w + e), 64(2w + e) + 64) of A and of B, e < 2;
+    // chunk c is A (k = c / 32, p = c % 32) and B (j = c / 8, q = (c % 8) ^ (j & 7))
+    const int rlast = (int)min((int64_t)(BM - 1), m - 1 - row0), clast = (int)min((int64_t)(BN - 1), n - 1 - col0);
+    uint32_t aoff[2], boff[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int c = 64 * (2 * wave + e) + lane;
+        aoff[e] = (uint32_t)(8 * (min(2 * (c & 31), rlast) + (c >> 5) * lda));
+        const int j = c >> 3, q = (c & 7) ^ (j & 7);
+        boff[e] = (uint32_t)(8 * (2 * q + min(j, clast) * ldb));
+    }
+    auto issue = [&](int kt, int st) {
+        const char *ab = reinterpret_cast<const char *>(Aslab + (int64_t)kt * BK * lda);
+        const char *bb = reinterpret_cast<const char *>(Bslab + (int64_t)kt * BK);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(ab + aoff[e]),
+                (__attribute__((address_space(3))) void *)&Ls[st][0][2 * 64 * (2 * wave + e)], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(bb + boff[e]),
+                (__attribute__((address_space(3))) void *)&Ls[st][1][2 * 64 * (2 * wave + e)], 16, 0, 0);
+        }
+    };
+    // register path: thread t fills chunks t and t + 256 of A and of B, zero past C's
+    // rows/columns and past K
+    auto fill = [&](int kt, int st) {
+        const int64_t k0 = kbeg + (int64_t)kt * BK;
+        const int ilim = rlast + 1, jlim = clast + 1;
+        const int klim = (int)min((int64_t)BK, K - k0);
+        const double *ab = Aslab + (int64_t)kt * BK * lda;
+        const double *bb = Bslab + (int64_t)kt * BK;
+        d2v va[2], vb[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int c = tid + e * NT;
+            const int k = c >> 5, i = 2 * (c & 31);
+            va[e].x = (i < ilim && k < klim) ? ab[i + k * lda] : 0.0;
+            va[e].y = (i + 1 < ilim && k < klim) ? ab[i + 1 + k * lda] : 0.0;
+            const int j = c >> 3, q = (c & 7) ^ (j & 7);
+            vb[e].x = (j < jlim && 2 * q < klim) ? bb[2 * q + j * ldb] : 0.0;
+            vb[e].y = (j < jlim && 2 * q + 1 < klim) ? bb[2 * q + 1 + j * ldb] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int c = tid + e * NT;
+            *reinterpret_cast<d2v *>(&Ls[st][0][2 * c]) = va[e];
+            *reinterpret_cast<d2v *>(&Ls[st][1][2 * c]) = vb[e];
+        }
+    };
+    // fragments that hold some row and column of C (edge tiles skip the others, a test
+    // uniform over the wave): fragment fi holds rows wm·32 + 2i + fi, fj columns
+    // wn·32 + 16fj + j
+    bool frag_ok[2][2];
+#pragma unroll
+    for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+        for (int fj = 0; fj < 2; ++fj) frag_ok[fi][fj] = wm * 32 + fi <= rlast && wn * 32 + fj * 16 <= clast;
+    // one slab from stage st: two rounds of 8 k; in step s of a round, lane group g4 holds
+    // k = 8·round + 2·g4 + s for A and B alike
+    auto mma = [&](int st, auto edge) {
+        constexpr bool EDGE = decltype(edge)::value;
+        const double *As = Ls[st][0], *Bs = Ls[st][1];
+#pragma unroll
+        for (int rd = 0; rd < 2; ++rd) {
+            d2v a[2], b[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                a[s] = *reinterpret_cast<const d2v *>(&As[2 * ((8 * rd + 2 * g4 + s) * 32 + wm * 16 + r16)]);
+#pragma unroll
+            for (int fj = 0; fj < 2; ++fj) {
+                const int j = wn * 32 + fj * 16 + r16, q = 4 * rd + g4;
+                b[fj] = *reinterpret_cast<const d2v *>(&Bs[2 * (8 * j + (q ^ (j & 7)))]);
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int fi = 0; fi < 2; ++fi)
+#pragma unroll
+                    for (int fj = 0; fj < 2; ++fj)
+                        if (!EDGE || frag_ok[fi][fj])
+                            acc[fi][fj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][fi], b[fj][s], acc[fi][fj], 0, 0, 0);
+        }
+    };
+    auto run = [&](auto edge) {
+        if (ndirect > 0) issue(0, 0);
+        for (int kt = 0; kt < ndirect; ++kt) {
+            // this wave's loads of slab kt have landed; the barrier then covers every
+            // wave's, and every wave is done reading the stage that slab kt + 1 refills.
+            // A plain barrier: __syncthreads' release fence would also wait for loads
+            // issued later; the waves' LDS reads of the previous slab are complete (their
+            // MFMAs consumed them), and the asm statements keep the compiler from moving
+            // LDS accesses across.
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (kt + 1 < ndirect) issue(kt + 1, (kt + 1) & 1);
+            mma(kt & 1, edge);
+        }
+        for (int kt = ndirect; kt < nslab; ++kt) {
+            __syncthreads();
+            fill(kt, kt & 1);
+            __syncthreads();
+            mma(kt & 1, edge);
+        }
+    };
+    if (m_edge || n_edge)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
+
+    // epilogue: fragment fi holds rows wm·32 + 2i + fi, i = fragment row, so a lane holds
+    // the two adjacent rows (2i, 2i + 1) of its column in acc[0] and acc[1]: one 16-byte
+    // store writes both, and the 16 lanes of a row group cover 8 consecutive rows of 16
+    // columns (whole 64-byte segments rather than every other row)
+    const bool partial = G.splits > 1;
+    double *Cg = partial ? ws + G.ws_off + z * m * n : C + G.c_off;
+    const int64_t ldc = partial ? m : G.ldc;
+    const double a_ = partial ? 1.0 : alpha;
+    const bool has_beta = !partial && beta != 0.0;
+#pragma unroll
+    for (int fj = 0; fj < 2; ++fj) {
+        const int64_t col = col0 + wn * 32 + fj * 16 + r16;
+        if (col >= n) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = row0 + wm * 32 + 2 * (g4 + 4 * v);
+            double *p = Cg + row + col * ldc;
+            const double r0 = a_ * acc[0][fj][v], r1 = a_ * acc[1][fj][v];
+            if (row + 1 < m) {
+                d2v o;
+                if (has_beta) {
+                    d2v c;
+                    __builtin_memcpy(&c, p, sizeof(c));
+                    o = d2v{r0 + beta * c.x, r1 + beta * c.y};
+                } else {
+                    o = d2v{r0, r1};
+                }
+                __builtin_memcpy(p, &o, sizeof(o));
+            } else if (row < m) {
+                put_c(p, r0, beta, has_beta);
+            }
+        }
+    }
+}
+
+// The direct-load kernel on 128×128, 128×64 and 64×128 tiles: the same slab layouts
+// and edge rules as gemm_k4d_kernel, with 4 waves (2×2) each taking a (BM/2)×(BN/2) block
+// of 16×16 fragments.  A's slab chunk (k, p) sits at slot (BM/2)k + p, and a wave's rows
+// split into halves of 32, each read as the even and odd fragment pair; each LDS read
+// and each loaded slab element feeds more MFMAs (LDS and L2 traffic per flop down by a
+// quarter at 128×64, a half at 128×128), and the per-tile prologue and epilogue spread
+// over more work, at three (128×64, 64×128: 48 KB of LDS) or two (128×128: 64 KB)
+// workgroups per CU.  (The 64×64 tile keeps its own kernel: built from this template it
+// measured 8–9% slower.)
 template <int BM, int BN>
-__global__ __launch_bounds__(NT, BM * BN > 8192 ? 2 : (BM * BN > 4096 ? 3 : 4)) void gemm_k4d_kernel(
+__global__ __launch_bounds__(NT, BM * BN > 8192 ? 2 : (BM * BN > 4096 ? 3 : 4)) void gemm_k4dw_kernel(
     const GGroup *__restrict__ groups, int64_t ngroups, int64_t ntiles, const double *__restrict__ A,
     const double *__restrict__ Bm, double *__restrict__ C, double *__restrict__ ws, double alpha, double beta) {
-    static_assert((BM == 64 || BM == 128) && (BN == 64 || BN == 128), "tile edges of 64 or 128");
+    static_assert((BM == 64 || BM == 128) && (BN == 64 || BN == 128) && BM * BN > 4096, "128-wide tiles");
     constexpr int BK = 16;
     constexpr int H = BM / 64;              // 32-row halves of a wave's rows
     constexpr int FN = BN / 32;             // 16-column fragments of a wave's columns
@@ -1944,9 +2136,12 @@ int kernel_occupancy() {
     else if constexpr (NBUF == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void *>(&gemm_k4_kernel<double, double, BM, BN, BK>), NT, 0);
+    else if constexpr (NBUF == 6 && BM == 64 && BN == 64)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&gemm_k4d_kernel), NT, 0);
     else if constexpr (NBUF == 6)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void *>(&gemm_k4d_kernel<BM, BN>), NT, 0);
+            &nb, reinterpret_cast<const void *>(&gemm_k4dw_kernel<BM, BN>), NT, 0);
     else
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void *>(&gemm_k4s_kernel<double, double, BM, BN, BK>), NT, 0);
@@ -2204,9 +2399,13 @@ int build_ztiles(tkmi_gemm_plan_s *p) {
 
 int auto_cfg(const tkmi_gemm_plan_s *p, int prec) {
     // FP64 with untransposed operands: the direct-load kernel with the model's split factor
-    // (8–10% faster than the one-buffer kernel on the sector GEMMs measured).  Otherwise
-    // 64×64×16 with one LDS buffer: the fastest on the sector GEMMs measured (K or N of a
-    // few hundred).  The autotuner moves a plan elsewhere when another is > 3% faster.
+    // (8–10% faster than the one-buffer kernel on the sector GEMMs measured), on 128×64
+    // tiles when they fill at least four rounds of its workgroup slots (4–8% faster than
+    // 64×64 on large products with K ≤ 1024; on launches of one or two rounds, such as
+    // the H_eff sectors, the 64×64 tiles' shorter tail wins).  Otherwise 64×64×16 with one
+    // LDS buffer: the fastest on the sector GEMMs measured (K or N of a few hundred).  The
+    // autotuner moves a plan elsewhere when another is > 3% faster.
+    if (prec == TKMI_PREC_F64 && p->ts[18].direct_ok && p->ts[18].ntiles >= 4 * cfg_slots(18)) return 18;
     return (prec == TKMI_PREC_F64 && p->ts[13].direct_ok) ? 13 : 5;
 }
 
@@ -2233,8 +2432,12 @@ void launch_k4(const TileSet &ts, const TD *A, const TD *B, TD *C, double alpha,
         // otherwise the one-buffer kernel of the same tiles
         if constexpr (std::is_same<TC, double>::value && std::is_same<TD, double>::value) {
             if (ts.direct_ok) {
-                launch_timed(gemm_k4d_kernel<BM, BN>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws,
-                             alpha, beta);
+                if constexpr (BM == 64 && BN == 64)
+                    launch_timed(gemm_k4d_kernel, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B, C, ts.ws,
+                                 alpha, beta);
+                else
+                    launch_timed(gemm_k4dw_kernel<BM, BN>, grid, block, s, ev, ts.d_groups, ng, ts.ntiles, A, B,
+                                 C, ts.ws, alpha, beta);
                 return;
             }
         }

@@ -140,6 +140,37 @@ def test_direct_load_kernel_exact(lib, tile):
     assert np.linalg.norm(out - ref) <= 1e-6 * np.linalg.norm(ref)
 
 
+def test_default_config_large_products(lib):
+    """Without autotuning, an FP64 plan of untransposed products whose 128×64 tiles fill at
+    least four rounds of the workgroup slots runs the direct-load kernel on 128×64 tiles
+    (configuration 18), a plan of short launches on 64×64 (13); both exact on integer data."""
+    _lib = lib
+    L = lib.load()
+    rng = np.random.default_rng(61)
+    for shapes, want in (([(2048, 2048, 40)] * 8, 18), ([(300, 280, 290), (130, 70, 33)], 13)):
+        tab, na, nb, nc = _make_groups(rng, shapes, [0] * len(shapes), [0] * len(shapes))
+        A = rng.integers(-8, 9, na).astype(np.float64)
+        B = rng.integers(-8, 9, nb).astype(np.float64)
+        Cm = rng.integers(-8, 9, nc).astype(np.float64)
+        arr = np.array(tab, dtype=np.int64).reshape(-1, 11)
+        cols = [_lib.i64(arr[:, i]) for i in range(11)]
+        h = C.c_void_p()
+        _lib.check(L.tkmi_gemm_plan_create(C.byref(h), len(tab), cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                           cols[4][1], _lib.i32(arr[:, 5])[1], cols[6][1], cols[7][1],
+                                           _lib.i32(arr[:, 8])[1], cols[9][1], cols[10][1]), "plan")
+        try:
+            cfg, nt = C.c_int32(), C.c_int64()
+            _lib.check(L.tkmi_gemm_plan_query(h, _lib.PREC_F64, C.byref(cfg), C.byref(nt)), "query")
+            assert cfg.value == want, (shapes, cfg.value)
+            dA, dB, dC = (torch.from_numpy(x).cuda() for x in (A, B, Cm))
+            s = torch.cuda.current_stream().cuda_stream
+            _lib.check(L.tkmi_gemm_exec(h, _lib.PREC_F64, 2.0, -1.0, dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), s),
+                       "exec")
+            np.testing.assert_array_equal(dC.cpu().numpy(), _ref(tab, A, B, Cm, 2.0, -1.0))
+        finally:
+            L.tkmi_gemm_plan_destroy(h)
+
+
 def test_f64_random(lib):
     rng = np.random.default_rng(7)
     shapes = [(int(rng.integers(1, 400)), int(rng.integers(1, 400)), int(rng.integers(1, 300))) for _ in range(12)]
