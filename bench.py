@@ -106,6 +106,56 @@ def gemm_kernel_name(prec, cfg, L):
             2: f"gemm_bf16x3_kernel<{bm}, {bn}>"}[prec]
 
 
+def mfma_calibration(torch, _lib):
+    """Sustained FP64 MFMA rate with no memory traffic (tkmi_calib_mfma_f64): 10 launches
+    per grid size after one warm-up, HIP events around them."""
+    L = _lib.load()
+    out = torch.empty(5120 * 256, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    iters = 2000
+    for blocks in (1024, 1280, 2560, 5120):
+        run = lambda: _lib.check(L.tkmi_calib_mfma_f64(blocks, iters, out.data_ptr(), st.cuda_stream), "calib")
+        ms = time_launches(run, 10, torch)
+        tf = blocks * 4 * iters * 16 * 2048.0 / ms / 1e9
+        log(f"mfma calibration: {blocks} workgroups x 4 waves x {iters} x 16 MFMA: {ms:.3f} ms per launch, "
+            f"{tf:.2f} TFLOP/s, {tf / PEAK_FP64_TF:.3f} of {PEAK_FP64_TF}")
+
+
+def gemm_sweep(torch, _lib):
+    """FP64 grouped GEMM on one uniform 8192×8192×K product (seeded normal operands),
+    tile configurations 13 (64×64), 17 (128×128), 18 (128×64) and 19 (64×128) of the
+    direct-load kernel: 200 launches after 20 warm-ups per configuration (20 after 5 for
+    K ≥ 1024), HIP events around them."""
+    import ctypes as C
+    import numpy as np
+    L = _lib.load()
+    st = torch.cuda.current_stream()
+    m = n = 8192
+    for k in (64, 128, 290, 512, 1024, 2048):
+        tab = np.array([[m, n, k, 0, m, 0, 0, k, 0, 0, m]], dtype=np.int64)
+        cols = [_lib.i64(tab[:, i]) for i in range(11)]
+        h = C.c_void_p()
+        _lib.check(L.tkmi_gemm_plan_create(C.byref(h), 1, cols[0][1], cols[1][1], cols[2][1], cols[3][1],
+                                           cols[4][1], _lib.i32(tab[:, 5])[1], cols[6][1], cols[7][1],
+                                           _lib.i32(tab[:, 8])[1], cols[9][1], cols[10][1]), "plan")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(k)
+        A = torch.randn(m * k, dtype=torch.float64, device="cuda", generator=g)
+        B = torch.randn(k * n, dtype=torch.float64, device="cuda", generator=g)
+        Cm = torch.zeros(m * n, dtype=torch.float64, device="cuda")
+        row = []
+        for cfg in (13, 17, 18, 19):
+            _lib.check(L.tkmi_gemm_plan_set_tile(h, _lib.PREC_F64, 1000 + cfg), "tile")
+            run = lambda: _lib.check(L.tkmi_gemm_exec(h, _lib.PREC_F64, 1.0, 0.0, A.data_ptr(), B.data_ptr(),
+                                                      Cm.data_ptr(), st.cuda_stream), "exec")
+            for _ in range(20 if k < 1024 else 5):
+                run()
+            ms = time_launches(run, 200 if k < 1024 else 20, torch)
+            row.append(f"cfg {cfg}: {ms * 1e3:8.1f} us {2.0 * m * n * k / ms / 1e9:6.2f} TFLOP/s")
+        log(f"gemm sweep 8192x8192x{k}: " + " | ".join(row))
+        L.tkmi_gemm_plan_destroy(h)
+
+
 def time_launches(fn, reps, torch):
     """Average duration (ms) of `fn()` launches on the current stream, HIP events."""
     st = torch.cuda.current_stream()
@@ -358,6 +408,10 @@ def main():
     ap.add_argument("--gemm-configs", default="", metavar="C0,C1,…",
                     help="counter workload: FP64 tile configuration per grouped GEMM (as the "
                          "timed run logs them) instead of autotuning under the profiler")
+    # calibration modes (a table on stderr, no JSON line): the sustained FP64 MFMA rate
+    # with no memory traffic, and the direct-load GEMM on uniform 8192×8192×K products
+    ap.add_argument("--mfma-calibration", action="store_true")
+    ap.add_argument("--gemm-sweep", action="store_true")
     ap.add_argument("--counter-form", choices=("f64", "bf16x3"), default="f64",
                     help="counter workload: the FP64 chain, or the float32 chain with 3×BF16 "
                          "GEMMs (environments split once, as in the timed graph)")
@@ -383,6 +437,12 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"calibration copy: 3 dispatches of {8 * n} bytes read + {8 * n} bytes written")
+        return
+    if args.mfma_calibration:
+        mfma_calibration(torch, _lib)
+        return
+    if args.gemm_sweep:
+        gemm_sweep(torch, _lib)
         return
     V, P, W = two_site_spaces(D=args.D, qmax=8, mode="round")
     log(f"V dim {V.dim()}, x dim {TwoSiteHeff.__name__}")

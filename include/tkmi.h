@@ -208,8 +208,7 @@ int tkmi_gemm_plan_set_tile(tkmi_gemm_plan plan, int32_t precision, int32_t tile
  * Configurations 17, 18 and 19 are the direct-load kernel on 128×128, 128×64 and 64×128
  * tiles (gemm_k4dw_kernel; model split factor; FP64 with untransposed operands, else the
  * one-buffer kernel of the same tiles).  Untuned FP64 plans of untransposed products
- * whose 128×64 tiles fill at least four rounds of the GPU's workgroup slots run 18, other
- * untransposed FP64 plans 13, the rest 5.                                              */
+ * run 13, the rest 5.                                                                 */
 int tkmi_gemm_config_info(int32_t config, int32_t *bm, int32_t *bn, int32_t *bk, int32_t *nbuf);
 /* Same as tkmi_gemm_exec restricted to a subset of the plan's launches:
  * parts bit 0 = MFMA tiles (groups with min(m,n) > 32 or k > 64) with their split-K
@@ -279,6 +278,11 @@ int tkmi_gemm_plan_invalidate(tkmi_gemm_plan plan);
  * clears it.  No counterpart in the reference: a measurement hook for timing a launch
  * where it runs, between the other launches of a chain; not for graph capture.      */
 int tkmi_gemm_plan_time_next(tkmi_gemm_plan plan, void *start_event, void *stop_event);
+/* Calibration of the FP64 matrix-core rate (no counterpart in the reference: it serves
+ * the roofline of the grouped GEMM): `blocks` workgroups of 4 waves, each wave `iters`
+ * rounds of 16 v_mfma_f64_16x16x4_f64 (2 048 flops each) with no memory traffic;
+ * out holds blocks × 256 doubles.  flops = blocks · 4 · iters · 16 · 2048.           */
+int tkmi_calib_mfma_f64(int64_t blocks, int32_t iters, double *out, void *stream);
 /* Configuration index and tile count the next exec with `precision` uses.      */
 int tkmi_gemm_plan_query(tkmi_gemm_plan plan, int32_t precision, int32_t *config, int64_t *ntiles);
 int tkmi_gemm_plan_destroy(tkmi_gemm_plan plan);

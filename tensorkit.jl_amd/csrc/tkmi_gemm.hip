@@ -826,6 +826,23 @@ __global__ __launch_bounds__(NT, BM * BN > 8192 ? 2 : (BM * BN > 4096 ? 3 : 4)) 
     }
 }
 
+// Calibration of the FP64 matrix-core rate: `iters` rounds of 16 v_mfma_f64_16x16x4_f64
+// per wave on four independent accumulators, no memory traffic but one store per lane
+// at the end (out[blockIdx·NT + thread], which keeps the work live).
+__global__ __launch_bounds__(NT) void mfma_f64_calib_kernel(double *__restrict__ out, int iters, double x) {
+    d4_t acc[4] = {};
+    const double a = x + threadIdx.x, b = x - threadIdx.x;
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+    double t = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+    out[(int64_t)blockIdx.x * NT + threadIdx.x] = t;
+}
+
 // --------------------------------------------------------------------------------
 // FP32 kernel with direct global → LDS loads (FP32 data, A and B untransposed): the
 // FP32 counterpart of gemm_k4d_kernel.  A 16-byte chunk holds 4 floats, so a wave takes a
@@ -2398,14 +2415,12 @@ int build_ztiles(tkmi_gemm_plan_s *p) {
 }
 
 int auto_cfg(const tkmi_gemm_plan_s *p, int prec) {
-    // FP64 with untransposed operands: the direct-load kernel with the model's split factor
-    // (8–10% faster than the one-buffer kernel on the sector GEMMs measured), on 128×64
-    // tiles when they fill at least four rounds of its workgroup slots (4–8% faster than
-    // 64×64 on large products with K ≤ 1024; on launches of one or two rounds, such as
-    // the H_eff sectors, the 64×64 tiles' shorter tail wins).  Otherwise 64×64×16 with one
-    // LDS buffer: the fastest on the sector GEMMs measured (K or N of a few hundred).  The
-    // autotuner moves a plan elsewhere when another is > 3% faster.
-    if (prec == TKMI_PREC_F64 && p->ts[18].direct_ok && p->ts[18].ntiles >= 4 * cfg_slots(18)) return 18;
+    // FP64 with untransposed operands: the direct-load kernel on 64×64 tiles with the
+    // model's split factor (8–10% faster than the one-buffer kernel on the sector GEMMs
+    // measured, and 2–7% faster than its 128-wide tiles on uniform 8192×8192×K products
+    // for K from 128 to 2048).  Otherwise 64×64×16 with one LDS buffer: the fastest on the
+    // sector GEMMs measured (K or N of a few hundred).  The autotuner moves a plan
+    // elsewhere when another is > 3% faster.
     return (prec == TKMI_PREC_F64 && p->ts[13].direct_ok) ? 13 : 5;
 }
 
@@ -2997,6 +3012,15 @@ int tkmi_gemm_plan_time_next(tkmi_gemm_plan plan, void *start_event, void *stop_
     if (!start_event != !stop_event) TKMI_FAIL(TKMI_ERR_ARGUMENT, "gemm: time_next needs both events or neither");
     plan->time_ev[0] = (hipEvent_t)start_event;
     plan->time_ev[1] = (hipEvent_t)stop_event;
+    return TKMI_OK;
+}
+
+int tkmi_calib_mfma_f64(int64_t blocks, int32_t iters, double *out, void *stream) {
+    if (blocks < 1 || blocks > ((int64_t)1 << 20) || iters < 1 || !out)
+        TKMI_FAIL(TKMI_ERR_ARGUMENT, "calib: blocks in [1, 2^20], iters >= 1 and an output buffer");
+    hipLaunchKernelGGL(mfma_f64_calib_kernel, dim3((unsigned)blocks), dim3(NT), 0, (hipStream_t)stream, out, iters,
+                       1.0);
+    TKMI_HIP(hipGetLastError());
     return TKMI_OK;
 }
 

@@ -61,6 +61,7 @@ SIGNATURES = [
     ("tkmi_gemm_plan_splitk", C.c_int, [c_vp, C.c_int32, c_i64p, c_i64p]),
     ("tkmi_gemm_plan_query", C.c_int, [c_vp, C.c_int32, c_i32p, c_i64p]),
     ("tkmi_gemm_plan_time_next", C.c_int, [c_vp, c_vp, c_vp]),
+    ("tkmi_calib_mfma_f64", C.c_int, [C.c_int64, C.c_int32, c_vp, c_vp]),
     ("tkmi_gemm_plan_autotune", C.c_int, [c_vp, C.c_int32, c_vp, c_vp, c_vp, C.c_int32, c_vp]),
     ("tkmi_gemm_plan_autotune_dtype", C.c_int, [c_vp, C.c_int32, C.c_int32, c_vp, c_vp, c_vp, C.c_int32, c_vp]),
     ("tkmi_gemm_plan_set_constant", C.c_int, [c_vp, C.c_int32]),

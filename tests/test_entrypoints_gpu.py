@@ -207,3 +207,26 @@ def test_overlapping_vectors_are_refused(T):
     np.testing.assert_array_equal(A.numpy(), 0.5 * a0 + 0.5 * a0)
     T.permute_(A, A, (1, 0), (), 0.0, 2.0)              # α = 0: nothing read
     np.testing.assert_array_equal(A.numpy(), 2.0 * a0)
+
+
+def test_calib_mfma_f64():
+    """tkmi_calib_mfma_f64: the per-wave sum of its outputs is the closed form
+    16·iters·Σ_k (Σ_i a_{i+16k})(Σ_j b_{j+16k}) over the wave's lanes l (a = 1 + t,
+    b = 1 − t, t the thread index): four accumulators, four MFMAs each per round, and
+    every MFMA adds A·B (the sum over the output fragment does not depend on its lane
+    layout).  Bad arguments are rejected."""
+    from tkmi import _lib
+    L = _lib.load()
+    blocks, iters = 3, 5
+    out = torch.zeros(blocks * 256, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.check(L.tkmi_calib_mfma_f64(blocks, iters, out.data_ptr(), s), "calib")
+    got = out.cpu().numpy().reshape(blocks, 4, 64).sum(axis=2)
+    for w in range(4):
+        t = 64 * w + np.arange(64)
+        a, b = 1.0 + t, 1.0 - t
+        want = 16 * iters * sum(a[16 * k:16 * k + 16].sum() * b[16 * k:16 * k + 16].sum() for k in range(4))
+        np.testing.assert_array_equal(got[:, w], np.full(blocks, want))
+    assert L.tkmi_calib_mfma_f64(0, iters, out.data_ptr(), s) != 0
+    assert L.tkmi_calib_mfma_f64(blocks, 0, out.data_ptr(), s) != 0
+    assert L.tkmi_calib_mfma_f64(blocks, iters, None, s) != 0

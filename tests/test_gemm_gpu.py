@@ -141,13 +141,13 @@ def test_direct_load_kernel_exact(lib, tile):
 
 
 def test_default_config_large_products(lib):
-    """Without autotuning, an FP64 plan of untransposed products whose 128×64 tiles fill at
-    least four rounds of the workgroup slots runs the direct-load kernel on 128×64 tiles
-    (configuration 18), a plan of short launches on 64×64 (13); both exact on integer data."""
+    """Without autotuning, FP64 plans of untransposed products run the direct-load kernel on
+    64×64 tiles (configuration 13), large uniform products and short ragged ones alike;
+    exact on integer data."""
     _lib = lib
     L = lib.load()
     rng = np.random.default_rng(61)
-    for shapes, want in (([(2048, 2048, 40)] * 8, 18), ([(300, 280, 290), (130, 70, 33)], 13)):
+    for shapes, want in (([(2048, 2048, 40)] * 8, 13), ([(300, 280, 290), (130, 70, 33)], 13)):
         tab, na, nb, nc = _make_groups(rng, shapes, [0] * len(shapes), [0] * len(shapes))
         A = rng.integers(-8, 9, na).astype(np.float64)
         B = rng.integers(-8, 9, nb).astype(np.float64)
