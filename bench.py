@@ -106,6 +106,16 @@ def gemm_kernel_name(prec, cfg, L):
             2: f"gemm_bf16x3_kernel<{bm}, {bn}>"}[prec]
 
 
+def mfma_ceiling_tf(torch, _lib, blocks=2560, iters=2000, reps=5):
+    """Sustained FP64 MFMA TFLOP/s of tkmi_calib_mfma_f64 (no memory traffic)."""
+    L = _lib.load()
+    out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    run = lambda: _lib.check(L.tkmi_calib_mfma_f64(blocks, iters, out.data_ptr(), st.cuda_stream), "calib")
+    ms = time_launches(run, reps, torch)
+    return blocks * 4 * iters * 16 * 2048.0 / ms / 1e9
+
+
 def mfma_calibration(torch, _lib):
     """Sustained FP64 MFMA rate with no memory traffic (tkmi_calib_mfma_f64): 10 launches
     per grid size after one warm-up, HIP events around them."""
@@ -678,6 +688,11 @@ def main():
         torch.cuda.empty_cache()
 
 
+    # the FP64 rate the matrix cores sustain on this part with no memory traffic at all
+    # (tkmi_calib_mfma_f64: 2 560 workgroups, about 10 ms per launch), after every timed
+    # phase
+    calib_tf = mfma_ceiling_tf(torch, _lib)
+    log(f"FP64 MFMA calibration: {calib_tf:.2f} TFLOP/s sustained with no memory traffic")
     out = {
         "metric": METRIC,
         "value": round(flops / ms64 / 1e6, 2),
@@ -711,7 +726,9 @@ def main():
                      "launch_grids": [g["grid_x"] for g in dom["launches"]], "calls_per_step": dom["calls"],
                      "avg_launch_us": round(dom["ms"] / dom["calls"] * 1e3, 2),
                      "splitk_reduce_us": [round(g["reduce_ms"] * 1e3, 2) if g.get("reduce_ms") else None
-                                          for g in dom["launches"]]},
+                                          for g in dom["launches"]],
+                     "mfma_calibrated_tf": round(calib_tf, 2),
+                     "frac_of_calibrated": round(achieved_tf / calib_tf, 4)},
         "graph_vs_eager_maxabs": eager_err,
         "svd_trunc": svd_report,
         "permute_su2_full": su2_report,
